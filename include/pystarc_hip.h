@@ -1,0 +1,149 @@
+/*
+ * pystarc_hip.h - C ABI of libpystarc_hip.so, the MI355X (gfx950) kernels
+ * behind the pystarc batch Brownian-dynamics force operator and step.
+ *
+ * Plain C types only: device buffers are raw pointers to device memory
+ * (e.g. torch.Tensor.data_ptr() of a CUDA/HIP tensor), host buffers are raw
+ * host pointers, sizes are ints.  Every entry point returns 0 on success and a
+ * non-zero code on failure; pst_last_error() then holds the message.  One
+ * context belongs to one host thread; all work of a context is enqueued on
+ * its stream.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the PySTARC source tree).
+ */
+#ifndef PYSTARC_HIP_H
+#define PYSTARC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pst_ctx pst_ctx;
+typedef struct pst_sim pst_sim;
+
+/* Grid kinds for pst_add_grid. */
+#define PST_GRID_REC_ELEC 0 /* receptor potential grids  (receptor[0-9].dx)      */
+#define PST_GRID_REC_BORN 1 /* receptor Born grids       (receptor[0-9]_born.dx) */
+#define PST_GRID_LIG_BORN 2 /* ligand Born grids         (ligand[0-9]_born.dx)   */
+
+/* Data types. */
+#define PST_F32 0
+#define PST_F64 1
+
+/* ---- context -------------------------------------------------------------
+ * Replaces the CuPy device set-up of GPUBatchForceEngine.__init__
+ * (pystarc/forces/gpu_batch_engine.py:155-160).  With use_stream != 0 all
+ * work goes to `stream` (NULL = the legacy default stream, which orders the
+ * library with a caller such as torch); otherwise the library creates its own
+ * non-blocking stream at default priority.  */
+int pst_ctx_create(pst_ctx** out, int device, void* stream, int use_stream);
+int pst_ctx_destroy(pst_ctx* ctx);
+const char* pst_last_error(const pst_ctx* ctx);
+int pst_abi_version(void);
+/* sizeof of the structs a binding mirrors: 0 = StepParams, 1 = RunCounters
+ * (no reference counterpart; lets a binding verify its struct layout). */
+int pst_struct_size(int which);
+
+/* ---- system set-up -------------------------------------------------------
+ * Replaces GPUBatchForceEngine.__init__ / _upload_grids
+ * (gpu_batch_engine.py:139-246, :248-287).  Grids are C-order (x slowest),
+ * copied from host memory once and kept resident.  The caller passes the grids
+ * already selected as the constructor selects them (finest first, :186-211). */
+int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int nx, int ny, int nz,
+                 const double* origin3, const double* spacing3);
+int pst_clear_grids(pst_ctx* ctx);
+/* Yukawa far-field (gpu_batch_engine.py:161-175): dipole3 / quad9 may be NULL
+ * (no multipole expansion, monopole only). */
+int pst_set_farfield(pst_ctx* ctx, double receptor_charge, double debye_length, double sdie,
+                     const double* dipole3, const double* quad9);
+/* Born desolvation alpha (gpu_batch_engine.py:143). */
+int pst_set_alpha(pst_ctx* ctx, double alpha);
+/* Receptor atoms for reverse Born and overlap (gpu_batch_engine.py:207-217,
+ * gpu_batch_simulator.py:775-777); host arrays (n,3) and (n,). */
+int pst_set_receptor(pst_ctx* ctx, int n, const double* pos, const double* charges);
+/* Ligand body-frame coordinates about the centroid and charges
+ * (gpu_batch_simulator.py:171-176); host arrays (m,3) and (m,). */
+int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charges);
+/* 0 = reference torque (zero, gpu_batch_engine.py:597-599);
+ * 1 = lever arm about the centroid of charged atoms (pystarc/forces/engine.py:144-183). */
+int pst_set_torque_mode(pst_ctx* ctx, int mode);
+/* Lanes per trajectory for the force kernel (1..64, power of two; 0 = automatic). */
+int pst_set_force_group(pst_ctx* ctx, int lanes);
+
+/* ---- force operator ------------------------------------------------------
+ * Replaces GPUBatchForceEngine.__call__(lig_positions, lig_charges,
+ * R_matrices, centroids) (gpu_batch_engine.py:531-605).  All pointers are
+ * device pointers.  lig_pos (n,m,3) f64, lig_q (m,) f64, rot (n,3,3) and
+ * cen (n,3) f64 (both NULL disables reverse Born, as in the reference);
+ * outputs F (n,3), T (n,3), E (n,) f64.  precision 0 = FP64 gather and sums,
+ * 1 = FP32 gather with FP64 cross-lane reduction.  rev_skip 1 reproduces the
+ * reference's 500 MB chunk skip (:626-639), 0 culls per trajectory exactly. */
+int pst_forces_positions(pst_ctx* ctx, int n, int m, const double* lig_pos, const double* lig_q,
+                         const double* rot, const double* cen, double* F, double* T, double* E,
+                         int precision, int rev_skip);
+/* Fused variant: ligand atoms placed in registers from the pose (centroid
+ * pos (n,3), quaternion quat (n,4) w,x,y,z) as gpu_batch_simulator.py:935-943
+ * does, without materialising (n,m,3). */
+int pst_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, double* F,
+                    double* T, double* E, int precision);
+/* Times the fused force kernel: `reps` launches on the context stream,
+ * bracketed by HIP events; writes the mean launch time in ms and, when
+ * census4 is non-NULL, {trajectories, elec-grid atoms, born-grid atoms,
+ * reverse-grid atoms} for one launch. */
+int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, double* F,
+                         double* T, double* E, int precision, int reps, double* ms_per_launch,
+                         double* census4);
+
+/* ---- simulator -----------------------------------------------------------
+ * Replaces GPUBatchSimulator (pystarc/simulation/gpu_batch_simulator.py:143-1751).
+ * `params` points to a pst_step_params struct whose layout is StepParams
+ * in pystarc_amd/csrc/pst_types.hpp (mirrored by the ctypes wrapper). */
+int pst_sim_create(pst_ctx* ctx, const void* params, int precision, pst_sim** out);
+int pst_sim_destroy(pst_sim* sim);
+/* Host state in / out, (n,3) positions, (n,4) quaternions. */
+int pst_sim_set_state(pst_sim* sim, int n, const double* pos, const double* quat);
+int pst_sim_get_state(pst_sim* sim, double* pos, double* quat, int* status, long long* n_steps);
+
+/* Replay mode: one reference loop iteration (gpu_batch_simulator.py:783-1254)
+ * in four phases, the host supplying NumPy draws in the reference's order.
+ * begin:  reaction check + escape candidates (:855-884).
+ * escape: u_outer (n_esc) and u_quat (n_ret,3) host arrays (:884-911).
+ * move:   forces, dt, D, noise_t / noise_r (n_sr,3) host arrays (:931-1128).
+ * bridge: u_bb (n_sr, n_pairs) host array or NULL (:1142-1254).        */
+int pst_replay_begin(pst_sim* sim, int* n_run, int* n_esc, int* n_react);
+int pst_replay_escape(pst_sim* sim, const double* u_outer, int n_esc, const double* u_quat,
+                      int n_ret, int* n_sr);
+int pst_replay_move(pst_sim* sim, const double* noise_t, const double* noise_r, int n_sr,
+                    int* any_above, int* n_overlap);
+int pst_replay_bridge(pst_sim* sim, const double* u_bb, int n_sr, int* n_bridge);
+/* Forces of the last move phase for the ordered still-running list (n_sr,3). */
+int pst_replay_forces(pst_sim* sim, double* F_host, int n_sr);
+
+/* Production mode: Philox noise on the device, a fixed pool of slots refilled
+ * from the trajectory counter, no host synchronisation inside a launch.
+ * record != 0 keeps per-trajectory outcome / step count (n_total each). */
+int pst_prod_init(pst_sim* sim, int record);
+/* Runs up to max_steps device steps, `graph_steps` per captured hipGraph
+ * launch (0 = plain launches); stops early at the reference's 99.5 % rule
+ * (gpu_batch_simulator.py:1347-1348) or when every slot is idle.
+ * counters16 receives {n_reacted, n_escaped, n_maxsteps, total_steps,
+ * next_traj, n_returns, n_overlap, n_bridge, steps_run, stop, n_sr, n_mlp,
+ * 0, 0, 0, 0} as doubles. */
+int pst_prod_run(pst_sim* sim, long long max_steps, int graph_steps, double* counters16);
+int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
+/* Mean device time per step (ms) of the last pst_prod_run, HIP events. */
+int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);
+
+/* ---- synthetic grids -----------------------------------------------------
+ * Fills a host array (nx,ny,nz) C-order with a Debye-Hueckel potential
+ * (kind 0) or an analytic Born desolvation field (kind 1) of `atoms`
+ * (n,5: x, y, z, charge, radius), evaluated on the device. dtype PST_F32/PST_F64. */
+int pst_make_grid(pst_ctx* ctx, int kind, int nx, int ny, int nz, const double* origin3,
+                  const double* spacing3, const double* atoms, int n_atoms, double debye_length,
+                  double eps_out, double eps_in, int dtype, void* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYSTARC_HIP_H */

@@ -1,0 +1,905 @@
+// C ABI of libpystarc_hip.so (declared in include/pystarc_hip.h): contexts
+// owning resident grids, ligand / receptor tables and simulator state, and
+// the entry points that enqueue the gfx950 kernels on the context stream.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pystarc_hip.h"
+#include "pst_kernels.hpp"
+
+using namespace pst;
+
+namespace {
+
+struct DevGrid {
+  GridDev g;
+  void* mem = nullptr;
+};
+
+}  // namespace
+
+struct pst_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  std::vector<DevGrid> elec, born, lig_born;
+  double receptor_charge = 0, debye = 7.858, sdie = 78.0;
+  bool has_mp = false;
+  double dip[3] = {0, 0, 0}, quad[9] = {0};
+  double alpha = 0.07957747;
+  int M = 0;
+  double* d_body = nullptr;
+  double* d_q = nullptr;
+  double cq_body[3] = {0, 0, 0};
+  int n_rec = 0;
+  double* d_rec = nullptr;
+  double* d_rec_q = nullptr;
+  double rec_max_r = 0;
+  int torque_mode = 0;
+  int force_group = 0;
+  // scratch for the reference chunk skip
+  double* d_chunk = nullptr;
+  int chunk_cap = 0;
+};
+
+struct pst_sim {
+  pst_ctx* ctx = nullptr;
+  StepParams P;
+  int precision = 0;
+  int S = 0;
+  // shared state
+  double* pos = nullptr;
+  double* quat = nullptr;
+  int* status = nullptr;
+  double* F = nullptr;
+  double* T = nullptr;
+  double* dt = nullptr;
+  double* x0 = nullptr;
+  double* x1 = nullptr;
+  // replay
+  long long* n_steps64 = nullptr;
+  unsigned char* flags = nullptr;  // 4 * S
+  int* lists = nullptr;            // 4 * S
+  int* counts = nullptr;           // 8
+  double* h_buf = nullptr;         // device staging for host noise
+  size_t h_buf_cap = 0;
+  // production
+  unsigned int* n_steps32 = nullptr;
+  long long* traj = nullptr;
+  int* plain_list = nullptr;
+  int* mlp_list = nullptr;
+  RunCounters* ctr = nullptr;
+  signed char* out_status = nullptr;
+  int* out_steps = nullptr;
+  double last_ms_per_step = 0;
+  bool prod_ready = false;
+};
+
+#define CK(expr)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+      return 2;                                                                       \
+    }                                                                                 \
+  } while (0)
+
+#define FAIL(msg)          \
+  do {                     \
+    ctx->err = (msg);      \
+    return 1;              \
+  } while (0)
+
+static hipError_t set_dev(pst_ctx* ctx) { return hipSetDevice(ctx->device); }
+
+static GridDev grid_record(const DevGrid& d) { return d.g; }
+
+static void fill_sys(pst_ctx* ctx, ForceSys& S) {
+  memset(&S, 0, sizeof(S));
+  S.n_elec = (int)ctx->elec.size();
+  for (int i = 0; i < S.n_elec; ++i) S.elec[i] = grid_record(ctx->elec[i]);
+  S.n_born = (int)ctx->born.size();
+  for (int i = 0; i < S.n_born; ++i) S.born[i] = grid_record(ctx->born[i]);
+  S.has_rev = (!ctx->lig_born.empty() && ctx->n_rec > 0) ? 1 : 0;
+  if (S.has_rev) {
+    S.lig_born = ctx->lig_born[0].g;
+    // skip radius of the reference (:620-623) and exact cull radius
+    S.rev_radius = fmax(fabs(S.lig_born.lo[0]), fabs(S.lig_born.hi[0]));
+    double m = 0;
+    for (int c = 0; c < 8; ++c) {
+      double x = (c & 1) ? S.lig_born.hi[0] : S.lig_born.lo[0];
+      double y = (c & 2) ? S.lig_born.hi[1] : S.lig_born.lo[1];
+      double z = (c & 4) ? S.lig_born.hi[2] : S.lig_born.lo[2];
+      m = fmax(m, sqrt(x * x + y * y + z * z));
+    }
+    S.rev_cull = m * (1.0 + 1e-12) + 1e-9;
+  }
+  S.alpha = ctx->alpha;
+  const double eps_s = ctx->sdie * 0.000142;
+  S.V_fac = eps_s > 0 ? ctx->receptor_charge / (4.0 * M_PI * eps_s) : 0.0;
+  S.has_yukawa = (fabs(ctx->receptor_charge) > 1e-9 && ctx->debye > 0) ? 1 : 0;
+  S.debye = ctx->debye;
+  S.fpe = 4.0 * M_PI * eps_s;
+  if (ctx->has_mp) {
+    double pm = sqrt(ctx->dip[0] * ctx->dip[0] + ctx->dip[1] * ctx->dip[1] + ctx->dip[2] * ctx->dip[2]);
+    double qm = 0;
+    for (int i = 0; i < 9; ++i) qm += ctx->quad[i] * ctx->quad[i];
+    qm = sqrt(qm);
+    S.has_dip = pm > 1e-9;
+    S.has_quad = qm > 1e-9;
+    for (int i = 0; i < 3; ++i) S.dip[i] = ctx->dip[i];
+    for (int i = 0; i < 9; ++i) S.quad[i] = ctx->quad[i];
+  }
+  S.M = ctx->M;
+  S.lig_body = ctx->d_body;
+  S.lig_q = ctx->d_q;
+  for (int i = 0; i < 3; ++i) S.cq_body[i] = ctx->cq_body[i];
+  S.n_rec = ctx->n_rec;
+  S.rec_pos = ctx->d_rec;
+  S.rec_q = ctx->d_rec_q;
+  S.rec_max_r = ctx->rec_max_r;
+  S.torque_mode = ctx->torque_mode;
+}
+
+static int group_for(pst_ctx* ctx, const ForceSys& S) {
+  if (ctx->force_group > 0) return ctx->force_group;
+  return pick_group(S.M, S.has_rev ? S.n_rec : 0);
+}
+
+extern "C" {
+
+int pst_abi_version(void) { return 1; }
+
+int pst_struct_size(int which) {
+  if (which == 0) return (int)sizeof(StepParams);
+  if (which == 1) return (int)sizeof(RunCounters);
+  return -1;
+}
+
+int pst_ctx_create(pst_ctx** out, int device, void* stream, int use_stream) {
+  if (!out) return 1;
+  pst_ctx* ctx = new pst_ctx();
+  ctx->device = device;
+  *out = ctx;
+  CK(set_dev(ctx));
+  if (use_stream) {
+    ctx->stream = (hipStream_t)stream;
+  } else {
+    CK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  return 0;
+}
+
+static void free_grids(std::vector<DevGrid>& v) {
+  for (auto& g : v)
+    if (g.mem) hipFree(g.mem);
+  v.clear();
+}
+
+int pst_ctx_destroy(pst_ctx* ctx) {
+  if (!ctx) return 0;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_grids(ctx->elec);
+  free_grids(ctx->born);
+  free_grids(ctx->lig_born);
+  if (ctx->d_body) hipFree(ctx->d_body);
+  if (ctx->d_q) hipFree(ctx->d_q);
+  if (ctx->d_rec) hipFree(ctx->d_rec);
+  if (ctx->d_rec_q) hipFree(ctx->d_rec_q);
+  if (ctx->d_chunk) hipFree(ctx->d_chunk);
+  if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+const char* pst_last_error(const pst_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int nx, int ny, int nz,
+                 const double* origin3, const double* spacing3) {
+  if (!ctx) return 1;
+  if (kind < 0 || kind > 2) FAIL("pst_add_grid: unknown grid kind");
+  if (dtype != PST_F32 && dtype != PST_F64) FAIL("pst_add_grid: dtype must be PST_F32 or PST_F64");
+  if (nx < 2 || ny < 2 || nz < 2) FAIL("pst_add_grid: every grid dimension must be >= 2");
+  if (!host_data || !origin3 || !spacing3) FAIL("pst_add_grid: null pointer");
+  std::vector<DevGrid>& dst = kind == 0 ? ctx->elec : (kind == 1 ? ctx->born : ctx->lig_born);
+  if ((int)dst.size() >= PST_MAX_GRIDS) FAIL("pst_add_grid: too many grids of this kind");
+  for (int i = 0; i < 3; ++i)
+    if (!(spacing3[i] > 0)) FAIL("pst_add_grid: spacing must be positive");
+  CK(set_dev(ctx));
+  DevGrid d;
+  const size_t count = (size_t)nx * ny * nz;
+  const size_t bytes = count * (dtype == PST_F64 ? 8 : 4);
+  CK(hipMalloc(&d.mem, bytes));
+  hipError_t e = hipMemcpy(d.mem, host_data, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(d.mem);
+    ctx->err = std::string("pst_add_grid copy: ") + hipGetErrorString(e);
+    return 2;
+  }
+  d.g.data = d.mem;
+  d.g.dtype = dtype;
+  d.g.nx = nx; d.g.ny = ny; d.g.nz = nz;
+  const int n[3] = {nx, ny, nz};
+  for (int i = 0; i < 3; ++i) {
+    d.g.o[i] = origin3[i];
+    d.g.sp[i] = spacing3[i];
+    d.g.inv[i] = 1.0 / spacing3[i];
+    d.g.lo[i] = origin3[i] + 3.0 * spacing3[i];
+    d.g.hi[i] = origin3[i] + (n[i] - 1 - 3.0) * spacing3[i];
+  }
+  dst.push_back(d);
+  return 0;
+}
+
+int pst_clear_grids(pst_ctx* ctx) {
+  if (!ctx) return 1;
+  CK(set_dev(ctx));
+  CK(hipStreamSynchronize(ctx->stream));
+  free_grids(ctx->elec);
+  free_grids(ctx->born);
+  free_grids(ctx->lig_born);
+  return 0;
+}
+
+int pst_set_farfield(pst_ctx* ctx, double receptor_charge, double debye_length, double sdie,
+                     const double* dipole3, const double* quad9) {
+  if (!ctx) return 1;
+  ctx->receptor_charge = receptor_charge;
+  ctx->debye = debye_length;
+  ctx->sdie = sdie;
+  ctx->has_mp = dipole3 != nullptr && quad9 != nullptr;
+  for (int i = 0; i < 3; ++i) ctx->dip[i] = dipole3 ? dipole3[i] : 0.0;
+  for (int i = 0; i < 9; ++i) ctx->quad[i] = quad9 ? quad9[i] : 0.0;
+  return 0;
+}
+
+int pst_set_alpha(pst_ctx* ctx, double alpha) {
+  if (!ctx) return 1;
+  ctx->alpha = alpha;
+  return 0;
+}
+
+int pst_set_torque_mode(pst_ctx* ctx, int mode) {
+  if (!ctx) return 1;
+  if (mode != 0 && mode != 1) FAIL("pst_set_torque_mode: mode must be 0 or 1");
+  ctx->torque_mode = mode;
+  return 0;
+}
+
+int pst_set_force_group(pst_ctx* ctx, int lanes) {
+  if (!ctx) return 1;
+  if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 &&
+      lanes != 32 && lanes != 64)
+    FAIL("pst_set_force_group: lanes must be 0 or a power of two <= 64");
+  ctx->force_group = lanes;
+  return 0;
+}
+
+int pst_set_receptor(pst_ctx* ctx, int n, const double* pos, const double* charges) {
+  if (!ctx) return 1;
+  if (n < 0 || (n > 0 && (!pos || !charges))) FAIL("pst_set_receptor: bad arguments");
+  CK(set_dev(ctx));
+  if (ctx->d_rec) { hipFree(ctx->d_rec); ctx->d_rec = nullptr; }
+  if (ctx->d_rec_q) { hipFree(ctx->d_rec_q); ctx->d_rec_q = nullptr; }
+  ctx->n_rec = n;
+  ctx->rec_max_r = 0;
+  if (n == 0) return 0;
+  CK(hipMalloc(&ctx->d_rec, sizeof(double) * 3 * n));
+  CK(hipMalloc(&ctx->d_rec_q, sizeof(double) * n));
+  CK(hipMemcpy(ctx->d_rec, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->d_rec_q, charges, sizeof(double) * n, hipMemcpyHostToDevice));
+  for (int j = 0; j < n; ++j) {
+    const double r = sqrt(pos[3 * j] * pos[3 * j] + pos[3 * j + 1] * pos[3 * j + 1] +
+                          pos[3 * j + 2] * pos[3 * j + 2]);
+    if (r > ctx->rec_max_r) ctx->rec_max_r = r;
+  }
+  return 0;
+}
+
+int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charges) {
+  if (!ctx) return 1;
+  if (m <= 0 || !body || !charges) FAIL("pst_set_ligand: bad arguments");
+  CK(set_dev(ctx));
+  if (ctx->d_body) { hipFree(ctx->d_body); ctx->d_body = nullptr; }
+  if (ctx->d_q) { hipFree(ctx->d_q); ctx->d_q = nullptr; }
+  ctx->M = m;
+  CK(hipMalloc(&ctx->d_body, sizeof(double) * 3 * m));
+  CK(hipMalloc(&ctx->d_q, sizeof(double) * m));
+  CK(hipMemcpy(ctx->d_body, body, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->d_q, charges, sizeof(double) * m, hipMemcpyHostToDevice));
+  double c[3] = {0, 0, 0};
+  int nc = 0;
+  for (int i = 0; i < m; ++i)
+    if (fabs(charges[i]) > 1e-9) {
+      for (int d = 0; d < 3; ++d) c[d] += body[3 * i + d];
+      ++nc;
+    }
+  for (int d = 0; d < 3; ++d) ctx->cq_body[d] = nc ? c[d] / nc : 0.0;
+  return 0;
+}
+
+static int ensure_chunk(pst_ctx* ctx, int n) {
+  if (n <= ctx->chunk_cap) return 0;
+  if (ctx->d_chunk) hipFree(ctx->d_chunk);
+  ctx->d_chunk = nullptr;
+  ctx->chunk_cap = 0;
+  CK(hipMalloc(&ctx->d_chunk, sizeof(double) * n));
+  ctx->chunk_cap = n;
+  return 0;
+}
+
+// Reference chunking of a force batch: 500 MiB / (N_rec * 24) trajectories
+// (gpu_batch_engine.py:627-629) within force batches of `fb` trajectories.
+static void chunk_layout(const pst_ctx* ctx, int n, int fb, int* cs, int* cpf, int* n_chunks) {
+  int c = (int)fmax(1.0, floor(500.0 * 1024 * 1024 / ((double)ctx->n_rec * 3 * 8)));
+  if (c > fb) c = fb;
+  *cs = c;
+  *cpf = (fb + c - 1) / c;
+  const int nfb = (n + fb - 1) / fb;
+  *n_chunks = nfb * (*cpf);
+}
+
+static int forces_common(pst_ctx* ctx, ForceWork& W, int n, int precision, int fb) {
+  ForceSys S;
+  fill_sys(ctx, S);
+  if (W.rev_skip_mode == 1 && S.has_rev && n > 0) {
+    int cs, cpf, nch;
+    chunk_layout(ctx, n, fb, &cs, &cpf, &nch);
+    if (ensure_chunk(ctx, nch)) return 2;
+    CK(launch_chunk_min_r(W.pos, W.list, n, fb, cs, cpf, ctx->d_chunk, nch, ctx->stream));
+    W.chunk_min_r = ctx->d_chunk;
+    W.chunk_fb = fb;
+    W.chunk_cs = cs;
+    W.chunks_per_fb = cpf;
+  }
+  CK(launch_forces(S, W, precision, group_for(ctx, S), n, 2048, ctx->stream));
+  return 0;
+}
+
+int pst_forces_positions(pst_ctx* ctx, int n, int m, const double* lig_pos, const double* lig_q,
+                         const double* rot, const double* cen, double* F, double* T, double* E,
+                         int precision, int rev_skip) {
+  if (!ctx) return 1;
+  if (n < 0 || m <= 0) FAIL("pst_forces_positions: bad sizes");
+  if (n == 0) return 0;
+  if (!lig_pos || !lig_q || !F) FAIL("pst_forces_positions: null pointer");
+  if ((rot == nullptr) != (cen == nullptr)) FAIL("pst_forces_positions: rot and cen go together");
+  if (precision != 0 && precision != 1) FAIL("pst_forces_positions: precision must be 0 or 1");
+  CK(set_dev(ctx));
+  const int saved_M = ctx->M;
+  ForceWork W;
+  memset(&W, 0, sizeof(W));
+  W.n = n;
+  W.pos = cen;
+  W.rot = rot;
+  W.lig_pos = lig_pos;
+  W.lig_q = lig_q;
+  W.F = F; W.T = T; W.E = E;
+  W.rev_skip_mode = rev_skip;
+  ctx->M = m;
+  int rc = forces_common(ctx, W, n, precision, n);
+  ctx->M = saved_M;
+  return rc;
+}
+
+int pst_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, double* F,
+                    double* T, double* E, int precision) {
+  if (!ctx) return 1;
+  if (n < 0) FAIL("pst_forces_pose: bad size");
+  if (n == 0) return 0;
+  if (!pos || !quat || !F) FAIL("pst_forces_pose: null pointer");
+  if (ctx->M <= 0) FAIL("pst_forces_pose: ligand not set");
+  CK(set_dev(ctx));
+  ForceWork W;
+  memset(&W, 0, sizeof(W));
+  W.n = n;
+  W.pos = pos;
+  W.quat = quat;
+  W.F = F; W.T = T; W.E = E;
+  return forces_common(ctx, W, n, precision, n);
+}
+
+int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, double* F,
+                         double* T, double* E, int precision, int reps, double* ms_per_launch,
+                         double* census4) {
+  if (!ctx) return 1;
+  if (n <= 0 || reps <= 0 || !ms_per_launch) FAIL("pst_time_forces_pose: bad arguments");
+  CK(set_dev(ctx));
+  hipEvent_t a, b;
+  CK(hipEventCreate(&a));
+  CK(hipEventCreate(&b));
+  int rc = pst_forces_pose(ctx, n, pos, quat, F, T, E, precision);  // warm-up
+  if (rc) return rc;
+  CK(hipEventRecord(a, ctx->stream));
+  for (int i = 0; i < reps; ++i) {
+    rc = pst_forces_pose(ctx, n, pos, quat, F, T, E, precision);
+    if (rc) return rc;
+  }
+  CK(hipEventRecord(b, ctx->stream));
+  CK(hipEventSynchronize(b));
+  float ms = 0;
+  CK(hipEventElapsedTime(&ms, a, b));
+  *ms_per_launch = ms / reps;
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (census4) {
+    // atom census on the host from the current poses (one launch's work)
+    std::vector<double> hp(3 * (size_t)n), hq(4 * (size_t)n), body(3 * (size_t)ctx->M),
+        lq(ctx->M), rp(3 * (size_t)ctx->n_rec), rq(ctx->n_rec);
+    CK(hipMemcpy(hp.data(), pos, hp.size() * 8, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(hq.data(), quat, hq.size() * 8, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(body.data(), ctx->d_body, body.size() * 8, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(lq.data(), ctx->d_q, lq.size() * 8, hipMemcpyDeviceToHost));
+    if (ctx->n_rec) {
+      CK(hipMemcpy(rp.data(), ctx->d_rec, rp.size() * 8, hipMemcpyDeviceToHost));
+      CK(hipMemcpy(rq.data(), ctx->d_rec_q, rq.size() * 8, hipMemcpyDeviceToHost));
+    }
+    ForceSys S;
+    fill_sys(ctx, S);
+    double ne = 0, nb = 0, nr = 0;
+    auto inside = [](const GridDev& g, double x, double y, double z) {
+      return x > g.lo[0] && y > g.lo[1] && z > g.lo[2] && x < g.hi[0] && y < g.hi[1] && z < g.hi[2];
+    };
+    for (int t = 0; t < n; ++t) {
+      const double* q = &hq[4 * (size_t)t];
+      const double w = q[0], x = q[1], y = q[2], z = q[3];
+      const double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                           2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                           2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)};
+      const double cx = hp[3 * t], cy = hp[3 * t + 1], cz = hp[3 * t + 2];
+      for (int i = 0; i < ctx->M; ++i) {
+        if (fabs(lq[i]) < 1e-9) continue;
+        const double bx = body[3 * i], by = body[3 * i + 1], bz = body[3 * i + 2];
+        const double px = R[0] * bx + R[1] * by + R[2] * bz + cx;
+        const double py = R[3] * bx + R[4] * by + R[5] * bz + cy;
+        const double pz = R[6] * bx + R[7] * by + R[8] * bz + cz;
+        for (int g = 0; g < S.n_elec; ++g)
+          if (inside(S.elec[g], px, py, pz)) { ne += 1; break; }
+        for (int g = 0; g < S.n_born; ++g)
+          if (inside(S.born[g], px, py, pz)) { nb += 1; break; }
+      }
+      if (S.has_rev) {
+        const double rc = sqrt(cx * cx + cy * cy + cz * cz);
+        if (rc - S.rec_max_r > S.rev_cull) continue;
+        for (int j = 0; j < ctx->n_rec; ++j) {
+          if (fabs(rq[j]) < 1e-9) continue;
+          const double dx = rp[3 * j] - cx, dy = rp[3 * j + 1] - cy, dz = rp[3 * j + 2] - cz;
+          const double xx = R[0] * dx + R[3] * dy + R[6] * dz;
+          const double yy = R[1] * dx + R[4] * dy + R[7] * dz;
+          const double zz = R[2] * dx + R[5] * dy + R[8] * dz;
+          if (inside(S.lig_born, xx, yy, zz)) nr += 1;
+        }
+      }
+    }
+    census4[0] = n; census4[1] = ne; census4[2] = nb; census4[3] = nr;
+  }
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
+// simulator
+// ----------------------------------------------------------------------------
+int pst_sim_create(pst_ctx* ctx, const void* params, int precision, pst_sim** out) {
+  if (!ctx) return 1;
+  if (!params || !out) FAIL("pst_sim_create: null pointer");
+  if (precision != 0 && precision != 1) FAIL("pst_sim_create: precision must be 0 or 1");
+  const StepParams* P = (const StepParams*)params;
+  if (P->n_slots <= 0 || P->n_slots > (1ll << 30)) FAIL("pst_sim_create: n_slots out of range");
+  if (P->n_pairs < 0 || P->n_pairs > PST_MAX_PAIRS) FAIL("pst_sim_create: too many reaction pairs");
+  if (ctx->M <= 0) FAIL("pst_sim_create: ligand not set");
+  CK(set_dev(ctx));
+  pst_sim* s = new pst_sim();
+  s->ctx = ctx;
+  s->P = *P;
+  s->precision = precision;
+  s->S = (int)P->n_slots;
+  const size_t S = s->S;
+  *out = s;
+  CK(hipMalloc(&s->pos, S * 3 * 8));
+  CK(hipMalloc(&s->quat, S * 4 * 8));
+  CK(hipMalloc(&s->status, S * 4));
+  CK(hipMalloc(&s->F, S * 3 * 8));
+  CK(hipMalloc(&s->T, S * 3 * 8));
+  CK(hipMalloc(&s->dt, S * 8));
+  CK(hipMalloc(&s->x0, S * PST_MAX_PAIRS * 8));
+  CK(hipMalloc(&s->x1, S * PST_MAX_PAIRS * 8));
+  CK(hipMemsetAsync(s->F, 0, S * 3 * 8, ctx->stream));
+  CK(hipMemsetAsync(s->T, 0, S * 3 * 8, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pst_sim_destroy(pst_sim* s) {
+  if (!s) return 0;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
+                  s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
+                  s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete s;
+  return 0;
+}
+
+static int ensure_replay(pst_sim* s) {
+  pst_ctx* ctx = s->ctx;
+  if (s->flags) return 0;
+  const size_t S = s->S;
+  CK(hipMalloc(&s->n_steps64, S * 8));
+  CK(hipMalloc(&s->flags, S * 4));
+  CK(hipMalloc(&s->lists, S * 4 * 4));
+  CK(hipMalloc(&s->counts, 8 * 4));
+  CK(hipMemsetAsync(s->n_steps64, 0, S * 8, ctx->stream));
+  CK(hipMemsetAsync(s->counts, 0, 8 * 4, ctx->stream));
+  return 0;
+}
+
+static ReplayBufs replay_bufs(pst_sim* s) {
+  ReplayBufs B;
+  const size_t S = s->S;
+  B.pos = s->pos; B.quat = s->quat; B.status = s->status; B.n_steps = s->n_steps64;
+  B.f_run = s->flags; B.f_esc = s->flags + S; B.f_ret = s->flags + 2 * S; B.f_sr = s->flags + 3 * S;
+  B.run_list = s->lists; B.esc_list = s->lists + S; B.ret_list = s->lists + 2 * S;
+  B.sr_list = s->lists + 3 * S;
+  B.counts = s->counts;
+  B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
+  B.old_pos = nullptr; B.old_quat = nullptr;
+  B.n_slots = s->S;
+  return B;
+}
+
+static int stage(pst_sim* s, const double* host, size_t count, double** dev) {
+  pst_ctx* ctx = s->ctx;
+  if (count == 0) { *dev = s->h_buf; return 0; }
+  if (count > s->h_buf_cap) {
+    if (s->h_buf) hipFree(s->h_buf);
+    s->h_buf = nullptr;
+    s->h_buf_cap = 0;
+    CK(hipMalloc(&s->h_buf, count * 8));
+    s->h_buf_cap = count;
+  }
+  CK(hipMemcpyAsync(s->h_buf, host, count * 8, hipMemcpyHostToDevice, ctx->stream));
+  *dev = s->h_buf;
+  return 0;
+}
+
+int pst_sim_set_state(pst_sim* s, int n, const double* pos, const double* quat) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (n != s->S || !pos || !quat) FAIL("pst_sim_set_state: n must equal n_slots");
+  CK(set_dev(ctx));
+  if (ensure_replay(s)) return 2;
+  CK(hipMemcpyAsync(s->pos, pos, (size_t)n * 24, hipMemcpyHostToDevice, ctx->stream));
+  CK(hipMemcpyAsync(s->quat, quat, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream));
+  CK(hipMemsetAsync(s->status, 0, (size_t)n * 4, ctx->stream));
+  CK(hipMemsetAsync(s->n_steps64, 0, (size_t)n * 8, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pst_sim_get_state(pst_sim* s, double* pos, double* quat, int* status, long long* n_steps) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  CK(hipStreamSynchronize(ctx->stream));
+  const size_t S = s->S;
+  if (pos) CK(hipMemcpy(pos, s->pos, S * 24, hipMemcpyDeviceToHost));
+  if (quat) CK(hipMemcpy(quat, s->quat, S * 32, hipMemcpyDeviceToHost));
+  if (status) CK(hipMemcpy(status, s->status, S * 4, hipMemcpyDeviceToHost));
+  if (n_steps) {
+    if (s->n_steps64) {
+      CK(hipMemcpy(n_steps, s->n_steps64, S * 8, hipMemcpyDeviceToHost));
+    } else if (s->n_steps32) {
+      std::vector<unsigned int> tmp(S);
+      CK(hipMemcpy(tmp.data(), s->n_steps32, S * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < S; ++i) n_steps[i] = tmp[i];
+    }
+  }
+  return 0;
+}
+
+int pst_replay_begin(pst_sim* s, int* n_run, int* n_esc, int* n_react) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  if (ensure_replay(s)) return 2;
+  ReplayBufs B = replay_bufs(s);
+  CK(replay_begin(s->P, B, ctx->stream));
+  int h[8];
+  CK(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  if (n_run) *n_run = h[0];
+  if (n_esc) *n_esc = h[1];
+  if (n_react) *n_react = h[7];
+  return 0;
+}
+
+int pst_replay_escape(pst_sim* s, const double* u_outer, int n_esc, const double* u_quat, int n_ret,
+                      int* n_sr) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  ReplayBufs B = replay_bufs(s);
+  int h[8];
+  CK(hipMemcpy(h, s->counts, sizeof(h), hipMemcpyDeviceToHost));
+  if (h[1] != n_esc) FAIL("pst_replay_escape: n_esc does not match the device escape list");
+  if (n_esc > 0) {
+    if (!u_outer) FAIL("pst_replay_escape: u_outer is null");
+    double* du;
+    if (stage(s, u_outer, (size_t)n_esc, &du)) return 2;
+    CK(replay_escape_decide(s->P, B, du, ctx->stream));
+    CK(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (h[2] != n_ret) FAIL("pst_replay_escape: n_ret does not match u_outer < return_prob");
+    if (n_ret > 0) {
+      if (!u_quat) FAIL("pst_replay_escape: u_quat is null");
+      double* dq;
+      if (stage(s, u_quat, (size_t)n_ret * 3, &dq)) return 2;
+      CK(replay_escape_apply(s->P, B, dq, ctx->stream));
+    }
+  } else if (n_ret != 0) {
+    FAIL("pst_replay_escape: n_ret must be 0 when n_esc is 0");
+  } else {
+    CK(hipMemsetAsync(s->counts + 2, 0, 4, ctx->stream));
+  }
+  CK(replay_build_sr(B, ctx->stream));
+  CK(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  if (n_sr) *n_sr = h[3];
+  return 0;
+}
+
+int pst_replay_move(pst_sim* s, const double* noise_t, const double* noise_r, int n_sr,
+                    int* any_above, int* n_overlap) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  ReplayBufs B = replay_bufs(s);
+  int h[8];
+  CK(hipMemcpy(h, s->counts, sizeof(h), hipMemcpyDeviceToHost));
+  if (h[3] != n_sr) FAIL("pst_replay_move: n_sr does not match the device list");
+  if (n_sr > 0) {
+    if (!noise_t || !noise_r) FAIL("pst_replay_move: null noise");
+    // forces on the ordered still-running list, reference batching
+    ForceWork W;
+    memset(&W, 0, sizeof(W));
+    W.list = B.sr_list;
+    W.n = n_sr;
+    W.pos = s->pos;
+    W.quat = s->quat;
+    W.F = s->F; W.T = s->T; W.E = nullptr;
+    W.rev_skip_mode = 1;
+    const int M = ctx->M;
+    int fb = (int)fmax(1000.0, floor(4.0 * 1024 * 1024 * 1024 / fmax(1.0, (double)M * 150)));
+    if (fb > n_sr) fb = n_sr;
+    int rc = forces_common(ctx, W, n_sr, s->precision, fb);
+    if (rc) return rc;
+    // host noise staged after the forces, in one buffer
+    std::vector<double> buf((size_t)n_sr * 6);
+    memcpy(buf.data(), noise_t, (size_t)n_sr * 24);
+    memcpy(buf.data() + (size_t)n_sr * 3, noise_r, (size_t)n_sr * 24);
+    double* dn;
+    if (stage(s, buf.data(), buf.size(), &dn)) return 2;
+    ForceSys S;
+    fill_sys(ctx, S);
+    CK(hipMemsetAsync(s->counts + 4, 0, 4, ctx->stream));
+    CK(hipMemsetAsync(s->counts + 6, 0, 4, ctx->stream));
+    CK(replay_move(s->P, S, B, dn, dn + (size_t)n_sr * 3, n_sr, ctx->stream));
+  }
+  CK(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  if (any_above) *any_above = n_sr > 0 ? h[4] : 0;
+  if (n_overlap) *n_overlap = n_sr > 0 ? h[6] : 0;
+  return 0;
+}
+
+int pst_replay_bridge(pst_sim* s, const double* u_bb, int n_sr, int* n_bridge) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  ReplayBufs B = replay_bufs(s);
+  CK(hipMemsetAsync(s->counts + 5, 0, 4, ctx->stream));
+  double* du = nullptr;
+  if (u_bb && n_sr > 0 && s->P.n_pairs > 0) {
+    if (stage(s, u_bb, (size_t)n_sr * s->P.n_pairs, &du)) return 2;
+  }
+  CK(replay_bridge(s->P, B, du, n_sr, ctx->stream));
+  int h[8];
+  CK(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  if (n_bridge) *n_bridge = h[5];
+  return 0;
+}
+
+int pst_replay_forces(pst_sim* s, double* F_host, int n_sr) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  if (!s->lists) FAIL("pst_replay_forces: no replay state");
+  std::vector<int> idx(n_sr);
+  std::vector<double> all((size_t)s->S * 3);
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(idx.data(), s->lists + 3 * (size_t)s->S, (size_t)n_sr * 4, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(all.data(), s->F, all.size() * 8, hipMemcpyDeviceToHost));
+  for (int w = 0; w < n_sr; ++w)
+    for (int d = 0; d < 3; ++d) F_host[3 * w + d] = all[3 * (size_t)idx[w] + d];
+  return 0;
+}
+
+// ---- production ------------------------------------------------------------
+static ProdBufs prod_bufs(pst_sim* s) {
+  ProdBufs B;
+  B.pos = s->pos; B.quat = s->quat; B.status = s->status; B.n_steps = s->n_steps32;
+  B.traj = s->traj; B.plain_list = s->plain_list; B.mlp_list = s->mlp_list;
+  B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
+  B.ctr = s->ctr; B.out_status = s->out_status; B.out_steps = s->out_steps;
+  B.n_slots = s->S;
+  return B;
+}
+
+int pst_prod_init(pst_sim* s, int record) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  const size_t S = s->S;
+  if (!s->n_steps32) {
+    CK(hipMalloc(&s->n_steps32, S * 4));
+    CK(hipMalloc(&s->traj, S * 8));
+    CK(hipMalloc(&s->plain_list, S * 4));
+    CK(hipMalloc(&s->mlp_list, S * 4));
+    CK(hipMalloc(&s->ctr, sizeof(RunCounters)));
+  }
+  if (s->out_status) { hipFree(s->out_status); s->out_status = nullptr; }
+  if (s->out_steps) { hipFree(s->out_steps); s->out_steps = nullptr; }
+  if (record) {
+    const size_t N = (size_t)s->P.n_total;
+    CK(hipMalloc(&s->out_status, N));
+    CK(hipMalloc(&s->out_steps, N * 4));
+    CK(hipMemsetAsync(s->out_status, 0, N, ctx->stream));
+    CK(hipMemsetAsync(s->out_steps, 0, N * 4, ctx->stream));
+  }
+  CK(prod_init(s->P, prod_bufs(s), ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  s->prod_ready = true;
+  return 0;
+}
+
+static int enqueue_step(pst_sim* s, int par) {
+  pst_ctx* ctx = s->ctx;
+  ProdBufs B = prod_bufs(s);
+  CK(prod_step_pre(s->P, B, par, ctx->stream));
+  ForceWork W;
+  memset(&W, 0, sizeof(W));
+  W.list = s->plain_list;
+  W.n_dev = &s->ctr->n_sr;
+  W.pos = s->pos;
+  W.quat = s->quat;
+  W.F = s->F; W.T = s->T; W.E = nullptr;
+  W.rev_skip_mode = 0;
+  ForceSys S;
+  fill_sys(ctx, S);
+  CK(launch_forces(S, W, s->precision, group_for(ctx, S), s->S, 2048, ctx->stream));
+  CK(prod_step_post(s->P, S, B, par, ctx->stream));
+  CK(prod_step_bridge(s->P, B, par, ctx->stream));
+  return 0;
+}
+
+int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_run: call pst_prod_init first");
+  if (max_steps < 0 || graph_steps < 0) FAIL("pst_prod_run: bad arguments");
+  CK(set_dev(ctx));
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  long long steps = 0;
+  RunCounters h;
+  memset(&h, 0, sizeof(h));
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t cap = nullptr;
+  const int gs = graph_steps;
+  if (gs > 0) {
+    // capture on a dedicated stream (the legacy null stream cannot be captured)
+    CK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    CK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    hipStream_t saved = ctx->stream;
+    ctx->stream = cap;
+    int rc = 0;
+    for (int i = 0; i < gs && !rc; ++i) rc = enqueue_step(s, i & 1);
+    ctx->stream = saved;
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(cap, &g);
+    if (rc) { if (g) hipGraphDestroy(g); hipStreamDestroy(cap); return rc; }
+    if (ee != hipSuccess) { hipStreamDestroy(cap); ctx->err = hipGetErrorString(ee); return 2; }
+    graph = g;
+    CK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  CK(hipEventRecord(e0, ctx->stream));
+  bool stop = false;
+  while (steps < max_steps && !stop) {
+    if (gs > 0 && steps + gs <= max_steps) {
+      CK(hipGraphLaunch(exec, ctx->stream));
+      steps += gs;
+    } else {
+      int rc = enqueue_step(s, (int)(steps & 1));
+      if (rc) return rc;
+      steps += 1;
+    }
+    CK(hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (h.stop || (h.n_sr + h.n_mlp) == 0) stop = true;
+  }
+  CK(hipEventRecord(e1, ctx->stream));
+  CK(hipEventSynchronize(e1));
+  float ms = 0;
+  CK(hipEventElapsedTime(&ms, e0, e1));
+  s->last_ms_per_step = steps ? ms / steps : 0.0;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  if (cap) hipStreamDestroy(cap);
+  if (c16) {
+    double v[16] = {(double)h.n_reacted, (double)h.n_escaped, (double)h.n_maxsteps,
+                    (double)h.total_steps, (double)h.next_traj, (double)h.n_returns,
+                    (double)h.n_overlap, (double)h.n_bridge, (double)steps, (double)h.stop,
+                    (double)h.n_sr, (double)h.n_mlp, 0, 0, 0, 0};
+    memcpy(c16, v, sizeof(v));
+  }
+  return 0;
+}
+
+int pst_prod_outcomes(pst_sim* s, signed char* status, int* steps) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->out_status) FAIL("pst_prod_outcomes: run was not recorded (pst_prod_init record=0)");
+  CK(set_dev(ctx));
+  CK(hipStreamSynchronize(ctx->stream));
+  const size_t N = (size_t)s->P.n_total;
+  if (status) CK(hipMemcpy(status, s->out_status, N, hipMemcpyDeviceToHost));
+  if (steps) CK(hipMemcpy(steps, s->out_steps, N * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pst_prod_last_ms_per_step(pst_sim* s, double* ms) {
+  if (!s || !ms) return 1;
+  *ms = s->last_ms_per_step;
+  return 0;
+}
+
+// ---- synthetic grids ---------------------------------------------------------
+int pst_make_grid(pst_ctx* ctx, int kind, int nx, int ny, int nz, const double* origin3,
+                  const double* spacing3, const double* atoms, int n_atoms, double debye_length,
+                  double eps_out, double eps_in, int dtype, void* host_out) {
+  if (!ctx) return 1;
+  if (kind != 0 && kind != 1) FAIL("pst_make_grid: kind must be 0 or 1");
+  if (nx < 2 || ny < 2 || nz < 2 || n_atoms <= 0 || !atoms || !host_out || !origin3 || !spacing3)
+    FAIL("pst_make_grid: bad arguments");
+  CK(set_dev(ctx));
+  const size_t count = (size_t)nx * ny * nz;
+  double* d_atoms = nullptr;
+  void* d_out = nullptr;
+  CK(hipMalloc(&d_atoms, (size_t)n_atoms * 5 * 8));
+  CK(hipMalloc(&d_out, count * (dtype == PST_F64 ? 8 : 4)));
+  CK(hipMemcpy(d_atoms, atoms, (size_t)n_atoms * 40, hipMemcpyHostToDevice));
+  CK(launch_grid_dh(dtype == PST_F64 ? nullptr : (float*)d_out,
+                    dtype == PST_F64 ? (double*)d_out : nullptr, nx, ny, nz, origin3, spacing3,
+                    d_atoms, n_atoms, debye_length, eps_out, eps_in, kind, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(host_out, d_out, count * (dtype == PST_F64 ? 8 : 4), hipMemcpyDeviceToHost));
+  hipFree(d_atoms);
+  hipFree(d_out);
+  return 0;
+}
+
+}  // extern "C"

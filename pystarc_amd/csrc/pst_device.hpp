@@ -1,0 +1,199 @@
+// Device helpers for the batched BD step on gfx950: counter-based RNG,
+// quaternion algebra, trilinear grid gather and the Yukawa multipole
+// far-field.  Every physics formula cites the PySTARC reference it restates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pst_types.hpp"
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11, "Parallel random numbers: as easy as
+// 1, 2, 3"): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
+// 0x9E3779B9 / 0xBB67AE85.  Counter = (traj lo, traj hi, step, stream).
+// ---------------------------------------------------------------------------
+struct PhiloxOut { uint32_t v[4]; };
+
+__host__ __device__ inline PhiloxOut philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                   uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  PhiloxOut o;
+  o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+  return o;
+}
+
+enum { RNG_INIT = 1, RNG_ESC = 2, RNG_RETQ = 3, RNG_NOISE = 4, RNG_BRIDGE = 5 };
+
+// Two uniforms in [0,1) with 53 random bits each.
+__device__ inline void philox_uniform2(uint64_t seed, uint64_t traj, uint32_t step,
+                                       uint32_t stream, uint32_t sub, double& u0, double& u1) {
+  PhiloxOut o = philox4x32_10((uint32_t)traj, (uint32_t)(traj >> 32), step,
+                              stream | (sub << 8), (uint32_t)seed, (uint32_t)(seed >> 32));
+  uint64_t a = ((uint64_t)o.v[0] << 21) ^ (uint64_t)(o.v[1] >> 11);
+  uint64_t b = ((uint64_t)o.v[2] << 21) ^ (uint64_t)(o.v[3] >> 11);
+  a &= ((1ull << 53) - 1);
+  b &= ((1ull << 53) - 1);
+  const double s = 1.0 / 9007199254740992.0;  // 2^-53
+  u0 = (double)a * s;
+  u1 = (double)b * s;
+}
+
+// Two standard normals by Box-Muller from one Philox block.
+__device__ inline void philox_normal2(uint64_t seed, uint64_t traj, uint32_t step,
+                                      uint32_t stream, uint32_t sub, double& z0, double& z1) {
+  double u0, u1;
+  philox_uniform2(seed, traj, step, stream, sub, u0, u1);
+  const double one_minus = 1.0 - u0;  // in (0, 1]
+  double r = sqrt(-2.0 * log(one_minus));
+  double s, c;
+  sincos(2.0 * M_PI * u1, &s, &c);
+  z0 = r * c;
+  z1 = r * s;
+}
+
+// ---------------------------------------------------------------------------
+// Quaternions, (w, x, y, z) order.
+// ---------------------------------------------------------------------------
+// R(q), gpu_batch_simulator.py:1611-1629.
+__device__ inline void quat_to_rot(const double* q, double* R) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - w * z); R[2] = 2 * (x * z + w * y);
+  R[3] = 2 * (x * y + w * z); R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
+  R[6] = 2 * (x * z - w * y); R[7] = 2 * (y * z + w * x); R[8] = 1 - 2 * (x * x + y * y);
+}
+
+// Shoemake uniform rotation from three uniforms, gpu_batch_simulator.py:1597-1609.
+__device__ inline void shoemake(double u0, double u1, double u2, double* q) {
+  const double a = sqrt(1.0 - u0), b = sqrt(u0);
+  q[0] = a * sin(2.0 * M_PI * u1);
+  q[1] = a * cos(2.0 * M_PI * u1);
+  q[2] = b * sin(2.0 * M_PI * u2);
+  q[3] = b * cos(2.0 * M_PI * u2);
+}
+
+// q <- normalize(q (x) dq(omega)), gpu_batch_simulator.py:1631-1660.
+__device__ inline void apply_rotation(double* q, double ox, double oy, double oz) {
+  const double ang = sqrt(ox * ox + oy * oy + oz * oz);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (ang > 1e-10) { ax = ox / ang; ay = oy / ang; az = oz / ang; }
+  const double half = ang * 0.5;
+  const double s = sin(half), c = cos(half);
+  const double w2 = c, x2 = ax * s, y2 = ay * s, z2 = az * s;
+  const double w1 = q[0], x1 = q[1], y1 = q[2], z1 = q[3];
+  double nw = w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2;
+  double nx = w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2;
+  double ny = w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2;
+  double nz = w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2;
+  const double n = sqrt(nw * nw + nx * nx + ny * ny + nz * nz);
+  q[0] = nw / n; q[1] = nx / n; q[2] = ny / n; q[3] = nz / n;
+}
+
+__device__ inline double norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+// ---------------------------------------------------------------------------
+// Trilinear gather, batch_force_kernel (gpu_batch_engine.py:95-110): cell
+// index floor((p - o) / sp); value 0 when the cell is outside [0, n-2].
+// Index arithmetic is FP64; weights and sums are in TC (float or double).
+// ---------------------------------------------------------------------------
+template <typename TG, typename TC>
+__device__ inline TC grid_interp(const GridDev& g, double x, double y, double z) {
+  const double ix = (x - g.o[0]) * g.inv[0];
+  const double iy = (y - g.o[1]) * g.inv[1];
+  const double iz = (z - g.o[2]) * g.inv[2];
+  const double fi = floor(ix), fj = floor(iy), fk = floor(iz);
+  const int i0 = (int)fi, j0 = (int)fj, k0 = (int)fk;
+  if (i0 < 0 || i0 >= g.nx - 1 || j0 < 0 || j0 >= g.ny - 1 || k0 < 0 || k0 >= g.nz - 1)
+    return (TC)0;
+  const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
+  const TG* __restrict__ d = (const TG*)g.data;
+  const size_t sxy = (size_t)g.ny * g.nz;
+  const size_t base = (size_t)i0 * sxy + (size_t)j0 * g.nz + k0;
+  const TC c000 = (TC)d[base], c001 = (TC)d[base + 1];
+  const TC c010 = (TC)d[base + g.nz], c011 = (TC)d[base + g.nz + 1];
+  const TC c100 = (TC)d[base + sxy], c101 = (TC)d[base + sxy + 1];
+  const TC c110 = (TC)d[base + sxy + g.nz], c111 = (TC)d[base + sxy + g.nz + 1];
+  const TC gx = (TC)1 - fx, gy = (TC)1 - fy, gz = (TC)1 - fz;
+  return c000 * gx * gy * gz + c100 * fx * gy * gz + c010 * gx * fy * gz + c001 * gx * gy * fz +
+         c110 * fx * fy * gz + c101 * fx * gy * fz + c011 * gx * fy * fz + c111 * fx * fy * fz;
+}
+
+// Strictly inside the valid box (gpu_batch_engine.py:317-319).
+__device__ inline bool grid_inside(const GridDev& g, double x, double y, double z) {
+  return x > g.lo[0] && y > g.lo[1] && z > g.lo[2] && x < g.hi[0] && y < g.hi[1] && z < g.hi[2];
+}
+
+// phi and grad phi by central differences at +-sp/2 (gpu_batch_engine.py:111-115).
+template <typename TG, typename TC>
+__device__ inline void grid_phi_grad(const GridDev& g, double px, double py, double pz, TC& phi,
+                                     TC& gx, TC& gy, TC& gz) {
+  const double h0 = g.sp[0] * 0.5, h1 = g.sp[1] * 0.5, h2 = g.sp[2] * 0.5;
+  gx = (grid_interp<TG, TC>(g, px + h0, py, pz) - grid_interp<TG, TC>(g, px - h0, py, pz)) /
+       (TC)(2 * h0);
+  gy = (grid_interp<TG, TC>(g, px, py + h1, pz) - grid_interp<TG, TC>(g, px, py - h1, pz)) /
+       (TC)(2 * h1);
+  gz = (grid_interp<TG, TC>(g, px, py, pz + h2) - grid_interp<TG, TC>(g, px, py, pz - h2)) /
+       (TC)(2 * h2);
+  phi = grid_interp<TG, TC>(g, px, py, pz);
+}
+
+template <typename TC>
+__device__ inline void grid_phi_grad_any(const GridDev& g, double px, double py, double pz,
+                                         TC& phi, TC& gx, TC& gy, TC& gz) {
+  if (g.dtype == 1)
+    grid_phi_grad<double, TC>(g, px, py, pz, phi, gx, gy, gz);
+  else
+    grid_phi_grad<float, TC>(g, px, py, pz, phi, gx, gy, gz);
+}
+
+template <typename TC> __device__ inline TC pst_exp(TC x);
+template <> __device__ inline double pst_exp<double>(double x) { return exp(x); }
+template <> __device__ inline float pst_exp<float>(float x) { return __expf(x); }
+
+// Yukawa monopole + dipole + quadrupole with the reference's radial-only
+// gradient (gpu_batch_engine.py:401-465).  Accumulates F += -q dphi/dr r_hat,
+// E += q phi for an atom at lab position p (receptor at the origin).
+template <typename TC>
+__device__ inline void yukawa_atom(const ForceSys& S, TC px, TC py, TC pz, TC q, TC& fx, TC& fy,
+                                   TC& fz, TC& e) {
+  const TC lam = (TC)S.debye;
+  const TC r = sqrt(px * px + py * py + pz * pz);
+  const TC sr = r > (TC)1 ? r : (TC)1;
+  const TC ex = pst_exp<TC>(-sr / lam);
+  const TC inv = (TC)1 / sr;
+  const TC rx = px * inv, ry = py * inv, rz = pz * inv;
+  const TC V = (TC)S.V_fac;
+  TC phi = V * ex * inv;
+  TC dphi = V * ex * (-inv * inv - inv / lam);
+  if (S.has_dip) {
+    const TC fpe = (TC)S.fpe;
+    const TC pdr = rx * (TC)S.dip[0] + ry * (TC)S.dip[1] + rz * (TC)S.dip[2];
+    phi += pdr / (fpe * sr * sr) * ((TC)1 + sr / lam) * ex;
+    dphi += pdr / fpe * ex *
+            ((TC)-2 * inv * inv * inv - (TC)2 * inv * inv / lam - inv / (lam * lam));
+  }
+  if (S.has_quad) {
+    const TC fpe = (TC)S.fpe;
+    const TC qx = (TC)S.quad[0] * rx + (TC)S.quad[1] * ry + (TC)S.quad[2] * rz;
+    const TC qy = (TC)S.quad[3] * rx + (TC)S.quad[4] * ry + (TC)S.quad[5] * rz;
+    const TC qz = (TC)S.quad[6] * rx + (TC)S.quad[7] * ry + (TC)S.quad[8] * rz;
+    const TC rQr = rx * qx + ry * qy + rz * qz;
+    phi += rQr / (fpe * sr * sr * sr) * ((TC)1 + sr / lam + sr * sr / ((TC)3 * lam * lam)) * ex;
+    const TC i2 = inv * inv;
+    dphi += rQr / fpe * ex *
+            ((TC)-3 * i2 * i2 - (TC)3 * i2 * inv / lam - i2 / (lam * lam) -
+             inv / ((TC)3 * lam * lam * lam));
+  }
+  fx -= q * dphi * rx;
+  fy -= q * dphi * ry;
+  fz -= q * dphi * rz;
+  e += q * phi;
+}

@@ -1,0 +1,73 @@
+// Synthetic APBS-style grids evaluated on the device, one thread per grid
+// node, atoms streamed through LDS in tiles.  Used to build the benchmark
+// systems (APBS itself is not part of this library).
+//
+// kind 0, screened Coulomb / Debye-Hueckel potential in kBT/e:
+//   phi(x) = sum_j q_j exp(-(d - a_j)/lambda) / ((1 + a_j/lambda) 4 pi eps0 eps_out d),
+//   d = max(|x - r_j|, a_j, 0.5 A); with a_j = 0 this is the reference's
+//   Yukawa monopole V_fac e^{-r/lambda}/r (gpu_batch_engine.py:407-410).
+// kind 1, analytic Born desolvation field:
+//   phi_B(x) = (1/eps_in - 1/eps_out) / (8 pi eps0) sum_j a_j^3 / max(|x - r_j|, a_j)^4,
+//   multiplied by alpha q^2 in the force kernel (gpu_batch_engine.py:116-121).
+#include <hip/hip_runtime.h>
+#include "pst_kernels.hpp"
+
+namespace pst {
+
+constexpr double kEps0 = 0.000142;  // e^2/(kBT A), pystarc/global_defs/constants.py:47-49
+
+__global__ void __launch_bounds__(256) k_grid(float* out32, double* out64, int nx, int ny, int nz,
+                                              double ox, double oy, double oz, double sx,
+                                              double sy, double sz, const double* atoms, int n,
+                                              double lam, double eps_out, double eps_in, int kind) {
+  __shared__ double tile[256 * 5];
+  const long long total = (long long)nx * ny * nz;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = idx < total;
+  const int k = valid ? (int)(idx % nz) : 0;
+  const int j = valid ? (int)((idx / nz) % ny) : 0;
+  const int i = valid ? (int)(idx / ((long long)ny * nz)) : 0;
+  const double x = ox + i * sx, y = oy + j * sy, z = oz + k * sz;
+  double acc = 0.0;
+  const double pref0 = 1.0 / (4.0 * M_PI * kEps0 * eps_out);
+  const double pref1 = (1.0 / eps_in - 1.0 / eps_out) / (8.0 * M_PI * kEps0);
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    const int a = base + threadIdx.x;
+    if (a < n) {
+      for (int c = 0; c < 5; ++c) tile[threadIdx.x * 5 + c] = atoms[5 * a + c];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    for (int t = 0; t < cnt; ++t) {
+      const double dx = x - tile[5 * t], dy = y - tile[5 * t + 1], dz = z - tile[5 * t + 2];
+      const double q = tile[5 * t + 3], rad = tile[5 * t + 4];
+      double d = sqrt(dx * dx + dy * dy + dz * dz);
+      if (kind == 0) {
+        d = fmax(d, fmax(rad, 0.5));
+        acc += q * exp(-(d - rad) / lam) / ((1.0 + rad / lam) * d) * pref0;
+      } else {
+        d = fmax(d, fmax(rad, 0.5));
+        const double d2 = d * d;
+        acc += pref1 * rad * rad * rad / (d2 * d2);
+      }
+    }
+  }
+  if (valid) {
+    if (out32) out32[idx] = (float)acc;
+    if (out64) out64[idx] = acc;
+  }
+}
+
+hipError_t launch_grid_dh(float* out, double* out64, int nx, int ny, int nz, const double* origin,
+                          const double* sp, const double* atoms, int n_atoms, double lam,
+                          double eps_out, double eps_in, int kind, hipStream_t st) {
+  const long long total = (long long)nx * ny * nz;
+  const long long blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(k_grid, dim3((unsigned)blocks), dim3(256), 0, st, out, out64, nx, ny, nz,
+                     origin[0], origin[1], origin[2], sp[0], sp[1], sp[2], atoms, n_atoms, lam,
+                     eps_out, eps_in, kind);
+  return hipGetLastError();
+}
+
+}  // namespace pst

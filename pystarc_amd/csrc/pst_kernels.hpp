@@ -1,0 +1,60 @@
+// Kernel launchers shared between the translation units of libpystarc_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pst_types.hpp"
+
+namespace pst {
+
+hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, int G, int max_n,
+                         int blocks_cap, hipStream_t st);
+hipError_t launch_chunk_min_r(const double* pos, const int* list, int n, int fb, int cs, int cpf,
+                              double* out, int n_chunks, hipStream_t st);
+int pick_group(int M, int n_rec_active);
+
+// Ordered (stable) compaction of in_list (or 0..n-1) by flag[slot].
+hipError_t launch_compact_ordered(const int* in_list, int n_in, const int* n_in_dev,
+                                  const unsigned char* flag, int* out, int* n_out, hipStream_t st);
+
+// Replay-mode step phases (host-supplied noise, reference draw order).
+struct ReplayBufs {
+  double* pos; double* quat; int* status; long long* n_steps;
+  unsigned char* f_run; unsigned char* f_esc; unsigned char* f_ret; unsigned char* f_sr;
+  int* run_list; int* esc_list; int* ret_list; int* sr_list;
+  int* counts;           // [0]=n_run [1]=n_esc [2]=n_ret [3]=n_sr [4]=any_above [5]=n_bridge [6]=n_overlap [7]=n_react_start
+  double* F; double* T; double* dt; double* x0; double* x1;
+  double* old_pos; double* old_quat;
+  int n_slots;
+};
+hipError_t replay_begin(const StepParams& P, const ReplayBufs& B, hipStream_t st);
+hipError_t replay_escape_decide(const StepParams& P, const ReplayBufs& B, const double* u_outer,
+                                hipStream_t st);
+hipError_t replay_escape_apply(const StepParams& P, const ReplayBufs& B, const double* u_quat,
+                               hipStream_t st);
+hipError_t replay_build_sr(const ReplayBufs& B, hipStream_t st);
+hipError_t replay_move(const StepParams& P, const ForceSys& S, const ReplayBufs& B,
+                       const double* noise_t, const double* noise_r, int n_sr, hipStream_t st);
+hipError_t replay_bridge(const StepParams& P, const ReplayBufs& B, const double* u_bb, int n_sr,
+                         hipStream_t st);
+
+// Production step (device Philox, refilling slots).
+struct ProdBufs {
+  double* pos; double* quat; int* status; unsigned int* n_steps; long long* traj;
+  int* plain_list; int* mlp_list;
+  double* F; double* T; double* dt; double* x0; double* x1;
+  RunCounters* ctr;
+  signed char* out_status;     // per trajectory id (nullable)
+  int* out_steps;              // per trajectory id (nullable)
+  int n_slots;
+};
+hipError_t prod_init(const StepParams& P, const ProdBufs& B, hipStream_t st);
+hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
+hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs& B, int par,
+                          hipStream_t st);
+hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
+
+// Synthetic grids.
+hipError_t launch_grid_dh(float* out, double* out64, int nx, int ny, int nz, const double* origin,
+                          const double* sp, const double* atoms, int n_atoms, double kappa,
+                          double eps_out, double eps_in, int kind, hipStream_t st);
+
+}  // namespace pst

@@ -1,0 +1,556 @@
+// Per-trajectory BD step kernels (FP64 state) for gfx950.
+//
+// Restates the flattened-pairs step of GPUBatchSimulator.run
+// (pystarc/simulation/gpu_batch_simulator.py:783-1356):
+//   reaction check :855-871, escape / return :879-911, adaptive dt :961-1067,
+//   HI D_parallel :1074-1084, Ermak-McCammon translation :1085-1094,
+//   rotation :1095-1101, overlap rejection :1106-1128, Brownian bridge
+//   :1142-1252, bookkeeping :1253-1254, early stop :1347-1348.
+//
+// Two drivers share the same __device__ physics:
+//  * replay kernels take the reference's host NumPy draws, indexed by the
+//    rank of a trajectory in the reference's ordered index lists, so a run
+//    reproduces the reference's reaction / escape flags bit for bit;
+//  * production kernels draw Philox4x32-10 numbers keyed by (trajectory id,
+//    trajectory step, stream), keep a fixed pool of slots busy by refilling a
+//    finished slot with the next trajectory, and never synchronise with the host.
+#include <hip/hip_runtime.h>
+#include "pst_device.hpp"
+#include "pst_kernels.hpp"
+
+namespace pst {
+
+constexpr int kStepBlock = 256;
+
+// ----------------------------------------------------------------------------
+// shared step physics
+// ----------------------------------------------------------------------------
+// Pair distances |R b_p + c - r_p| (gpu_batch_simulator.py:1737-1751).
+__device__ inline void pair_dists(const StepParams& P, const double* R, double cx, double cy,
+                                  double cz, double* d) {
+  for (int p = 0; p < P.n_pairs; ++p) {
+    const double bx = P.lig_gho[3 * p], by = P.lig_gho[3 * p + 1], bz = P.lig_gho[3 * p + 2];
+    const double gx = R[0] * bx + R[1] * by + R[2] * bz + cx - P.rec_gho[3 * p];
+    const double gy = R[3] * bx + R[4] * by + R[5] * bz + cy - P.rec_gho[3 * p + 1];
+    const double gz = R[6] * bx + R[7] * by + R[8] * bz + cz - P.rec_gho[3 * p + 2];
+    d[p] = sqrt(gx * gx + gy * gy + gz * gz);
+  }
+}
+
+__device__ inline bool reacted_at(const StepParams& P, const double* pos, const double* q) {
+  if (P.n_pairs == 0) return false;
+  double R[9], d[PST_MAX_PAIRS];
+  quat_to_rot(q, R);
+  pair_dists(P, R, pos[0], pos[1], pos[2], d);
+  int n = 0;
+  for (int p = 0; p < P.n_pairs; ++p) n += d[p] < P.cutoff[p] ? 1 : 0;
+  return n >= P.n_needed;
+}
+
+// dt per trajectory, flattened mode (:975-1004, :1061-1067).
+__device__ inline double adaptive_dt(const StepParams& P, double r) {
+  const double D_t = P.D_t;
+  const bool in_outer = r > 1.1 * P.r_b;
+  const double dt_pair = (0.1 * 0.1 / 2.0) * r * r / D_t;
+  double dt_force = dt_pair;
+  if (P.adt_has_force) {
+    const double lam = P.adt_debye;
+    const double e = exp(-r / lam);
+    const double Fr0 = P.adt_V0 * e * (1.0 / (r * r) + 1.0 / (r * lam));
+    const double Fr1 = -P.adt_V0 * e / (lam * lam) - 2.0 * Fr0 / r;
+    const double DFr1 = fabs(D_t * Fr1);
+    if (DFr1 > 1e-15 && r < 3.0 * lam) dt_force = 0.01 / DFr1;
+  }
+  const double dist_b = fmax(r - P.r_b, 1e-3);
+  const double dist_e = fmax(P.r_esc - r, 1e-3);
+  const double de = fmin(dist_b, dist_e);
+  const double dt_edge = de * de / (18.0 * D_t);
+  double dt = in_outer ? fmin(fmin(dt_force, dt_edge), dt_pair) : fmin(dt_force, dt_pair);
+  if (P.max_dt > 0) dt = fmin(dt, P.max_dt);
+  if (P.min_core_dt > 0) dt = fmax(dt, P.min_core_dt);
+  return dt;
+}
+
+// Scalar relative diffusion, isotropic D_parallel with HI (:1074-1084).
+__device__ inline double diffusion(const StepParams& P, double r) {
+  if (!P.use_hi) return P.D_t;
+  const double rm1 = 1.0 / fmax(r, 0.01);
+  const double rm3 = rm1 * rm1 * rm1;
+  const double Di = -2.0 * P.hi_Df * (2.0 * rm1 - (4.0 / 3.0) * P.hi_a2 * rm3) / (8.0 * M_PI);
+  return fmax(P.hi_Df * P.hi_ainv + Di, 1e-6);
+}
+
+// Overlap: min_j |c - r_j| < threshold (:1106-1128).  The loop is skipped
+// when |c| - max|r_j| already exceeds the threshold, which cannot change the
+// outcome.
+__device__ inline bool overlaps(const StepParams& P, const ForceSys& S, double cx, double cy,
+                                double cz) {
+  const double rc = norm3(cx, cy, cz);
+  if (rc - S.rec_max_r > P.overlap_thresh + 1e-9) return false;
+  double m2 = 1e300;
+  for (int j = 0; j < S.n_rec; ++j) {
+    const double dx = cx - S.rec_pos[3 * j], dy = cy - S.rec_pos[3 * j + 1],
+                 dz = cz - S.rec_pos[3 * j + 2];
+    m2 = fmin(m2, dx * dx + dy * dy + dz * dz);
+  }
+  return sqrt(m2) < P.overlap_thresh;
+}
+
+// Translate + rotate + overlap + new pair distances.  Returns whether any
+// pair is above its cutoff both before and after the move.
+__device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* pos, double* q,
+                                const double* Fv, const double* Tv, const double* nt,
+                                const double* nr, double dt, double D, double* x0, double* x1,
+                                bool* rejected) {
+  double R[9], d[PST_MAX_PAIRS];
+  quat_to_rot(q, R);
+  pair_dists(P, R, pos[0], pos[1], pos[2], d);
+  for (int p = 0; p < P.n_pairs; ++p) x0[p] = d[p] - P.cutoff[p];
+  const double op0 = pos[0], op1 = pos[1], op2 = pos[2];
+  const double oq0 = q[0], oq1 = q[1], oq2 = q[2], oq3 = q[3];
+  // Ermak-McCammon translation (:1085-1094)
+  const double sig = sqrt(2.0 * D * dt);
+  pos[0] += D * dt * Fv[0] + sig * nt[0];
+  pos[1] += D * dt * Fv[1] + sig * nt[1];
+  pos[2] += D * dt * Fv[2] + sig * nt[2];
+  // rotation (:1095-1101)
+  const double sr = sqrt(2.0 * P.D_r * dt);
+  apply_rotation(q, P.D_r * dt * Tv[0] + sr * nr[0], P.D_r * dt * Tv[1] + sr * nr[1],
+                 P.D_r * dt * Tv[2] + sr * nr[2]);
+  *rejected = false;
+  if (P.use_overlap && S.n_rec > 0 && overlaps(P, S, pos[0], pos[1], pos[2])) {
+    pos[0] = op0; pos[1] = op1; pos[2] = op2;
+    q[0] = oq0; q[1] = oq1; q[2] = oq2; q[3] = oq3;
+    *rejected = true;
+  }
+  bool any = false;
+  if (P.n_pairs) {
+    quat_to_rot(q, R);
+    pair_dists(P, R, pos[0], pos[1], pos[2], d);
+    for (int p = 0; p < P.n_pairs; ++p) {
+      x1[p] = d[p] - P.cutoff[p];
+      any = any || (x0[p] > 0 && x1[p] > 0);
+    }
+  }
+  return any;
+}
+
+// Brownian-bridge pair count (:1157-1184, :1236-1239).
+__device__ inline int bridge_count(const StepParams& P, const double* x0, const double* x1,
+                                   double dt, const double* u) {
+  int n = 0;
+  for (int p = 0; p < P.n_pairs; ++p) {
+    double pc = 0.0;
+    if (x0[p] > 0 && x1[p] > 0) {
+      const double Ddt = fmax((P.has_bb_deff ? P.bb_deff[p] : P.D_t) * dt, 1e-30);
+      pc = exp(-x0[p] * x1[p] / Ddt);
+    }
+    const bool fired = (u[p] < pc) || (x1[p] < 0.0);
+    n += fired ? 1 : 0;
+  }
+  return n;
+}
+
+// ----------------------------------------------------------------------------
+// ordered compaction (one workgroup; replay mode and small lists)
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_compact_ordered(const int* in_list, int n_in,
+                                                          const int* n_in_dev,
+                                                          const unsigned char* flag, int* out,
+                                                          int* n_out) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int n = n_in_dev ? *n_in_dev : n_in;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + tid;
+    int s = -1;
+    bool f = false;
+    if (i < n) {
+      s = in_list ? in_list[i] : i;
+      f = flag[s] != 0;
+    }
+    const unsigned long long m = __ballot(f);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = __popcll(m);
+    __syncthreads();
+    int wp = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+      if (k < wave) wp += wsum[k];
+      tot += wsum[k];
+    }
+    if (f) out[carry + wp + before] = s;
+    __syncthreads();
+    if (tid == 0) carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) *n_out = carry;
+}
+
+hipError_t launch_compact_ordered(const int* in_list, int n_in, const int* n_in_dev,
+                                  const unsigned char* flag, int* out, int* n_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_compact_ordered, dim3(1), dim3(1024), 0, st, in_list, n_in, n_in_dev, flag,
+                     out, n_out);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// replay kernels
+// ----------------------------------------------------------------------------
+// Phase A: reaction check on running trajectories; flag escape candidates.
+__global__ void k_replay_begin(StepParams P, ReplayBufs B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.n_slots) return;
+  B.f_esc[s] = 0;
+  B.f_ret[s] = 0;
+  B.f_sr[s] = 0;
+  if (B.status[s] != PST_RUNNING) { B.f_run[s] = 0; return; }
+  B.f_run[s] = 1;
+  atomicAdd(&B.counts[0], 1);
+  const double* pos = B.pos + 3 * (size_t)s;
+  const bool react = reacted_at(P, pos, B.quat + 4 * (size_t)s);
+  if (react) {
+    B.status[s] = PST_REACTED;
+    atomicAdd(&B.counts[7], 1);
+  }
+  const double r = norm3(pos[0], pos[1], pos[2]);
+  if (r >= P.r_esc && !react) B.f_esc[s] = 1;
+}
+
+hipError_t replay_begin(const StepParams& P, const ReplayBufs& B, hipStream_t st) {
+  hipError_t e = hipMemsetAsync(B.counts, 0, 8 * sizeof(int), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_replay_begin, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = launch_compact_ordered(nullptr, B.n_slots, nullptr, B.f_run, B.run_list, &B.counts[0], st);
+  if (e != hipSuccess) return e;
+  return launch_compact_ordered(B.run_list, 0, &B.counts[0], B.f_esc, B.esc_list, &B.counts[1],
+                                st);
+}
+
+// Phase B1: u_outer[e] < return_prob decides return vs escape (:884-911).
+__global__ void k_replay_esc_decide(StepParams P, ReplayBufs B, const double* u) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B.counts[1]) return;
+  const int s = B.esc_list[e];
+  if (u[e] < P.return_prob) {
+    B.f_ret[s] = 1;
+  } else {
+    B.status[s] = PST_ESCAPED;
+  }
+}
+
+hipError_t replay_escape_decide(const StepParams& P, const ReplayBufs& B, const double* u_outer,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_replay_esc_decide, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B, u_outer);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_compact_ordered(B.esc_list, 0, &B.counts[1], B.f_ret, B.ret_list, &B.counts[2],
+                                st);
+}
+
+// Phase B2: snap returners to the b-sphere with a fresh Shoemake rotation.
+__global__ void k_replay_esc_apply(StepParams P, ReplayBufs B, const double* uq) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B.counts[2]) return;
+  const int s = B.ret_list[j];
+  double* pos = B.pos + 3 * (size_t)s;
+  const double r = norm3(pos[0], pos[1], pos[2]);
+  pos[0] = pos[0] / r * P.r_b;
+  pos[1] = pos[1] / r * P.r_b;
+  pos[2] = pos[2] / r * P.r_b;
+  shoemake(uq[3 * j], uq[3 * j + 1], uq[3 * j + 2], B.quat + 4 * (size_t)s);
+}
+
+hipError_t replay_escape_apply(const StepParams& P, const ReplayBufs& B, const double* u_quat,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_replay_esc_apply, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B, u_quat);
+  return hipGetLastError();
+}
+
+__global__ void k_flag_sr(ReplayBufs B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.n_slots) return;
+  B.f_sr[s] = (B.f_run[s] && B.status[s] == PST_RUNNING) ? 1 : 0;
+}
+
+hipError_t replay_build_sr(const ReplayBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_flag_sr, dim3((B.n_slots + kStepBlock - 1) / kStepBlock), dim3(kStepBlock),
+                     0, st, B);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_compact_ordered(B.run_list, 0, &B.counts[0], B.f_sr, B.sr_list, &B.counts[3], st);
+}
+
+// Phase C: dt, D, Ermak-McCammon move, overlap, new pair distances.
+__global__ void k_replay_move(StepParams P, ForceSys S, ReplayBufs B, const double* nt,
+                              const double* nr, int n_sr) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_sr) return;
+  const int s = B.sr_list[w];
+  double* pos = B.pos + 3 * (size_t)s;
+  double* q = B.quat + 4 * (size_t)s;
+  const double r = norm3(pos[0], pos[1], pos[2]);
+  const double dt = adaptive_dt(P, r);
+  const double D = diffusion(P, r);
+  B.dt[w] = dt;
+  bool rej;
+  const bool any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt + 3 * (size_t)w,
+                            nr + 3 * (size_t)w, dt, D, B.x0 + (size_t)w * PST_MAX_PAIRS,
+                            B.x1 + (size_t)w * PST_MAX_PAIRS, &rej);
+  if (rej) atomicAdd(&B.counts[6], 1);
+  if (any) atomicOr(&B.counts[4], 1);
+}
+
+hipError_t replay_move(const StepParams& P, const ForceSys& S, const ReplayBufs& B,
+                       const double* noise_t, const double* noise_r, int n_sr, hipStream_t st) {
+  if (n_sr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_replay_move, dim3((n_sr + kStepBlock - 1) / kStepBlock), dim3(kStepBlock),
+                     0, st, P, S, B, noise_t, noise_r, n_sr);
+  return hipGetLastError();
+}
+
+// Phase D: bridge with host uniforms (only drawn when any pair is above on
+// both ends), then n_steps += 1 for every still-running trajectory.
+__global__ void k_replay_bridge(StepParams P, ReplayBufs B, const double* u_bb, int n_sr) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_sr) return;
+  const int s = B.sr_list[w];
+  if (u_bb && P.n_pairs) {
+    const int n = bridge_count(P, B.x0 + (size_t)w * PST_MAX_PAIRS,
+                               B.x1 + (size_t)w * PST_MAX_PAIRS, B.dt[w],
+                               u_bb + (size_t)w * P.n_pairs);
+    if (n >= P.n_needed && B.status[s] == PST_RUNNING) {
+      B.status[s] = PST_REACTED;
+      atomicAdd(&B.counts[5], 1);
+    }
+  }
+  B.n_steps[s] += 1;
+}
+
+hipError_t replay_bridge(const StepParams& P, const ReplayBufs& B, const double* u_bb, int n_sr,
+                         hipStream_t st) {
+  if (n_sr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_replay_bridge, dim3((n_sr + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B, u_bb, n_sr);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// production kernels
+// ----------------------------------------------------------------------------
+// Start trajectory t in slot s: centroid uniformly on the b-sphere, uniform
+// orientation (gpu_batch_simulator.py:509-513, Philox stream RNG_INIT).
+__device__ inline void start_traj(const StepParams& P, const ProdBufs& B, int s, long long t) {
+  double z0, z1, z2, z3, u0, u1, u2, u3;
+  philox_normal2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 0, z0, z1);
+  philox_normal2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 1, z2, z3);
+  philox_uniform2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 2, u0, u1);
+  philox_uniform2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 3, u2, u3);
+  const double n = norm3(z0, z1, z2);
+  double* pos = B.pos + 3 * (size_t)s;
+  pos[0] = z0 / n * P.r_b;
+  pos[1] = z1 / n * P.r_b;
+  pos[2] = z2 / n * P.r_b;
+  shoemake(u0, u1, u2, B.quat + 4 * (size_t)s);
+  B.traj[s] = t;
+  B.n_steps[s] = 0;
+  B.status[s] = PST_RUNNING;
+}
+
+__device__ inline void finish_traj(const StepParams& P, const ProdBufs& B, int s, int outcome) {
+  const long long t = B.traj[s];
+  if (outcome == PST_REACTED) atomicAdd(&B.ctr->n_reacted, 1ull);
+  if (outcome == PST_ESCAPED) atomicAdd(&B.ctr->n_escaped, 1ull);
+  if (outcome == PST_MAXSTEPS) atomicAdd(&B.ctr->n_maxsteps, 1ull);
+  if (B.out_status) B.out_status[t] = (signed char)outcome;
+  if (B.out_steps) B.out_steps[t] = (int)B.n_steps[s];
+  const unsigned long long nt = atomicAdd(&B.ctr->next_traj, 1ull);
+  if ((long long)nt < P.n_total) {
+    start_traj(P, B, s, (long long)nt);
+  } else {
+    B.status[s] = PST_IDLE;
+  }
+}
+
+__global__ void k_prod_init(StepParams P, ProdBufs B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.n_slots) return;
+  if (s < P.n_total) {
+    start_traj(P, B, s, s);
+  } else {
+    B.status[s] = PST_IDLE;
+  }
+  if (s == 0) {
+    RunCounters z = {};
+    z.next_traj = (unsigned long long)min((long long)B.n_slots, P.n_total);
+    *B.ctr = z;
+  }
+}
+
+hipError_t prod_init(const StepParams& P, const ProdBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_prod_init, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B);
+  return hipGetLastError();
+}
+
+// Wave-aggregated append to an unordered list.
+__device__ inline void wave_append(int* list, int* counter, int v, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// Start of step: stop test, reaction check, escape / return, refill,
+// partition the still-running slots into the grid list and the surrogate list.
+__global__ void k_prod_pre(StepParams P, ProdBufs B, int par) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const RunCounters* C = B.ctr;
+  const bool stop = C->stop != 0;
+  bool live = false, far = false;
+  if (s < B.n_slots && !stop && B.status[s] == PST_RUNNING) {
+    for (int guard = 0; guard < 8; ++guard) {
+      double* pos = B.pos + 3 * (size_t)s;
+      double* q = B.quat + 4 * (size_t)s;
+      const long long t = B.traj[s];
+      const unsigned int ns = B.n_steps[s];
+      if ((long long)ns >= P.max_steps) {
+        finish_traj(P, B, s, PST_MAXSTEPS);
+        if (B.status[s] != PST_RUNNING) break;
+        continue;
+      }
+      if (reacted_at(P, pos, q)) {
+        finish_traj(P, B, s, PST_REACTED);
+        if (B.status[s] != PST_RUNNING) break;
+        continue;
+      }
+      const double r = norm3(pos[0], pos[1], pos[2]);
+      if (r >= P.r_esc) {
+        double u, unused;
+        philox_uniform2(P.seed, (uint64_t)t, ns, RNG_ESC, 0, u, unused);
+        if (u < P.return_prob) {
+          pos[0] = pos[0] / r * P.r_b;
+          pos[1] = pos[1] / r * P.r_b;
+          pos[2] = pos[2] / r * P.r_b;
+          double u0, u1, u2, u3;
+          philox_uniform2(P.seed, (uint64_t)t, ns, RNG_RETQ, 0, u0, u1);
+          philox_uniform2(P.seed, (uint64_t)t, ns, RNG_RETQ, 1, u2, u3);
+          shoemake(u0, u1, u2, q);
+          atomicAdd(&B.ctr->n_returns, 1ull);
+        } else {
+          finish_traj(P, B, s, PST_ESCAPED);
+          if (B.status[s] != PST_RUNNING) break;
+          continue;
+        }
+      }
+      live = true;
+      far = P.surrogate_r > 0 && norm3(pos[0], pos[1], pos[2]) > P.surrogate_r;
+      break;
+    }
+  }
+  RunCounters* Cw = B.ctr;
+  wave_append(B.plain_list, &Cw->n_sr, s, live && !far);
+  wave_append(B.mlp_list, &Cw->n_mlp, s, live && far);
+}
+
+// Move every still-running slot (both lists).
+__global__ void k_prod_post(StepParams P, ForceSys S, ProdBufs B, int par) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_plain = B.ctr->n_sr, n_mlp = B.ctr->n_mlp;
+  if (w >= n_plain + n_mlp) return;
+  const int s = w < n_plain ? B.plain_list[w] : B.mlp_list[w - n_plain];
+  double* pos = B.pos + 3 * (size_t)s;
+  double* q = B.quat + 4 * (size_t)s;
+  const long long t = B.traj[s];
+  const unsigned int ns = B.n_steps[s];
+  const double r = norm3(pos[0], pos[1], pos[2]);
+  const double dt = adaptive_dt(P, r);
+  const double D = diffusion(P, r);
+  B.dt[s] = dt;
+  double nt[3], nr[3], extra;
+  philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 0, nt[0], nt[1]);
+  philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 1, nt[2], nr[0]);
+  philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 2, nr[1], nr[2]);
+  (void)extra;
+  bool rej;
+  const bool any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt,
+                            D, B.x0 + (size_t)s * PST_MAX_PAIRS, B.x1 + (size_t)s * PST_MAX_PAIRS,
+                            &rej);
+  if (rej) atomicAdd(&B.ctr->n_overlap, 1ull);
+  const unsigned long long m = __ballot(any);
+  if (m && __lane_id() == __ffsll((long long)m) - 1) atomicOr(&B.ctr->any_above[par], 1);
+}
+
+// Bridge decision, n_steps += 1, finish + refill of bridged trajectories.
+__global__ void k_prod_bridge(StepParams P, ProdBufs B, int par) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  RunCounters* C = B.ctr;
+  const int n_plain = C->n_sr, n_mlp = C->n_mlp;
+  if (w == 0) atomicAdd(&C->total_steps, (unsigned long long)(n_plain + n_mlp));
+  if (w >= n_plain + n_mlp) return;
+  const int s = w < n_plain ? B.plain_list[w] : B.mlp_list[w - n_plain];
+  const long long t = B.traj[s];
+  const unsigned int ns = B.n_steps[s];
+  bool react = false;
+  if (P.n_pairs && C->any_above[par]) {
+    double u[PST_MAX_PAIRS];
+    for (int p = 0; p < P.n_pairs; p += 2) {
+      double a, b;
+      philox_uniform2(P.seed, (uint64_t)t, ns, RNG_BRIDGE, (uint32_t)(p >> 1), a, b);
+      u[p] = a;
+      if (p + 1 < PST_MAX_PAIRS) u[p + 1] = b;
+    }
+    const int n = bridge_count(P, B.x0 + (size_t)s * PST_MAX_PAIRS,
+                               B.x1 + (size_t)s * PST_MAX_PAIRS, B.dt[s], u);
+    react = n >= P.n_needed;
+  }
+  B.n_steps[s] = ns + 1;
+  if (react) {
+    atomicAdd(&C->n_bridge, 1ull);
+    finish_traj(P, B, s, PST_REACTED);
+  }
+}
+
+// Between steps: decide the early stop (:1347-1348) and reset the per-step
+// counters for the next step.
+__global__ void k_prod_stop(StepParams P, RunCounters* C, int par) {
+  const unsigned long long done = C->n_reacted + C->n_escaped;
+  if ((long long)done >= P.stop_done) C->stop = 1;
+  C->n_sr = 0;
+  C->n_mlp = 0;
+  C->any_above[par] = 0;
+}
+
+hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStream_t st) {
+  hipLaunchKernelGGL(k_prod_stop, dim3(1), dim3(1), 0, st, P, B.ctr, par);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prod_pre, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B, par);
+  return hipGetLastError();
+}
+
+hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs& B, int par,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_prod_post, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, S, B, par);
+  return hipGetLastError();
+}
+
+hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st) {
+  hipLaunchKernelGGL(k_prod_bridge, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B, par);
+  return hipGetLastError();
+}
+
+}  // namespace pst

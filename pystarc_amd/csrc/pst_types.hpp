@@ -1,0 +1,123 @@
+// Plain-old-data structures shared by the host C-ABI layer and the gfx950
+// kernels.  Everything here is passed to kernels by value.
+#pragma once
+#include <stdint.h>
+
+#define PST_MAX_GRIDS 4
+#define PST_MAX_PAIRS 16
+
+// Trajectory status codes (pystarc/simulation/gpu_batch_simulator.py:514).
+enum { PST_RUNNING = 0, PST_REACTED = 1, PST_ESCAPED = 2, PST_MAXSTEPS = 3, PST_IDLE = 4 };
+
+// One APBS-style grid, C order (i*ny + j)*nz + k, z fastest
+// (pystarc/forces/electrostatic/grid_force.py:144).  lo/hi are the
+// 3-spacing BC-safe box of gpu_batch_engine.py:267-273.
+struct GridDev {
+  const void* data;
+  int dtype;  // 0 = float32, 1 = float64
+  int nx, ny, nz;
+  double o[3];
+  double sp[3];
+  double inv[3];
+  double lo[3];
+  double hi[3];
+};
+
+// Everything the force kernels need about the system.
+struct ForceSys {
+  GridDev elec[PST_MAX_GRIDS];
+  GridDev born[PST_MAX_GRIDS];
+  GridDev lig_born;
+  int n_elec, n_born, has_rev;
+  double alpha;
+  // Yukawa multipole far-field (gpu_batch_engine.py:161-175, :393-472)
+  int has_yukawa, has_dip, has_quad;
+  double V_fac, debye, fpe;
+  double dip[3];
+  double quad[9];
+  // ligand, body frame about the centroid
+  int M;
+  const double* lig_body;   // (M,3)
+  const double* lig_q;      // (M,)
+  const float* lig_f4;      // (M,4) x,y,z,q in float
+  double cq_body[3];        // centroid of charged atoms, body frame
+  // receptor atoms for reverse Born (and overlap)
+  int n_rec;
+  const double* rec_pos;    // (Nr,3)
+  const double* rec_q;      // (Nr,)
+  const float* rec_f4;      // (Nr,4)
+  double rec_max_r;         // max |r_rec|
+  double rev_radius;        // max(|lo_x|,|hi_x|) of the ligand Born grid (:623)
+  double rev_cull;          // exact cull: largest distance from the ligand centre to the valid box
+  int torque_mode;          // 0 = reference (zero), 1 = lever arm about charged centroid
+};
+
+// Work description of one force launch.
+struct ForceWork {
+  const int* list;          // trajectory indices, or null for 0..n-1
+  const int* n_dev;         // device count, or null
+  int n;                    // host count (used when n_dev is null)
+  // pose source: pose mode reads pos/quat; positions mode reads lig_pos (N,M,3)
+  const double* pos;        // (N,3) centroids
+  const double* quat;       // (N,4) or null
+  const double* rot;        // (N,9) or null
+  const double* lig_pos;    // (N,M,3) or null
+  const double* lig_q;      // (M,) charges for positions mode (else sys.lig_q)
+  double* F;                // (N,3)
+  double* T;                // (N,3)
+  double* E;                // (N,)
+  // reverse-Born skip: 0 = exact geometric cull per trajectory,
+  // 1 = the reference's chunked skip using chunk_min_r
+  int rev_skip_mode;
+  const double* chunk_min_r;
+  int chunk_fb;             // force-batch size
+  int chunk_cs;             // chunk size within a force batch
+  int chunks_per_fb;
+  int out_compact;          // 1: write outputs at list position, 0: at trajectory index
+};
+
+// Step parameters (gpu_batch_simulator.py:480-781).
+struct StepParams {
+  int64_t n_total;       // trajectories in the run
+  int64_t n_slots;       // concurrent slots
+  uint64_t seed;
+  double r_b, r_esc, return_prob;
+  double D_t, D_r;
+  int use_hi;
+  double hi_Df, hi_ainv, hi_a2;
+  double adt_debye, adt_V0;
+  int adt_has_force;
+  double min_core_dt, max_dt;
+  int use_overlap;
+  double overlap_thresh;
+  int n_pairs, n_needed, has_bb_deff;
+  double rec_gho[PST_MAX_PAIRS * 3];
+  double lig_gho[PST_MAX_PAIRS * 3];
+  double cutoff[PST_MAX_PAIRS];
+  double bb_deff[PST_MAX_PAIRS];
+  int64_t max_steps;
+  int hydro_mode;        // 0 = scalar D (reference), 1 = two-body 12x12 RPY
+  double a_rec, a_lig, kT_over_eta;
+  double surrogate_r;    // centre distance above which the surrogate is used (<=0: off)
+  int64_t stop_done;     // early stop when done >= this (int(0.995 N))
+};
+
+// Device-side run counters (production mode).
+struct RunCounters {
+  unsigned long long n_reacted;
+  unsigned long long n_escaped;
+  unsigned long long n_maxsteps;
+  unsigned long long total_steps;
+  unsigned long long next_traj;
+  unsigned long long n_returns;
+  unsigned long long n_overlap;
+  unsigned long long n_bridge;
+  unsigned long long audit_n;
+  int n_sr;
+  int n_mlp;
+  int any_above[2];
+  int stop;
+  int pad;
+  double audit_err2;
+  double audit_ref2;
+};

@@ -1,0 +1,82 @@
+"""OpenDX scalar grids (the APBS output format), as
+pystarc/forces/electrostatic/grid_force.py:90-145 reads them: ``counts``,
+``origin``, three ``delta`` rows, then values with z varying fastest, held as
+``data`` (nx, ny, nz) in C order."""
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+
+
+class DXGrid:
+    def __init__(self, origin, delta, data):
+        self.origin = np.asarray(origin, dtype=float)
+        self.delta = np.asarray(delta, dtype=float)
+        self.data = np.asarray(data)
+
+    @property
+    def spacing(self):
+        return np.array([self.delta[i, i] for i in range(3)])
+
+    @classmethod
+    def from_file(cls, path):
+        counts = origin = None
+        deltas = []
+        values = []
+        n_items = None
+        with open(Path(path), "r") as fh:
+            for line in fh:
+                s = line.strip()
+                if not s or s.startswith("#"):
+                    continue
+                if s.startswith("object"):
+                    m = re.search(r"gridpositions\s+counts\s+(\d+)\s+(\d+)\s+(\d+)", s)
+                    if m:
+                        counts = tuple(int(v) for v in m.groups())
+                    m = re.search(r"items\s+(\d+)", s)
+                    if m:
+                        n_items = int(m.group(1))
+                    continue
+                if s.startswith("origin"):
+                    origin = [float(v) for v in s.split()[1:4]]
+                    continue
+                if s.startswith("delta"):
+                    deltas.append([float(v) for v in s.split()[1:4]])
+                    continue
+                if s.startswith("attribute") or s.startswith("component"):
+                    continue
+                try:
+                    values.extend(float(v) for v in s.split())
+                except ValueError:
+                    continue
+        if counts is None or origin is None or len(deltas) != 3:
+            raise ValueError(f"{path}: not an OpenDX gridpositions file")
+        n = counts[0] * counts[1] * counts[2]
+        if n_items is not None and n_items != n:
+            raise ValueError(f"{path}: items {n_items} != counts product {n}")
+        if len(values) < n:
+            raise ValueError(f"{path}: {len(values)} values for {n} grid points")
+        data = np.asarray(values[:n], dtype=np.float64).reshape(counts)
+        return cls(origin, np.array(deltas), data)
+
+    def to_file(self, path, fmt="%.6e"):
+        nx, ny, nz = self.data.shape
+        flat = np.asarray(self.data, dtype=np.float64).ravel()
+        with open(Path(path), "w") as fh:
+            fh.write(f"object 1 class gridpositions counts {nx} {ny} {nz}\n")
+            fh.write("origin " + " ".join(f"{v:.6f}" for v in self.origin) + "\n")
+            for row in self.delta:
+                fh.write("delta " + " ".join(f"{v:.6f}" for v in row) + "\n")
+            fh.write(f"object 2 class gridconnections counts {nx} {ny} {nz}\n")
+            fh.write(f"object 3 class array type double rank 0 items {flat.size} data follows\n")
+            full = flat.size - flat.size % 3
+            if full:
+                np.savetxt(fh, flat[:full].reshape(-1, 3), fmt=fmt)
+            if flat.size % 3:
+                fh.write(" ".join(fmt % v for v in flat[full:]) + "\n")
+            fh.write('attribute "dep" string "positions"\n')
+            fh.write('object "regular positions regular connections" class field\n')
+            fh.write('component "positions" value 1\ncomponent "connections" value 2\n')
+            fh.write('component "data" value 3\n')

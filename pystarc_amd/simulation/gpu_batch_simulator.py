@@ -1,0 +1,451 @@
+"""Drop-in GPUBatchSimulator / GPUBatchResult on the gfx950 step kernels.
+
+Mirrors pystarc/simulation/gpu_batch_simulator.py: constructor
+``GPUBatchSimulator(mol1, mol2, mob, pathway_set, params, batch_engine)``
+(:159-318), ``run() -> GPUBatchResult`` (:462-1594) and the result dataclass
+with its Wilson interval and k_on (:45-141).  The parameters are read from
+``params`` exactly as the reference reads them, including the attributes the
+pipeline attaches with setattr (max_dt, _overlap_check, _kT_scale,
+gpu_force_batch; pystarc/pipeline/pipeline.py:346-353).
+
+Keyword-only extensions (defaults keep the reference's physics):
+  rng          "philox" (default): device counter-based noise, persistent
+               slots refilled from the trajectory counter, hipGraph-captured
+               steps, no host synchronisation inside a launch.
+               "numpy": replay mode - the host draws from
+               numpy.random.default_rng(seed) in the reference's order and
+               shapes and the kernels consume them, reproducing the reference's
+               reaction / escape decisions.
+  n_slots      concurrent trajectories in philox mode (default min(N, 2^20)).
+  precision    force precision "fp64" / "fp32" (default: the engine's).
+  graph_steps  steps per captured hipGraph launch in philox mode.
+  record       keep per-trajectory outcome and step counts (sim_data).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict
+
+import numpy as np
+
+from .._lib import MAX_PAIRS, StepParams, ptr
+from ..constants import CONV, EPS0, MU_REF, OVERLAP_THRESHOLD, QB_FACTOR
+from ..lib.numerical import romberg_integrate
+
+
+def _z_quantile(p):
+    lo, hi = -10.0, 10.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if 0.5 * math.erfc(-mid / math.sqrt(2.0)) < p:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+@dataclass
+class GPUBatchResult:
+    n_trajectories: int
+    n_reacted: int
+    n_escaped: int
+    n_max_steps: int
+    reaction_counts: Dict[str, int]
+    r_start: float
+    r_escape: float
+    dt: float
+    elapsed_sec: float
+    steps_per_sec: float
+    sim_data: Dict = None
+    total_steps: int = 0
+    k_b: float = 0.0
+    return_prob: float = 0.0
+
+    @property
+    def n_completed(self) -> int:
+        return self.n_reacted + self.n_escaped
+
+    @property
+    def reaction_probability(self) -> float:
+        n = self.n_completed
+        return self.n_reacted / n if n > 0 else 0.0
+
+    def reaction_probability_ci(self, confidence: float = 0.95):
+        """Wilson score interval (gpu_batch_simulator.py:70-80)."""
+        n = self.n_completed
+        if n == 0:
+            return (0.0, 1.0)
+        z = _z_quantile(0.5 + confidence / 2.0)
+        p = self.reaction_probability
+        a = z * z / n
+        centre = (p + a / 2.0) / (1.0 + a)
+        half = z * math.sqrt(max(p * (1 - p) / n + a / (4.0 * n), 0.0)) / (1.0 + a)
+        return (max(0.0, centre - half), min(1.0, centre + half))
+
+    def _k_of(self, P, D_rel, k_b):
+        if P <= 0.0:
+            return 0.0
+        if k_b > 0.0:
+            return CONV * k_b * P
+        k_D = 4.0 * math.pi * D_rel * self.r_start
+        beta = self.r_start / self.r_escape
+        return CONV * k_D * P / (1.0 - P * (1.0 - beta))
+
+    def rate_constant(self, D_rel: float, k_b: float = 0.0) -> float:
+        """k_on = CONV k_b P, Smoluchowski form when k_b = 0 (:82-96)."""
+        return self._k_of(self.reaction_probability, D_rel, k_b)
+
+    def rate_constant_ci(self, D_rel: float, k_b: float = 0.0, confidence: float = 0.95):
+        lo, hi = self.reaction_probability_ci(confidence)
+        return (self._k_of(lo, D_rel, k_b), self._k_of(hi, D_rel, k_b))
+
+    def summary(self, D_rel: float, k_b: float = 0.0, confidence: float = 0.95) -> str:
+        k = self.rate_constant(D_rel, k_b)
+        k_lo, k_hi = self.rate_constant_ci(D_rel, k_b, confidence)
+        p_lo, p_hi = self.reaction_probability_ci(confidence)
+        pct = int(confidence * 100)
+        lines = [f"  Trajectories : {self.n_trajectories:,}",
+                 f"  Completed    : {self.n_completed:,}  ({self.n_reacted:,} reacted + "
+                 f"{self.n_escaped:,} escaped)"]
+        if self.n_max_steps:
+            lines.append(f"  Max-steps    : {self.n_max_steps:,}  (excluded)")
+        lines += [f"  P_rxn        : {self.reaction_probability:.6f}  ({pct}% CI: [{p_lo:.6f}, {p_hi:.6f}])",
+                  f"  k_on         : {k:.4e} M-1 s-1",
+                  f"  {pct}% CI     : [{k_lo:.4e}, {k_hi:.4e}] M-1 s-1"]
+        if k > 0:
+            lines.append(f"  log10(k_on)  : {math.log10(k):.3f}")
+        lines += [f"  Wall time    : {self.elapsed_sec:.1f} s",
+                  f"  BD steps/sec : {self.steps_per_sec:,.0f}",
+                  "  Backend      : HIP gfx950 (batch GPU)"]
+        return "\n".join(lines)
+
+
+def _shoemake_host(u):
+    return np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]),
+                     np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
+                     np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]),
+                     np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], axis=1)
+
+
+class GPUBatchSimulator:
+    def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
+                 n_slots=None, precision=None, graph_steps=16, record=False):
+        if rng not in ("philox", "numpy"):
+            raise ValueError("rng must be 'philox' or 'numpy'")
+        self.mol1, self.mol2, self.mob = mol1, mol2, mob
+        self.pathway_set, self.params, self.engine = pathway_set, params, batch_engine
+        self.rng_mode = rng
+        self.graph_steps = int(graph_steps)
+        self.record = bool(record)
+        prec = precision if precision is not None else getattr(batch_engine, "precision", "fp64")
+        if prec not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = prec
+        c0 = mol2.centroid()
+        self._mol2_pos0 = np.asarray(mol2.positions_array() - c0, dtype=np.float64)
+        self._charges = np.asarray(mol2.charges_array(), dtype=np.float64)
+        self._N_lig = len(self._charges)
+        self._use_hi = bool(getattr(params, "hydrodynamic_interactions", False))
+        self._qb_factor = QB_FACTOR
+        # flattened reaction pairs (:186-205)
+        rec_idx, lig_idx, cut = [], [], []
+        for rxn in pathway_set.reactions:
+            for pair in rxn.criteria.pairs:
+                rec_idx.append(pair.mol1_atom_index)
+                lig_idx.append(pair.mol2_atom_index)
+                cut.append(pair.distance_cutoff)
+        self._rec_gho_indices, self._lig_gho_indices = rec_idx, lig_idx
+        self._rxn_cutoffs = np.array(cut, dtype=np.float64)
+        if len(cut) > MAX_PAIRS:
+            raise ValueError(f"at most {MAX_PAIRS} reaction pairs are supported")
+        if pathway_set.reactions:
+            nn = pathway_set.reactions[0].criteria.n_needed
+            self._rxn_n_needed = len(cut) if (nn < 0 or nn > len(cut)) else nn
+        else:
+            self._rxn_n_needed = 1
+        if any(getattr(r, "state_before", None) is not None for r in pathway_set.reactions):
+            raise NotImplementedError("state-machine reactions are not part of this build")
+        rec_pos = np.asarray(mol1.positions_array(), dtype=np.float64)
+        self._rec_gho_pos = rec_pos[rec_idx] if rec_idx else None
+        N = int(params.n_trajectories)
+        self.n_slots = int(n_slots) if n_slots else min(N, 1 << 20)
+        if rng == "numpy":
+            self.n_slots = N
+        self._return_prob = 0.0
+        self._k_b = 0.0
+
+    # -- outer propagator: k_b and return probability (:320-404) ----------------
+    def _compute_return_prob(self) -> float:
+        b = float(self.params.r_start)
+        q_out = float(self.params.r_escape)
+        q_rec = float(self.mol1.total_charge())
+        q_lig = float(self.mol2.total_charge())
+        debye = float(getattr(self.engine, "_debye", 7.858))
+        eps = 78.0 * EPS0
+        D_t = float(self.mob.relative_translational_diffusion())
+        kT_scale = float(getattr(self.params, "_kT_scale", 1.0))
+        if abs(q_rec * q_lig) < 1e-9:
+            self._k_b = 4.0 * math.pi * D_t * b
+            return b / q_out
+        if self._use_hi:
+            a0, a1 = float(self.mob.radius1), float(self.mob.radius2)
+            a2 = 0.5 * (a0 ** 2 + a1 ** 2)
+            ainv = 1.0 / a0 + 1.0 / a1
+            dpre = D_t / ainv
+
+        def integrand(s):
+            if s == 0.0:
+                return 1.0 / D_t
+            r = 1.0 / s
+            w = math.exp(q_rec * q_lig / (4.0 * math.pi * eps * r) * math.exp(-r / debye) / kT_scale)
+            if self._use_hi:
+                return w / (dpre * (ainv - 3.0 / r + 2.0 * a2 / r ** 3))
+            return w / D_t
+
+        try:
+            self._k_b = 4.0 * math.pi / romberg_integrate(integrand, 0.0, 1.0 / b, tol=1e-8)
+            k_q = 4.0 * math.pi / romberg_integrate(integrand, 0.0, 1.0 / q_out, tol=1e-8)
+            rp = self._k_b / k_q if k_q > 0 else 0.5
+            return float(max(0.0, min(1.0, rp)))
+        except (OverflowError, ZeroDivisionError, ValueError):
+            self._k_b = 4.0 * math.pi * D_t * b
+            return b / q_out
+
+    def _step_params(self) -> StepParams:
+        p = self.params
+        P = StepParams()
+        N = int(p.n_trajectories)
+        P.n_total = N
+        P.n_slots = self.n_slots
+        P.seed = int(p.seed if p.seed is not None else 0) & ((1 << 64) - 1)
+        P.r_b = float(p.r_start)
+        P.r_esc = float(p.r_escape)
+        P.return_prob = self._return_prob
+        D_t = float(self.mob.relative_translational_diffusion())
+        P.D_t = D_t
+        P.D_r = float(self.mob.relative_rotational_diffusion())
+        P.use_hi = int(self._use_hi)
+        if self._use_hi:
+            Df = 1.0 / MU_REF
+            a0 = float(getattr(self.mob, "radius1", 0.0))
+            a1 = float(getattr(self.mob, "radius2", 0.0))
+            P.hi_Df = Df
+            P.hi_a2 = 0.5 * (a0 ** 2 + a1 ** 2)
+            P.hi_ainv = (Df / (6.0 * math.pi * a0) + Df / (6.0 * math.pi * a1)) / Df
+        # adaptive-dt constants (:622-636)
+        P.adt_debye = float(getattr(self.engine, "_debye", 7.828))
+        V0 = float(self.mol1.total_charge()) * float(self.mol2.total_charge()) / (4.0 * math.pi * 78.0 * EPS0)
+        P.adt_V0 = V0
+        P.adt_has_force = int(abs(V0) > 1e-9)
+        P.min_core_dt = float(getattr(p, "minimum_core_dt", 0.0))
+        P.max_dt = float(getattr(p, "max_dt", 0.0) or 0.0)
+        P.use_overlap = int(bool(getattr(p, "_overlap_check", True)))
+        P.overlap_thresh = OVERLAP_THRESHOLD
+        n_pairs = len(self._rxn_cutoffs)
+        P.n_pairs = n_pairs
+        P.n_needed = self._rxn_n_needed
+        if n_pairs:
+            # Brownian-bridge D_eff per pair (:643-660)
+            a_r = float(getattr(self.mob, "radius1", 1.0))
+            a_l = float(getattr(self.mob, "radius2", 1.0))
+            a_sum = a_r + a_l if (a_r + a_l) > 0 else 1.0
+            Dtr, Dtl = D_t * a_l / a_sum, D_t * a_r / a_sum
+            Drr = Dtr * 3.0 / (4.0 * a_r ** 2) if a_r > 0 else 0.0
+            Drl = Dtl * 3.0 / (4.0 * a_l ** 2) if a_l > 0 else 0.0
+            lig_body = self._mol2_pos0[self._lig_gho_indices]
+            L_rec = np.linalg.norm(self._rec_gho_pos, axis=1)
+            L_lig = np.linalg.norm(lig_body, axis=1)
+            deff = D_t + Drr * L_rec ** 2 + Drl * L_lig ** 2
+            P.has_bb_deff = 1
+            for k in range(n_pairs):
+                for d in range(3):
+                    P.rec_gho[3 * k + d] = float(self._rec_gho_pos[k, d])
+                    P.lig_gho[3 * k + d] = float(lig_body[k, d])
+                P.cutoff[k] = float(self._rxn_cutoffs[k])
+                P.bb_deff[k] = float(deff[k])
+        P.max_steps = int(p.max_steps)
+        P.hydro_mode = 0
+        P.a_rec = float(getattr(self.mob, "radius1", 0.0))
+        P.a_lig = float(getattr(self.mob, "radius2", 0.0))
+        P.surrogate_r = -1.0
+        P.stop_done = int(N * 0.995)
+        return P
+
+    def _make_sim(self, P):
+        eng = self.engine
+        eng.set_ligand(self._mol2_pos0, self._charges)
+        h = C.c_void_p()
+        prec = 0 if self.precision == "fp64" else 1
+        eng.ctx.check(eng.ctx.lib.pst_sim_create(eng.ctx.h, C.byref(P), prec, C.byref(h)),
+                      "pst_sim_create")
+        return h
+
+    # -- run -----------------------------------------------------------------------
+    def run(self) -> GPUBatchResult:
+        self._return_prob = self._compute_return_prob()
+        P = self._step_params()
+        if self.rng_mode == "numpy":
+            return self._run_replay(P)
+        return self._run_philox(P)
+
+    def _result(self, N, n_r, n_e, n_m, total, elapsed, sim_data):
+        return GPUBatchResult(
+            n_trajectories=N, n_reacted=int(n_r), n_escaped=int(n_e), n_max_steps=int(n_m),
+            reaction_counts={"stage_0": int(n_r)}, r_start=float(self.params.r_start),
+            r_escape=float(self.params.r_escape), dt=float(getattr(self.params, "dt", 0.2)),
+            elapsed_sec=elapsed, steps_per_sec=total / elapsed if elapsed > 0 else 0.0,
+            sim_data=sim_data, total_steps=int(total), k_b=self._k_b, return_prob=self._return_prob)
+
+    def _run_philox(self, P) -> GPUBatchResult:
+        eng = self.engine
+        lib, ctx = eng.ctx.lib, eng.ctx
+        h = self._make_sim(P)
+        try:
+            ctx.check(lib.pst_prod_init(h, int(self.record)), "pst_prod_init")
+            c16 = np.zeros(16)
+            t0 = time.time()
+            steps = 0
+            chunk = max(self.graph_steps, 1) * 64
+            while steps < P.max_steps:
+                todo = int(min(chunk, P.max_steps - steps))
+                ctx.check(lib.pst_prod_run(h, todo, self.graph_steps, ptr(c16)), "pst_prod_run")
+                steps += int(c16[8])
+                if c16[9] or (c16[10] + c16[11]) == 0 or int(c16[8]) < todo:
+                    break
+            elapsed = time.time() - t0
+            N = int(P.n_total)
+            n_r, n_e = int(c16[0]), int(c16[1])
+            n_m = N - n_r - n_e
+            sim_data = {"n_returns": int(c16[5]), "n_overlap_rejected": int(c16[6]),
+                        "n_bridge": int(c16[7]), "bd_steps": steps}
+            if self.record:
+                st = np.zeros(N, dtype=np.int8)
+                ns = np.zeros(N, dtype=np.int32)
+                ctx.check(lib.pst_prod_outcomes(h, ptr(st), ptr(ns)), "pst_prod_outcomes")
+                sim_data["outcome"] = st
+                sim_data["n_steps"] = ns
+            return self._result(N, n_r, n_e, n_m, c16[3], elapsed, sim_data)
+        finally:
+            lib.pst_sim_destroy(h)
+
+    def replay_driver(self) -> "ReplayDriver":
+        """Step-by-step replay handle (run() drives it to completion)."""
+        self._return_prob = self._compute_return_prob()
+        return ReplayDriver(self, self._step_params())
+
+    def _run_replay(self, P) -> GPUBatchResult:
+        drv = ReplayDriver(self, P)
+        try:
+            t0 = time.time()
+            while drv.step():
+                pass
+            elapsed = time.time() - t0
+            st, ns = drv.status_steps()
+            N = int(P.n_total)
+            n_r, n_e = int((st == 1).sum()), int((st == 2).sum())
+            return self._result(N, n_r, n_e, int((st == 0).sum()), drv.total_steps, elapsed,
+                                {"outcome": st, "n_steps": ns})
+        finally:
+            drv.close()
+
+
+class ReplayDriver:
+    """One reference loop iteration per step() with host NumPy draws in the
+    reference's order (gpu_batch_simulator.py:472-513, :783-1356)."""
+
+    def __init__(self, sim: GPUBatchSimulator, P: StepParams):
+        self.sim = sim
+        self.P = P
+        eng = sim.engine
+        self.lib, self.ctx = eng.ctx.lib, eng.ctx
+        self.N = int(P.n_total)
+        self.rng = np.random.default_rng(sim.params.seed)
+        v = self.rng.standard_normal((self.N, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        pos = np.ascontiguousarray(v * float(P.r_b))
+        q = np.ascontiguousarray(_shoemake_host(self.rng.random((self.N, 3)).astype(np.float64)))
+        self.h = sim._make_sim(P)
+        self.ctx.check(self.lib.pst_sim_set_state(self.h, self.N, ptr(pos), ptr(q)),
+                       "pst_sim_set_state")
+        self.total_steps = 0
+        self.n_done = 0
+        self.stall_done = 0
+        self.step_index = 0
+        self.last = {}
+
+    def step(self) -> bool:
+        """Returns False when the reference loop would stop."""
+        lib, ctx, rng, h = self.lib, self.ctx, self.rng, self.h
+        P = self.P
+        if self.step_index >= int(P.max_steps):
+            return False
+        a, b_, c_ = C.c_int(), C.c_int(), C.c_int()
+        ctx.check(lib.pst_replay_begin(h, C.byref(a), C.byref(b_), C.byref(c_)), "pst_replay_begin")
+        n_run, n_esc, n_react = a.value, b_.value, c_.value
+        if n_run == 0:
+            return False
+        self.n_done += n_react
+        u_outer = u_quat = None
+        n_ret = 0
+        if n_esc:
+            u_outer = rng.random(n_esc)
+            n_ret = int((u_outer < self.sim._return_prob).sum())
+            if n_ret:
+                u_quat = np.ascontiguousarray(rng.random((n_ret, 3)).astype(np.float64))
+            self.n_done += n_esc - n_ret
+        ctx.check(lib.pst_replay_escape(h, ptr(u_outer), n_esc, ptr(u_quat), n_ret, C.byref(a)),
+                  "pst_replay_escape")
+        n_sr = a.value
+        self.last = {"n_run": n_run, "n_reacted_start": n_react, "n_return": n_ret,
+                     "n_escape": n_esc - n_ret, "n_sr": n_sr, "bridge": False, "n_bridge": 0}
+        step = self.step_index
+        self.step_index += 1
+        if n_sr == 0:
+            return self.step_index < int(P.max_steps)
+        nt = rng.standard_normal((n_sr, 3))
+        nr = rng.standard_normal((n_sr, 3))
+        ctx.check(lib.pst_replay_move(h, ptr(nt), ptr(nr), n_sr, C.byref(a), C.byref(b_)),
+                  "pst_replay_move")
+        self.last["n_overlap"] = b_.value
+        u_bb = None
+        if P.n_pairs and a.value:
+            u_bb = np.ascontiguousarray(rng.random((n_sr, int(P.n_pairs))))
+            self.last["bridge"] = True
+        ctx.check(lib.pst_replay_bridge(h, ptr(u_bb), n_sr, C.byref(a)), "pst_replay_bridge")
+        self.last["n_bridge"] = a.value
+        self.n_done += a.value
+        self.total_steps += n_sr
+        if self.n_done >= int(self.N * 0.995):
+            return False
+        if step > 0 and step % 10000 == 0:
+            if self.n_done > int(self.N * 0.95) and self.n_done - self.stall_done < max(2, int(self.N * 0.001)):
+                return False
+            self.stall_done = self.n_done
+        return self.step_index < int(P.max_steps)
+
+    def forces(self, n_sr):
+        F = np.zeros((n_sr, 3))
+        self.ctx.check(self.lib.pst_replay_forces(self.h, ptr(F), n_sr), "pst_replay_forces")
+        return F
+
+    def state(self):
+        N = self.N
+        pos = np.zeros((N, 3))
+        q = np.zeros((N, 4))
+        st = np.zeros(N, dtype=np.int32)
+        ns = np.zeros(N, dtype=np.int64)
+        self.ctx.check(self.lib.pst_sim_get_state(self.h, ptr(pos), ptr(q), ptr(st), ptr(ns)),
+                       "pst_sim_get_state")
+        return pos, q, st, ns
+
+    def status_steps(self):
+        _, _, st, ns = self.state()
+        return st, ns
+
+    def close(self):
+        if self.h:
+            self.lib.pst_sim_destroy(self.h)
+            self.h = None

@@ -1,0 +1,240 @@
+"""Synthetic benchmark systems for the batch BD step.
+
+Five workloads are defined on the path (two charged spheres; a 3,220-atom
+receptor with an 18-atom ligand; a 1,700 / 1,400-atom protein pair; the same
+pair or receptor with a learned surrogate).  This module builds the first
+three as the reference's own objects would carry them: receptor / ligand
+``Molecule``s, a ``MobilityTensor``, a ``PathwaySet`` of contact pairs,
+``NAMParameters`` with the pipeline's extra attributes, the keyword arguments
+of ``GPUBatchForceEngine`` and the grid specifications.
+
+Atom clouds: a 2.8 A Poisson-disc minimum separation cannot hold 3,220 atoms
+in a 22x20x18 A ellipsoid (nor 1,700 / 1,400 atoms in 17 / 15 A spheres), so
+atoms are drawn from a seeded jittered cubic lattice filling the stated
+volume; the minimum separation that results is recorded in ``meta``.
+Grids are evaluated on the device by ``pst_make_grid``: a Debye-Hueckel sum
+(eps_out 78, debye length 7.86 A at 0.15 M) for potentials and an analytic
+Born field (eps_in 4, eps_out 78) for desolvation.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List
+
+import numpy as np
+
+from .constants import ALPHA_DEFAULT
+from .forces.electrostatic.grid_force import DXGrid
+from .forces.multipole_farfield import MultipoleExpansion
+from .hydrodynamics.rotne_prager import MobilityTensor
+from .pathways.reaction_interface import PathwaySet, ReactionInterface
+from .simulation.nam_simulator import NAMParameters
+from .structures.molecules import ContactPair, Molecule, ReactionCriteria
+
+DEBYE_015M = 7.86
+
+
+@dataclass
+class GridSpec:
+    role: str              # "elec" | "born" | "lig_born"
+    kind: int              # 0 = Debye-Hueckel potential, 1 = Born field
+    shape: tuple
+    origin: np.ndarray
+    spacing: float
+    dtype: str             # "float32" | "float64"
+    atoms: np.ndarray      # (n,5): x, y, z, charge, radius
+    debye: float = DEBYE_015M
+    eps_out: float = 78.0
+    eps_in: float = 4.0
+
+
+@dataclass
+class BDSystem:
+    name: str
+    mol1: Molecule
+    mol2: Molecule
+    mob: MobilityTensor
+    pathway_set: PathwaySet
+    params: NAMParameters
+    engine_kwargs: Dict
+    grid_specs: List[GridSpec]
+    meta: Dict = field(default_factory=dict)
+
+
+def lattice_cloud(n, semi_axes, rng, jitter=0.2):
+    """n points in an ellipsoid from a jittered cubic lattice, centroid at 0."""
+    a = np.asarray(semi_axes, dtype=float)
+    vol = 4.0 / 3.0 * math.pi * a.prod()
+    s = (vol / (n * 1.12)) ** (1.0 / 3.0)
+    for _ in range(60):
+        ext = [np.arange(-ai, ai + s, s) for ai in a]
+        g = np.stack(np.meshgrid(*ext, indexing="ij"), axis=-1).reshape(-1, 3)
+        g = g - g.mean(axis=0)
+        inside = ((g / a) ** 2).sum(axis=1) <= 1.0
+        if inside.sum() >= n:
+            break
+        s *= 0.97
+    sites = g[inside]
+    pick = rng.choice(len(sites), size=n, replace=False)
+    pts = sites[pick] + rng.uniform(-jitter, jitter, size=(n, 3))
+    pts -= pts.mean(axis=0)
+    return pts, s
+
+
+def charges_with_net(n, sigma, net, rng):
+    q = rng.normal(0.0, sigma, size=n)
+    return q + (net - q.sum()) / n
+
+
+def min_separation(p):
+    """Smallest pairwise distance (cell-free O(n^2) in blocks)."""
+    best = np.inf
+    for i in range(0, len(p), 512):
+        d = np.linalg.norm(p[i:i + 512, None, :] - p[None, :, :], axis=2)
+        for k in range(d.shape[0]):
+            d[k, i + k] = np.inf
+        best = min(best, float(d.min()))
+    return best
+
+
+def _atoms5(pos, q, rad):
+    return np.column_stack([pos, q, rad]).astype(np.float64)
+
+
+def _cube_origin(n, h):
+    return np.full(3, -0.5 * (n - 1) * h)
+
+
+def _params(n_traj, b, seed, max_steps, overlap, hi=False, min_core_dt=0.2):
+    p = NAMParameters(n_trajectories=n_traj, r_start=b, r_escape=2.0 * b, seed=seed,
+                      max_steps=max_steps, hydrodynamic_interactions=hi,
+                      minimum_core_dt=min_core_dt)
+    p.max_dt = 0.0
+    p._overlap_check = overlap
+    p._kT_scale = 1.0
+    p.gpu_force_batch = 0
+    return p
+
+
+def two_spheres(n_traj=10_000, seed=11111113, grid_n=97, grid_h=0.5, max_steps=1_000_000,
+                debye=DEBYE_015M):
+    """Config 1: +1 e sphere (R 10 A) and -1 e sphere (R 2 A), reaction at
+    centre distance < 12.5 A, b = 30, r_esc = 60, FP64 grid of the analytic
+    screened Coulomb field, 0.2 ps dt floor.  Each molecule carries a charge-free
+    ghost atom at its centre that defines the reaction pair."""
+    z = np.zeros((2, 3))
+    mol1 = Molecule.from_arrays(z, [1.0, 0.0], [10.0, 0.0], name="receptor")
+    mol2 = Molecule.from_arrays(z, [-1.0, 0.0], [2.0, 0.0], name="ligand")
+    crit = ReactionCriteria(name="contact", pairs=[ContactPair(1, 1, 12.5)], n_needed=1)
+    ps = PathwaySet([ReactionInterface("rxn", crit)])
+    mob = MobilityTensor.from_radii(10.0, 2.0)
+    params = _params(n_traj, 30.0, seed, max_steps, overlap=False)
+    rec_atoms = _atoms5(np.zeros((1, 3)), [1.0], [0.0])
+    specs = [GridSpec("elec", 0, (grid_n,) * 3, _cube_origin(grid_n, grid_h), grid_h, "float64",
+                      rec_atoms, debye=debye)]
+    mp = MultipoleExpansion(mol1.positions_array(), mol1.charges_array(), debye, 78.0)
+    kw = dict(alpha=0.0, receptor_charge=1.0, debye_length=debye, sdie=78.0,
+              multipole_expansion=mp)
+    return BDSystem("two_spheres", mol1, mol2, mob, ps, params, kw, specs,
+                    {"config": "two charged spheres", "a_react": 12.5})
+
+
+def _contact_pairs(rec_pos, lig_pos, n_pairs, cutoff):
+    rec_top = np.argsort(-rec_pos[:, 2])[:n_pairs]
+    lig_bot = np.argsort(lig_pos[:, 2])[:n_pairs]
+    return [ContactPair(int(i), int(j), cutoff) for i, j in zip(rec_top, lig_bot)]
+
+
+def protein_ligand(n_traj=1_000_000, seed=2012, geom_seed=2002, n_rec=3220,
+                   rec_axes=(22.0, 20.0, 18.0), rec_grid=(161, 0.5), lig_grid=(65, 0.4),
+                   max_steps=1_000_000):
+    """Config 2: 3,220-atom receptor (net +6 e) and 18-atom planar ligand
+    (net +1 e); receptor potential + Born grids 161^3 at 0.5 A (FP32), ligand
+    Born grid 65^3 at 0.4 A; 2 of 3 contact pairs < 4.0 A; b = 40."""
+    rng = np.random.default_rng(geom_seed)
+    rp, s = lattice_cloud(n_rec, rec_axes, rng)
+    rq = charges_with_net(n_rec, 0.35, 6.0, rng)
+    rr = rng.uniform(1.4, 2.0, n_rec)
+    xs, ys = np.meshgrid(np.linspace(-3.0, 3.0, 6), np.linspace(-2.0, 2.0, 3), indexing="ij")
+    lp = np.column_stack([xs.ravel(), ys.ravel(), np.zeros(18)]) + rng.uniform(-0.1, 0.1, (18, 3))
+    lp -= lp.mean(axis=0)
+    lq = charges_with_net(18, 0.3, 1.0, rng)
+    lr = rng.uniform(1.4, 2.0, 18)
+    mol1 = Molecule.from_arrays(rp, rq, rr, name="receptor")
+    mol2 = Molecule.from_arrays(lp, lq, lr, name="ligand")
+    pairs = _contact_pairs(rp, lp, 3, 4.0)
+    ps = PathwaySet([ReactionInterface("rxn", ReactionCriteria("contact", pairs, n_needed=2))])
+    a_rec = float(np.prod(rec_axes) ** (1.0 / 3.0))
+    mob = MobilityTensor.from_radii(a_rec, 3.5)
+    params = _params(n_traj, 40.0, seed, max_steps, overlap=True)
+    ra = _atoms5(rp, rq, rr)
+    la = _atoms5(lp, lq, lr)
+    gn, gh = rec_grid
+    ln, lh = lig_grid
+    specs = [GridSpec("elec", 0, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", ra),
+             GridSpec("born", 1, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", ra),
+             GridSpec("lig_born", 1, (ln,) * 3, _cube_origin(ln, lh), lh, "float32", la)]
+    mp = MultipoleExpansion(rp, rq, DEBYE_015M, 78.0)
+    kw = dict(alpha=ALPHA_DEFAULT, receptor_charge=float(rq.sum()), debye_length=DEBYE_015M,
+              sdie=78.0, rec_positions=rp, rec_charges=rq, multipole_expansion=mp)
+    return BDSystem("protein_ligand", mol1, mol2, mob, ps, params, kw, specs,
+                    {"lattice_spacing": s, "geom_seed": geom_seed})
+
+
+def protein_protein(n_traj=10_000_000, seed=3013, n_rec=1700, n_lig=1400, r_rec=17.0, r_lig=15.0,
+                    grid=(193, 0.5), max_steps=1_000_000, geom_seeds=(3003, 3004)):
+    """Config 3: 1,700 / 1,400-atom globules (radius 17 / 15 A, net +2 / -6 e);
+    receptor potential + Born and ligand Born grids 193^3 at 0.5 A (FP32);
+    bidirectional Born; scalar-HI with hydrodynamic radii 17 / 15 A; 2 of 5
+    contact pairs < 5.5 A; b = 60, r_esc = 120."""
+    r1 = np.random.default_rng(geom_seeds[0])
+    r2 = np.random.default_rng(geom_seeds[1])
+    rp, s1 = lattice_cloud(n_rec, (r_rec,) * 3, r1)
+    rq = charges_with_net(n_rec, 0.35, 2.0, r1)
+    rr = r1.uniform(1.4, 2.0, n_rec)
+    lp, s2 = lattice_cloud(n_lig, (r_lig,) * 3, r2)
+    lq = charges_with_net(n_lig, 0.35, -6.0, r2)
+    lr = r2.uniform(1.4, 2.0, n_lig)
+    mol1 = Molecule.from_arrays(rp, rq, rr, name="receptor")
+    mol2 = Molecule.from_arrays(lp, lq, lr, name="ligand")
+    pairs = _contact_pairs(rp, lp, 5, 5.5)
+    ps = PathwaySet([ReactionInterface("rxn", ReactionCriteria("contact", pairs, n_needed=2))])
+    mob = MobilityTensor.from_radii(r_rec, r_lig)
+    params = _params(n_traj, 60.0, seed, max_steps, overlap=True, hi=True)
+    gn, gh = grid
+    ra = _atoms5(rp, rq, rr)
+    la = _atoms5(lp, lq, lr)
+    specs = [GridSpec("elec", 0, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", ra),
+             GridSpec("born", 1, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", ra),
+             GridSpec("lig_born", 1, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", la)]
+    mp = MultipoleExpansion(rp, rq, DEBYE_015M, 78.0)
+    kw = dict(alpha=ALPHA_DEFAULT, receptor_charge=float(rq.sum()), debye_length=DEBYE_015M,
+              sdie=78.0, rec_positions=rp, rec_charges=rq, multipole_expansion=mp)
+    return BDSystem("protein_protein", mol1, mol2, mob, ps, params, kw, specs,
+                    {"lattice_spacing": (s1, s2), "geom_seeds": geom_seeds})
+
+
+def build_grids(system: BDSystem, ctx) -> Dict[str, list]:
+    """Evaluate every grid of ``system`` on the device; returns role -> [DXGrid]."""
+    from ._lib import ptr
+    out: Dict[str, list] = {"elec": [], "born": [], "lig_born": []}
+    for gs in system.grid_specs:
+        arr = np.zeros(gs.shape, dtype=np.float32 if gs.dtype == "float32" else np.float64)
+        origin = np.ascontiguousarray(gs.origin, dtype=np.float64)
+        sp = np.full(3, gs.spacing, dtype=np.float64)
+        atoms = np.ascontiguousarray(gs.atoms, dtype=np.float64)
+        rc = ctx.lib.pst_make_grid(ctx.h, gs.kind, *gs.shape, ptr(origin), ptr(sp), ptr(atoms),
+                                   len(atoms), gs.debye, gs.eps_out, gs.eps_in,
+                                   0 if gs.dtype == "float32" else 1, ptr(arr))
+        ctx.check(rc, "pst_make_grid")
+        out[gs.role].append(DXGrid(origin, np.diag(sp), arr))
+    return out
+
+
+def make_engine(system: BDSystem, grids: Dict[str, list], **kw):
+    from .forces.gpu_batch_engine import GPUBatchForceEngine
+    args = dict(system.engine_kwargs)
+    args.update(kw)
+    return GPUBatchForceEngine(grids["elec"], grids["born"], lig_born_grids=grids["lig_born"] or None,
+                               **args)

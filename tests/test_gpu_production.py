@@ -1,0 +1,67 @@
+"""Production step (device Philox, refilling slots, hipGraph): statistics
+against the analytic two-sphere result, determinism, and independence of the
+per-trajectory outcome from slot count and graph capture."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle.physics import wilson_ci  # noqa: E402
+from pystarc_amd import systems as S  # noqa: E402
+from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchSimulator  # noqa: E402
+from tests._systems import dx_grids, numpy_grids, small_protein_ligand, two_spheres_small  # noqa: E402
+from tests.test_oracle_sim import analytic_p_rxn  # noqa: E402
+
+
+def run(sysm, eng, **kw):
+    return GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
+                             **kw).run()
+
+
+def test_two_spheres_k_on_matches_analytic():
+    sysm = two_spheres_small(n_traj=40000)
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+    res = run(sysm, eng)
+    lo, hi = wilson_ci(res.n_reacted, res.n_escaped, 0.999)
+    P = analytic_p_rxn(12.5, 30.0, 60.0, 7.86)
+    assert lo - 0.01 < P < hi + 0.01, (res.reaction_probability, lo, hi, P)
+    assert res.n_completed >= int(0.995 * 40000)
+    assert res.steps_per_sec > 0
+
+
+def test_same_seed_same_result_and_graph_invariance():
+    sysm = small_protein_ligand(n_traj=3000)
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+    a = run(sysm, eng, graph_steps=8, record=True)
+    b = run(sysm, eng, graph_steps=0, record=True)
+    assert (a.n_reacted, a.n_escaped, a.total_steps) == (b.n_reacted, b.n_escaped, b.total_steps)
+    assert np.array_equal(a.sim_data["outcome"], b.sim_data["outcome"])
+    assert np.array_equal(a.sim_data["n_steps"], b.sim_data["n_steps"])
+
+
+def test_slot_count_does_not_change_trajectories():
+    """Noise is keyed by (trajectory, its own step), so a trajectory's fate is
+    the same whether it ran in a pool of N slots or of N/8 refilled slots."""
+    sysm = small_protein_ligand(n_traj=2000)
+    sysm.params.max_steps = 20000
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+    full = run(sysm, eng, record=True)
+    pool = run(sysm, eng, record=True, n_slots=250)
+    o1, o2 = full.sim_data["outcome"], pool.sim_data["outcome"]
+    both = (o1 > 0) & (o2 > 0) & (o1 < 3) & (o2 < 3)
+    assert both.sum() > 1500
+    assert np.array_equal(o1[both], o2[both])
+    assert np.array_equal(full.sim_data["n_steps"][both], pool.sim_data["n_steps"][both])
+
+
+def test_fp32_forces_keep_k_on():
+    sysm = small_protein_ligand(n_traj=6000)
+    g = dx_grids(numpy_grids(sysm))
+    r64 = run(sysm, S.make_engine(sysm, g, precision="fp64"))
+    r32 = run(sysm, S.make_engine(sysm, g, precision="fp32"))
+    lo, hi = wilson_ci(r64.n_reacted, r64.n_escaped, 0.999)
+    assert lo - 0.01 <= r32.reaction_probability <= hi + 0.01
