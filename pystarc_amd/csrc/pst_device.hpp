@@ -145,13 +145,108 @@ __device__ inline void grid_phi_grad(const GridDev& g, double px, double py, dou
   phi = grid_interp<TG, TC>(g, px, py, pz);
 }
 
+// Same phi and central-difference gradient, evaluated from the 20 distinct
+// stencil nodes: the probes at p -+ sp/2 along an axis sit in neighbouring
+// cells with one common fraction f, so
+//   phi(p + h) - phi(p - h) = sum_yz w_yz [(1-f)(v[m+1]-v[m]) + f(v[m+2]-v[m+1])],
+// m = floor(i - 1/2).  Neighbouring grid values are subtracted before they
+// are weighted, which keeps FP32 free of the cancellation of two nearly
+// equal interpolated values, and every node is loaded once.  Callers
+// guarantee the whole stencil is inside the grid (the 3-spacing valid box).
+template <typename TG, typename TC>
+__device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double py, double pz,
+                                             TC& phi, TC& gx, TC& gy, TC& gz) {
+  const double ix = (px - g.o[0]) * g.inv[0];
+  const double iy = (py - g.o[1]) * g.inv[1];
+  const double iz = (pz - g.o[2]) * g.inv[2];
+  const double fi = floor(ix), fj = floor(iy), fk = floor(iz);
+  const int i0 = (int)fi, j0 = (int)fj, k0 = (int)fk;
+  const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
+  const double mi = floor(ix - 0.5), mj = floor(iy - 0.5), mk = floor(iz - 0.5);
+  const TC ex = (TC)(ix - 0.5 - mi), ey = (TC)(iy - 0.5 - mj), ez = (TC)(iz - 0.5 - mk);
+  const int ai = (int)mi - i0, aj = (int)mj - j0, ak = (int)mk - k0;  // -1 or 0
+  const TG* __restrict__ d = (const TG*)g.data;
+  const long sy = g.nz, sx = (long)g.ny * g.nz;
+  const long b = (long)i0 * sx + (long)j0 * sy + k0;
+  // centre cell
+  const TC c000 = (TC)d[b], c001 = (TC)d[b + 1], c010 = (TC)d[b + sy], c011 = (TC)d[b + sy + 1];
+  const TC c100 = (TC)d[b + sx], c101 = (TC)d[b + sx + 1], c110 = (TC)d[b + sx + sy],
+           c111 = (TC)d[b + sx + sy + 1];
+  // extra plane per axis: index -1 (a = -1) or +2 (a = 0)
+  const long xo = ai < 0 ? -sx : 2 * sx, yo = aj < 0 ? -sy : 2 * sy, zo = ak < 0 ? -1 : 2;
+  const TC x00 = (TC)d[b + xo], x01 = (TC)d[b + xo + 1], x10 = (TC)d[b + xo + sy],
+           x11 = (TC)d[b + xo + sy + 1];
+  const TC y00 = (TC)d[b + yo], y01 = (TC)d[b + yo + 1], y10 = (TC)d[b + yo + sx],
+           y11 = (TC)d[b + yo + sx + 1];
+  const TC z00 = (TC)d[b + zo], z01 = (TC)d[b + zo + sy], z10 = (TC)d[b + zo + sx],
+           z11 = (TC)d[b + zo + sx + sy];
+  const TC one = (TC)1;
+  const TC gxw = one - fx, gyw = one - fy, gzw = one - fz;
+  phi = c000 * gxw * gyw * gzw + c100 * fx * gyw * gzw + c010 * gxw * fy * gzw +
+        c001 * gxw * gyw * fz + c110 * fx * fy * gzw + c101 * fx * gyw * fz +
+        c011 * gxw * fy * fz + c111 * fx * fy * fz;
+  // x: nodes along x at (m, m+1, m+2) for the four (y,z) corners of the centre cell
+  {
+    // value at x offset o in {-1,0,1,2} relative to i0, per (jj,kk) corner
+    const TC n0_00 = ai < 0 ? x00 : c000, n1_00 = ai < 0 ? c000 : c100, n2_00 = ai < 0 ? c100 : x00;
+    const TC n0_01 = ai < 0 ? x01 : c001, n1_01 = ai < 0 ? c001 : c101, n2_01 = ai < 0 ? c101 : x01;
+    const TC n0_10 = ai < 0 ? x10 : c010, n1_10 = ai < 0 ? c010 : c110, n2_10 = ai < 0 ? c110 : x10;
+    const TC n0_11 = ai < 0 ? x11 : c011, n1_11 = ai < 0 ? c011 : c111, n2_11 = ai < 0 ? c111 : x11;
+    const TC e1 = one - ex;
+    const TC d00 = e1 * (n1_00 - n0_00) + ex * (n2_00 - n1_00);
+    const TC d01 = e1 * (n1_01 - n0_01) + ex * (n2_01 - n1_01);
+    const TC d10 = e1 * (n1_10 - n0_10) + ex * (n2_10 - n1_10);
+    const TC d11 = e1 * (n1_11 - n0_11) + ex * (n2_11 - n1_11);
+    gx = (d00 * gyw * gzw + d10 * fy * gzw + d01 * gyw * fz + d11 * fy * fz) * (TC)g.inv[0];
+  }
+  {
+    // along y for the four (x,z) corners; y-extra loads are ordered (x0 z0, x0 z1, x1 z0, x1 z1)
+    const TC n0_00 = aj < 0 ? y00 : c000, n1_00 = aj < 0 ? c000 : c010, n2_00 = aj < 0 ? c010 : y00;
+    const TC n0_01 = aj < 0 ? y01 : c001, n1_01 = aj < 0 ? c001 : c011, n2_01 = aj < 0 ? c011 : y01;
+    const TC n0_10 = aj < 0 ? y10 : c100, n1_10 = aj < 0 ? c100 : c110, n2_10 = aj < 0 ? c110 : y10;
+    const TC n0_11 = aj < 0 ? y11 : c101, n1_11 = aj < 0 ? c101 : c111, n2_11 = aj < 0 ? c111 : y11;
+    const TC e1 = one - ey;
+    const TC d00 = e1 * (n1_00 - n0_00) + ey * (n2_00 - n1_00);
+    const TC d01 = e1 * (n1_01 - n0_01) + ey * (n2_01 - n1_01);
+    const TC d10 = e1 * (n1_10 - n0_10) + ey * (n2_10 - n1_10);
+    const TC d11 = e1 * (n1_11 - n0_11) + ey * (n2_11 - n1_11);
+    gy = (d00 * gxw * gzw + d10 * fx * gzw + d01 * gxw * fz + d11 * fx * fz) * (TC)g.inv[1];
+  }
+  {
+    // along z for the four (x,y) corners; z-extra loads are ordered (x0 y0, x0 y1, x1 y0, x1 y1)
+    const TC n0_00 = ak < 0 ? z00 : c000, n1_00 = ak < 0 ? c000 : c001, n2_00 = ak < 0 ? c001 : z00;
+    const TC n0_01 = ak < 0 ? z01 : c010, n1_01 = ak < 0 ? c010 : c011, n2_01 = ak < 0 ? c011 : z01;
+    const TC n0_10 = ak < 0 ? z10 : c100, n1_10 = ak < 0 ? c100 : c101, n2_10 = ak < 0 ? c101 : z10;
+    const TC n0_11 = ak < 0 ? z11 : c110, n1_11 = ak < 0 ? c110 : c111, n2_11 = ak < 0 ? c111 : z11;
+    const TC e1 = one - ez;
+    const TC d00 = e1 * (n1_00 - n0_00) + ez * (n2_00 - n1_00);
+    const TC d01 = e1 * (n1_01 - n0_01) + ez * (n2_01 - n1_01);
+    const TC d10 = e1 * (n1_10 - n0_10) + ez * (n2_10 - n1_10);
+    const TC d11 = e1 * (n1_11 - n0_11) + ez * (n2_11 - n1_11);
+    gz = (d00 * gxw * gyw + d10 * fx * gyw + d01 * gxw * fy + d11 * fx * fy) * (TC)g.inv[2];
+  }
+}
+
+// FP64 follows the reference's seven interpolations literally (parity with
+// gpu_batch_engine.py to rounding); FP32 uses the node-difference stencil.
 template <typename TC>
 __device__ inline void grid_phi_grad_any(const GridDev& g, double px, double py, double pz,
-                                         TC& phi, TC& gx, TC& gy, TC& gz) {
+                                         TC& phi, TC& gx, TC& gy, TC& gz);
+template <>
+__device__ inline void grid_phi_grad_any<double>(const GridDev& g, double px, double py, double pz,
+                                                 double& phi, double& gx, double& gy, double& gz) {
   if (g.dtype == 1)
-    grid_phi_grad<double, TC>(g, px, py, pz, phi, gx, gy, gz);
+    grid_phi_grad<double, double>(g, px, py, pz, phi, gx, gy, gz);
   else
-    grid_phi_grad<float, TC>(g, px, py, pz, phi, gx, gy, gz);
+    grid_phi_grad<float, double>(g, px, py, pz, phi, gx, gy, gz);
+}
+template <>
+__device__ inline void grid_phi_grad_any<float>(const GridDev& g, double px, double py, double pz,
+                                                float& phi, float& gx, float& gy, float& gz) {
+  if (g.dtype == 1)
+    grid_phi_grad_stencil<double, float>(g, px, py, pz, phi, gx, gy, gz);
+  else
+    grid_phi_grad_stencil<float, float>(g, px, py, pz, phi, gx, gy, gz);
 }
 
 template <typename TC> __device__ inline TC pst_exp(TC x);

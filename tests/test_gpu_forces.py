@@ -3,10 +3,12 @@ oracle on identical inputs.
 
 Tolerances (stated per the precision contract):
   FP64 path: |F_gpu - F_oracle| <= 1e-12 * (|F_oracle| + 1e-3 * sum_i |f_i|)
-  FP32 path: |F_gpu - F_oracle| <= 1e-5  * (|F_oracle| + 1e-2 * sum_i |f_i|)
+  FP32 path: |F_gpu - F_oracle| <= 1e-5  * (|F_oracle| + 5e-2 * sum_i |f_i|)
 where sum_i |f_i| is the sum of per-atom force magnitudes (the scale of the
-cancellation in a net force of a near-neutral molecule).  Energies use the
-same form with per-atom |e_i|.
+cancellation in a net force of a near-neutral molecule).  The FP32 floor,
+5e-7 * sum_i |f_i|, is a few FP32 ulps of the per-atom terms: FP32 per-atom
+arithmetic cannot resolve a net force below it whatever the accumulation.
+Energies use the same form with per-atom |e_i|.
 """
 import numpy as np
 import pytest
@@ -72,7 +74,7 @@ def test_grid_generator_matches_numpy():
                                atol=1e-6 * np.abs(ref).max()), role
 
 
-@pytest.mark.parametrize("prec,rel,frac", [("fp64", 1e-12, 1e-3), ("fp32", 1e-5, 1e-2)])
+@pytest.mark.parametrize("prec,rel,frac", [("fp64", 1e-12, 1e-3), ("fp32", 1e-5, 5e-2)])
 def test_positions_mode_protein_ligand(prec, rel, frac):
     sysm = small_protein_ligand()
     g = numpy_grids(sysm)
@@ -97,8 +99,8 @@ def test_pose_mode_equals_positions_mode():
     c, q = poses(200, rng, 6.0, 26.0)
     F0, T0, E0, lab, R, sf, se = oracle_forces(sysm, g, c, q, chunk=0)
     Fp, Tp, Ep = eng.forces_pose(c, q)
-    # per-trajectory exact cull equals the oracle with a chunk of one trajectory
-    # whenever no receptor atom sits in a corner of the ligand box
+    # the exact per-trajectory cull never drops a contributing receptor atom,
+    # so it equals the oracle without any skip
     assert_close(Fp.cpu().numpy(), F0, sf, 1e-12, 1e-3, "pose F")
     assert_close(Ep.cpu().numpy()[:, None], E0[:, None], se, 1e-12, 1e-3, "pose E")
 
@@ -139,7 +141,7 @@ def test_two_spheres_grid_and_yukawa():
     assert_close(E.cpu().numpy()[:, None], E0[:, None], se, 1e-12, 1e-3, "two spheres E")
 
 
-@pytest.mark.parametrize("prec,rel,frac", [("fp64", 1e-12, 1e-3), ("fp32", 1e-5, 1e-2)])
+@pytest.mark.parametrize("prec,rel,frac", [("fp64", 1e-12, 1e-3), ("fp32", 1e-5, 5e-2)])
 def test_protein_protein_bidirectional_born(prec, rel, frac):
     sysm = small_protein_protein()
     g = numpy_grids(sysm)
