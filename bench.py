@@ -136,8 +136,26 @@ def cpu_baseline(sysm, grids, pos, quat, seconds=15.0, n_traj=128):
             "cpu_seconds": time.process_time() - t0}
 
 
+def pmc_mode(args):
+    """For rocprofv3 --pmc runs: a few production steps, `--pmc-launches` fused
+    force launches on the live poses, then one plain device copy of 256 MiB
+    whose FETCH_SIZE / WRITE_SIZE calibrate the counters' byte scale."""
+    import torch
+    sysm, grids = build_system(args)
+    prec = args.paths.split(",")[0]
+    r = run_path(sysm, grids, prec, args)
+    force_roofline(r["eng"], r["pos"], r["quat"], prec, reps=args.pmc_launches)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"pmc mode done: {len(r['pos'])} live poses, copy of {src.numel() * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--pmc", action="store_true", help="counter-collection mode (no JSON line)")
+    ap.add_argument("--pmc-launches", type=int, default=3)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=300)
@@ -156,6 +174,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(2)
+    if args.pmc:
+        pmc_mode(args)
+        return
     sysm, grids = build_system(args)
     results = {}
     for prec in args.paths.split(","):

@@ -36,6 +36,7 @@ struct pst_ctx {
   double* d_body = nullptr;
   double* d_q = nullptr;
   double cq_body[3] = {0, 0, 0};
+  double lig_rmax = 1e300;
   int n_rec = 0;
   double* d_rec = nullptr;
   double* d_rec_q = nullptr;
@@ -135,6 +136,26 @@ static void fill_sys(pst_ctx* ctx, ForceSys& S) {
     for (int i = 0; i < 3; ++i) S.dip[i] = ctx->dip[i];
     for (int i = 0; i < 9; ++i) S.quad[i] = ctx->quad[i];
   }
+  const double lam = ctx->debye > 0 ? ctx->debye : 1.0;
+  S.inv_lam = 1.0 / lam;
+  S.inv_lam2 = 1.0 / (lam * lam);
+  S.c3 = 1.0 / (3.0 * lam * lam);
+  S.c3l = 1.0 / (3.0 * lam * lam * lam);
+  for (int i = 0; i < 3; ++i) S.dip_s[i] = S.dip[i] / S.fpe;
+  for (int i = 0; i < 9; ++i) S.quad_s[i] = S.quad[i] / S.fpe;
+  S.lig_rmax = ctx->lig_rmax;
+  double reach = 0;
+  auto corner_reach = [&](const GridDev& g) {
+    for (int c = 0; c < 8; ++c) {
+      const double x = (c & 1) ? g.hi[0] : g.lo[0];
+      const double y = (c & 2) ? g.hi[1] : g.lo[1];
+      const double z = (c & 4) ? g.hi[2] : g.lo[2];
+      reach = fmax(reach, sqrt(x * x + y * y + z * z));
+    }
+  };
+  for (int i = 0; i < S.n_elec; ++i) corner_reach(S.elec[i]);
+  for (int i = 0; i < S.n_born; ++i) corner_reach(S.born[i]);
+  S.grid_reach = reach * (1.0 + 1e-12) + 1e-9;
   S.M = ctx->M;
   S.lig_body = ctx->d_body;
   S.lig_q = ctx->d_q;
@@ -322,6 +343,11 @@ int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charge
       ++nc;
     }
   for (int d = 0; d < 3; ++d) ctx->cq_body[d] = nc ? c[d] / nc : 0.0;
+  double rmax = 0;
+  for (int i = 0; i < m; ++i)
+    rmax = fmax(rmax, sqrt(body[3 * i] * body[3 * i] + body[3 * i + 1] * body[3 * i + 1] +
+                           body[3 * i + 2] * body[3 * i + 2]));
+  ctx->lig_rmax = rmax * (1.0 + 1e-12) + 1e-9;
   return 0;
 }
 

@@ -256,39 +256,53 @@ template <> __device__ inline float pst_exp<float>(float x) { return __expf(x); 
 // Yukawa monopole + dipole + quadrupole with the reference's radial-only
 // gradient (gpu_batch_engine.py:401-465).  Accumulates F += -q dphi/dr r_hat,
 // E += q phi for an atom at lab position p (receptor at the origin).
+template <typename TC> __device__ inline TC pst_rsqrt(TC x);
+template <> __device__ inline double pst_rsqrt<double>(double x) { return rsqrt(x); }
+template <> __device__ inline float pst_rsqrt<float>(float x) { return rsqrtf(x); }
+
+// Yukawa monopole + dipole + quadrupole with the reference's radial-only
+// gradient (gpu_batch_engine.py:401-465), r clamped to >= 1 A (:404):
+//   phi  = e^{-r/l} [V/r + (p.u)/(fpe r^2)(1 + r/l) + (uQu)/(fpe r^3)(1 + r/l + r^2/(3l^2))]
+//   dphi = e^{-r/l} [V(-1/r^2 - 1/(rl)) + (p.u)/fpe(-2/r^3 - 2/(r^2 l) - 1/(r l^2))
+//          + (uQu)/fpe(-3/r^4 - 3/(r^3 l) - 1/(r^2 l^2) - 1/(3 r l^3))]
+// with u = p/r.  The quadrupole derivative keeps the reference's
+// -1/(r^2 l^2) term.  One reciprocal square root per atom; every other
+// constant is pre-divided on the host.  Accumulates F += -q dphi u, E += q phi.
 template <typename TC>
 __device__ inline void yukawa_atom(const ForceSys& S, TC px, TC py, TC pz, TC q, TC& fx, TC& fy,
                                    TC& fz, TC& e) {
-  const TC lam = (TC)S.debye;
-  const TC r = sqrt(px * px + py * py + pz * pz);
-  const TC sr = r > (TC)1 ? r : (TC)1;
-  const TC ex = pst_exp<TC>(-sr / lam);
-  const TC inv = (TC)1 / sr;
+  const TC r2 = px * px + py * py + pz * pz;
+  TC inv, sr;
+  if (r2 > (TC)1) {
+    inv = pst_rsqrt<TC>(r2);
+    sr = r2 * inv;
+  } else {
+    inv = (TC)1;
+    sr = (TC)1;
+  }
+  const TC il = (TC)S.inv_lam;
+  const TC ex = pst_exp<TC>(-sr * il);
   const TC rx = px * inv, ry = py * inv, rz = pz * inv;
   const TC V = (TC)S.V_fac;
-  TC phi = V * ex * inv;
-  TC dphi = V * ex * (-inv * inv - inv / lam);
+  const TC i2 = inv * inv;
+  TC phi = V * inv;
+  TC dphi = -V * (i2 + inv * il);
   if (S.has_dip) {
-    const TC fpe = (TC)S.fpe;
-    const TC pdr = rx * (TC)S.dip[0] + ry * (TC)S.dip[1] + rz * (TC)S.dip[2];
-    phi += pdr / (fpe * sr * sr) * ((TC)1 + sr / lam) * ex;
-    dphi += pdr / fpe * ex *
-            ((TC)-2 * inv * inv * inv - (TC)2 * inv * inv / lam - inv / (lam * lam));
+    const TC pdr = rx * (TC)S.dip_s[0] + ry * (TC)S.dip_s[1] + rz * (TC)S.dip_s[2];
+    phi += pdr * i2 * ((TC)1 + sr * il);
+    dphi -= pdr * ((TC)2 * i2 * inv + (TC)2 * i2 * il + inv * (TC)S.inv_lam2);
   }
   if (S.has_quad) {
-    const TC fpe = (TC)S.fpe;
-    const TC qx = (TC)S.quad[0] * rx + (TC)S.quad[1] * ry + (TC)S.quad[2] * rz;
-    const TC qy = (TC)S.quad[3] * rx + (TC)S.quad[4] * ry + (TC)S.quad[5] * rz;
-    const TC qz = (TC)S.quad[6] * rx + (TC)S.quad[7] * ry + (TC)S.quad[8] * rz;
+    const TC qx = (TC)S.quad_s[0] * rx + (TC)S.quad_s[1] * ry + (TC)S.quad_s[2] * rz;
+    const TC qy = (TC)S.quad_s[3] * rx + (TC)S.quad_s[4] * ry + (TC)S.quad_s[5] * rz;
+    const TC qz = (TC)S.quad_s[6] * rx + (TC)S.quad_s[7] * ry + (TC)S.quad_s[8] * rz;
     const TC rQr = rx * qx + ry * qy + rz * qz;
-    phi += rQr / (fpe * sr * sr * sr) * ((TC)1 + sr / lam + sr * sr / ((TC)3 * lam * lam)) * ex;
-    const TC i2 = inv * inv;
-    dphi += rQr / fpe * ex *
-            ((TC)-3 * i2 * i2 - (TC)3 * i2 * inv / lam - i2 / (lam * lam) -
-             inv / ((TC)3 * lam * lam * lam));
+    phi += rQr * i2 * inv * ((TC)1 + sr * il + sr * sr * (TC)S.c3);
+    dphi -= rQr * ((TC)3 * i2 * i2 + (TC)3 * i2 * inv * il + i2 * (TC)S.inv_lam2 + inv * (TC)S.c3l);
   }
-  fx -= q * dphi * rx;
-  fy -= q * dphi * ry;
-  fz -= q * dphi * rz;
-  e += q * phi;
+  const TC qd = q * dphi * ex;
+  fx -= qd * rx;
+  fy -= qd * ry;
+  fz -= qd * rz;
+  e += q * phi * ex;
 }

@@ -27,9 +27,11 @@ template <> struct Vec4<float> { typedef float4 type; };
 // finest-first with Yukawa fallback (:289-366) and receptor Born grids (:571-576).
 template <typename TC>
 __device__ inline void atom_direct(const ForceSys& S, double px, double py, double pz, double q,
-                                   TC& fx, TC& fy, TC& fz, TC& e) {
+                                   bool grids, TC& fx, TC& fy, TC& fz, TC& e) {
   bool assigned = false;
-  for (int g = 0; g < S.n_elec; ++g) {
+  const int n_elec = grids ? S.n_elec : 0;
+  const int n_born = grids ? S.n_born : 0;
+  for (int g = 0; g < n_elec; ++g) {
     if (!assigned && grid_inside(S.elec[g], px, py, pz)) {
       assigned = true;
       if (fabs(q) >= 1e-9) {
@@ -44,7 +46,7 @@ __device__ inline void atom_direct(const ForceSys& S, double px, double py, doub
   if (!assigned && S.has_yukawa && q != 0.0)
     yukawa_atom<TC>(S, (TC)px, (TC)py, (TC)pz, (TC)q, fx, fy, fz, e);
   assigned = false;
-  for (int g = 0; g < S.n_born; ++g) {
+  for (int g = 0; g < n_born; ++g) {
     if (!assigned && grid_inside(S.born[g], px, py, pz)) {
       assigned = true;
       if (fabs(q) >= 1e-9) {
@@ -137,6 +139,8 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
         if (cnt > 0) { qx = sx / cnt; qy = sy / cnt; qz = sz / cnt; }
       }
     }
+    // no ligand atom can reach a receptor grid box: skip the per-atom box tests
+    const bool grids = !POSE || norm3(cx, cy, cz) - S.lig_rmax <= S.grid_reach;
     if (valid) {
       for (int i = lane; i < M; i += G) {
         double px, py, pz, q;
@@ -157,7 +161,7 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
           px = lp[0]; py = lp[1]; pz = lp[2]; q = lq[i];
         }
         TC ax = 0, ay = 0, az = 0, ae = 0;
-        atom_direct<TC>(S, px, py, pz, q, ax, ay, az, ae);
+        atom_direct<TC>(S, px, py, pz, q, grids, ax, ay, az, ae);
         fx += ax; fy += ay; fz += az; e += ae;
         if (S.torque_mode == 1) {
           const TC lx = (TC)(px - qx), ly = (TC)(py - qy), lz = (TC)(pz - qz);

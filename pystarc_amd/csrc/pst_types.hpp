@@ -35,6 +35,14 @@ struct ForceSys {
   double V_fac, debye, fpe;
   double dip[3];
   double quad[9];
+  // the same constants pre-divided on the host: 1/debye, 1/debye^2,
+  // 1/(3 debye^2), 1/(3 debye^3), dipole/fpe, quadrupole/fpe
+  double inv_lam, inv_lam2, c3, c3l;
+  double dip_s[3];
+  double quad_s[9];
+  // whole-ligand cull: no ligand atom can be inside any receptor grid box when
+  // |c| - lig_rmax > grid_reach
+  double lig_rmax, grid_reach;
   // ligand, body frame about the centroid
   int M;
   const double* lig_body;   // (M,3)
