@@ -1,18 +1,28 @@
-"""Benchmark: trajectory-steps/s of the batched BD step on one MI355X.
+"""Benchmark: trajectory-steps/s of the batched BD step on one MI355X,
+surrogate vs plain, with k_on of both paths from the same run.
 
 Workload: the 1,700 / 1,400-atom protein-protein pair (193^3 FP32 receptor
 potential + Born grids and ligand Born grid at 0.5 A, bidirectional Born,
-scalar HI, overlap rejection, 2-of-5 contact reaction, b = 60 A).  A "step" is
-one BD step of every live trajectory slot; value = trajectory-steps per second
-over the timed steps (inputs resident in HBM).  Synthetic data from seeded
-generators; random-free physics identical to the reference's step.
+scalar HI, overlap rejection, 2-of-5 contact reaction < 5.5 A, b = 60 A,
+r_esc = 120 A).  A "step" is one BD step of every live trajectory slot;
+value = trajectory-steps per second over the timed steps of the surrogate +
+FP32 path (inputs resident in HBM).  Paths timed on identical seeds:
+
+  fp64       plain grid path, FP64 forces (the reference's arithmetic)
+  fp32       plain grid path, FP32 forces
+  surrogate  MLP beyond contact + 8 A, FP32 grid path inside, per-step audit
+
+k_on: each of fp64 and surrogate then runs --kon-trajectories complete
+trajectories; both P_rxn with Wilson 95 % intervals go in the JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--slots S] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -25,16 +35,18 @@ if ROOT not in sys.path:
 
 METRIC = "trajectory-steps/sec on 1 MI355X, surrogate vs plain at equal k_on (Wilson 95%)"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_TRAJ = 2 * (9 * 256 + 3 * 256 * 256 + 256 * 13)   # 404,480
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_system(args):
+def build_system(args, n_traj=None):
     from pystarc_amd import systems as S
     from pystarc_amd._lib import Context
-    sysm = S.protein_protein(n_traj=args.trajectories, seed=3013)
+    sysm = S.protein_protein(n_traj=n_traj or args.trajectories, seed=3013)
     t0 = time.time()
     ctx = Context(0)
     grids = S.build_grids(sysm, ctx)
@@ -43,24 +55,31 @@ def build_system(args):
     return sysm, grids
 
 
-def run_path(sysm, grids, precision, args):
-    """Production run: warm-up steps, then exactly `steps` timed steps."""
-    import ctypes as C
-    import torch
-    from pystarc_amd import systems as S
-    from pystarc_amd._lib import ptr
+def make_sim(sysm, eng, args, surrogate=None, n_slots=None):
     from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchSimulator
-    eng = S.make_engine(sysm, grids, precision=precision)
-    sim = GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
-                            n_slots=args.slots, graph_steps=args.graph_steps)
+    return GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
+                             n_slots=n_slots or args.slots, graph_steps=args.graph_steps,
+                             surrogate=surrogate, audit_n=args.audit_n, audit_tol=args.audit_tol)
+
+
+def run_window(sysm, eng, args, surrogate=None):
+    """Production run: warm-up steps, then exactly `steps` timed steps."""
+    import torch
+    from pystarc_amd._lib import ptr
+    sim = make_sim(sysm, eng, args, surrogate)
     sim._return_prob = sim._compute_return_prob()
     P = sim._step_params()
     lib, ctx = eng.ctx.lib, eng.ctx
     h = sim._make_sim(P)
     c16 = np.zeros(16)
     ctx.check(lib.pst_prod_init(h, 0), "pst_prod_init")
+    if surrogate is not None:
+        p9 = np.ascontiguousarray(surrogate.sim_params(args.audit_n), dtype=np.float64)
+        ctx.check(lib.pst_sim_set_surrogate(h, surrogate.h, ptr(p9)), "pst_sim_set_surrogate")
     if args.warmup:
         ctx.check(lib.pst_prod_run(h, args.warmup, args.graph_steps, ptr(c16)), "warmup")
+    if surrogate is not None:
+        ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
     base_steps = c16[3]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -69,27 +88,24 @@ def run_path(sysm, grids, precision, args):
     wall = time.perf_counter() - t0
     ms = C.c_double()
     lib.pst_prod_last_ms_per_step(h, C.byref(ms))
-    steps_run = int(c16[8])
-    traj_steps = c16[3] - base_steps
-    # snapshot of the live state for the kernel roofline and the CPU baseline
     S_ = sim.n_slots
     pos = np.zeros((S_, 3)); q = np.zeros((S_, 4)); st = np.zeros(S_, dtype=np.int32)
     ctx.check(lib.pst_sim_get_state(h, ptr(pos), ptr(q), ptr(st), None), "get_state")
     lib.pst_sim_destroy(h)
     live = st == 0
-    return {"eng": eng, "wall_s": wall, "device_ms_per_step": ms.value, "steps": steps_run,
+    traj_steps = c16[3] - base_steps
+    audit = math.sqrt(c16[12] / c16[13]) if c16[13] > 0 else None
+    return {"wall_s": wall, "device_ms_per_step": ms.value, "steps": int(c16[8]),
             "traj_steps": traj_steps, "value": traj_steps / wall if wall > 0 else 0.0,
-            "pos": pos[live], "quat": q[live], "counters": c16.tolist()}
+            "pos": pos[live], "quat": q[live], "counters": c16.tolist(), "audit_rel_rms": audit,
+            "n_surrogate_last_step": int(c16[11])}
 
 
 def force_roofline(eng, pos, quat, precision, reps=20):
-    """Average duration of the fused force kernel on the live poses (HIP events
+    """Average duration of the fused force kernel on given poses (HIP events
     on the context stream) and its algorithmic bytes from an atom census."""
-    import ctypes as C
     import torch
     from pystarc_amd._lib import ptr
-    body = eng._lig_key is not None
-    assert body
     n = len(pos)
     dp = torch.as_tensor(np.ascontiguousarray(pos), device=eng.device)
     dq = torch.as_tensor(np.ascontiguousarray(quat), device=eng.device)
@@ -100,18 +116,51 @@ def force_roofline(eng, pos, quat, precision, reps=20):
     eng.ctx.check(eng.ctx.lib.pst_time_forces_pose(
         eng.ctx.h, n, ptr(dp), ptr(dq), ptr(F), ptr(T), None, 0 if precision == "fp64" else 1,
         reps, C.byref(ms), ptr(census)), "pst_time_forces_pose")
-    grid_bytes = 4  # FP32 grids
-    # per trajectory: pose in (24 + 32 B) and force + torque out (48 B);
-    # per in-grid atom and grid: 20 distinct stencil nodes
-    alg = census[0] * (56 + 48) + (census[1] + census[2] + census[3]) * 20 * grid_bytes
+    # per trajectory: pose in (24 + 32 B), force + torque out (48 B); per
+    # in-grid atom and grid: 20 distinct FP32 stencil nodes
+    alg = census[0] * (56 + 48) + (census[1] + census[2] + census[3]) * 20 * 4
     sec = ms.value * 1e-3
     return {"ms": ms.value, "bytes": alg, "census": census.tolist(),
-            "achieved_gbs": alg / sec / 1e9 if sec > 0 else 0.0}
+            "achieved_gbs": alg / sec / 1e9 if sec > 0 else 0.0, "n": n}
+
+
+def mlp_roofline(surr, pos, quat, reps=20):
+    """Average duration of the MLP forward on n poses (events on torch's
+    stream, which is the library's stream) and its FLOP rate."""
+    import torch
+    n = min(len(pos), surr.max_rows)
+    X = torch.as_tensor(surr.features(pos[:n], quat[:n]), device=surr.device)
+    surr.forward(X)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        surr.forward(X)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    return {"ms": ms, "n": n, "tflops": MLP_FLOP_PER_TRAJ * n / (ms * 1e-3) / 1e12}
+
+
+def full_run(sysm, eng, args, surrogate=None):
+    """Complete trajectories for k_on (99.5 % rule), returns P and Wilson CI."""
+    sysm.params.n_trajectories = args.kon_trajectories
+    res = make_sim(sysm, eng, args, surrogate, n_slots=min(args.kon_trajectories, args.slots)).run()
+    lo, hi = res.reaction_probability_ci(0.95)
+    out = {"n": res.n_trajectories, "reacted": res.n_reacted, "escaped": res.n_escaped,
+           "P": res.reaction_probability, "P_ci95": [lo, hi],
+           "k_on": res.rate_constant(0.0, res.k_b),
+           "k_on_ci95": list(res.rate_constant_ci(0.0, res.k_b)),
+           "traj_steps_per_s": res.steps_per_sec}
+    if surrogate is not None:
+        out["audit"] = {k: (float(v) if v is not None else None)
+                        for k, v in res.sim_data["surrogate_audit"].items()}
+    sysm.params.n_trajectories = args.trajectories
+    return out
 
 
 def cpu_baseline(sysm, grids, pos, quat, seconds=15.0, n_traj=128):
     """The NumPy FP64 oracle of the same step on a bounded sample: n_traj live
-    poses taken from the GPU run, stepped until ~`seconds` of CPU time."""
+    poses taken from the GPU run, stepped for about `seconds` of wall time."""
     from oracle.simulator import OracleSim
     from tests._systems import oracle_params, oracle_system
     np_grids = {k: [(np.asarray(g.data), np.asarray(g.origin), float(g.delta[0, 0])) for g in v]
@@ -120,7 +169,6 @@ def cpu_baseline(sysm, grids, pos, quat, seconds=15.0, n_traj=128):
     ora = OracleSim(oracle_system(sysm, np_grids), oracle_params(sysm, n_trajectories=len(sel)))
     ora.pos[:] = pos[sel]
     ora.q[:] = quat[sel]
-    t0 = time.process_time()
     w0 = time.perf_counter()
     steps = 0
     while time.perf_counter() - w0 < seconds:
@@ -132,19 +180,30 @@ def cpu_baseline(sysm, grids, pos, quat, seconds=15.0, n_traj=128):
     return {"value": steps / wall if wall > 0 else 0.0, "unit": "trajectory-steps/s", "cores": 1,
             "kind": "port",
             "sample": f"{len(sel)} live poses from the GPU run stepped {ora.step_index} times "
-                      f"({steps} trajectory-steps, {wall:.1f}s, single-threaded NumPy FP64 oracle)",
-            "cpu_seconds": time.process_time() - t0}
+                      f"({steps} trajectory-steps, {wall:.1f}s, single-threaded NumPy FP64 oracle)"}
+
+
+def train_surrogate(sysm, eng64, args):
+    from pystarc_amd.surrogate import build_surrogate
+    t0 = time.time()
+    s = build_surrogate(eng64, sysm, n_train=args.train_poses, epochs=args.train_epochs,
+                        batch=args.train_batch, max_rows=max(args.slots, args.train_batch),
+                        log=log)
+    log(f"surrogate trained in {time.time() - t0:.1f}s, validation force error {s.val_error:.3f}")
+    return s
 
 
 def pmc_mode(args):
-    """For rocprofv3 --pmc runs: a few production steps, `--pmc-launches` fused
-    force launches on the live poses, then one plain device copy of 256 MiB
-    whose FETCH_SIZE / WRITE_SIZE calibrate the counters' byte scale."""
+    """For rocprofv3 --pmc runs: a production window of the first path, three
+    fused force launches on the live poses, then one plain device copy of
+    256 MiB whose FETCH_SIZE / WRITE_SIZE calibrate the counters' byte scale."""
     import torch
+    from pystarc_amd import systems as S
     sysm, grids = build_system(args)
     prec = args.paths.split(",")[0]
-    r = run_path(sysm, grids, prec, args)
-    force_roofline(r["eng"], r["pos"], r["quat"], prec, reps=args.pmc_launches)
+    eng = S.make_engine(sysm, grids, precision=prec)
+    r = run_window(sysm, eng, args)
+    force_roofline(eng, r["pos"], r["quat"], prec, reps=args.pmc_launches)
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -154,16 +213,23 @@ def pmc_mode(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pmc", action="store_true", help="counter-collection mode (no JSON line)")
-    ap.add_argument("--pmc-launches", type=int, default=3)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--slots", type=int, default=1 << 17)
+    ap.add_argument("--slots", type=int, default=1 << 16)
     ap.add_argument("--trajectories", type=int, default=10_000_000)
     ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--paths", default="fp64,fp32,surrogate")
+    ap.add_argument("--kon-trajectories", type=int, default=100_000)
+    ap.add_argument("--train-poses", type=int, default=4_000_000)
+    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--audit-n", type=int, default=64)
+    ap.add_argument("--audit-tol", type=float, default=0.25)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--no-kon", action="store_true")
+    ap.add_argument("--pmc", action="store_true", help="counter-collection mode (no JSON line)")
+    ap.add_argument("--pmc-launches", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU in this build (asked for {args.gpus}); "
@@ -177,24 +243,66 @@ def main():
     if args.pmc:
         pmc_mode(args)
         return
+    from pystarc_amd import systems as S
     sysm, grids = build_system(args)
+    paths = args.paths.split(",")
+    engines = {p: S.make_engine(sysm, grids, precision="fp64" if p == "fp64" else "fp32")
+               for p in ("fp64", "fp32")}
+    surr = None
+    if "surrogate" in paths:
+        surr = train_surrogate(sysm, engines["fp64"], args)
     results = {}
-    for prec in args.paths.split(","):
-        log(f"path {prec}: warmup {args.warmup}, timed {args.steps} steps, {args.slots} slots")
-        r = run_path(sysm, grids, prec, args)
-        log(f"path {prec}: {r['value']:.4g} traj-steps/s, {r['device_ms_per_step']:.3f} ms/step device")
-        rf = force_roofline(r["eng"], r["pos"], r["quat"], prec)
-        log(f"path {prec}: force kernel {rf['ms']:.3f} ms, {rf['achieved_gbs']:.1f} GB/s algorithmic")
-        r["roof"] = rf
-        results[prec] = r
-    best_key = max(results, key=lambda k: results[k]["value"])
-    best = results[best_key]
+    for p in paths:
+        eng = engines["fp32" if p == "surrogate" else p]
+        log(f"path {p}: warmup {args.warmup}, timed {args.steps} steps, {args.slots} slots")
+        r = run_window(sysm, eng, args, surr if p == "surrogate" else None)
+        log(f"path {p}: {r['value']:.4g} traj-steps/s, {r['device_ms_per_step']:.3f} ms/step device"
+            + (f", audit rel rms {r['audit_rel_rms']:.3f}" if r["audit_rel_rms"] is not None else ""))
+        prec = "fp64" if p == "fp64" else "fp32"
+        if p == "surrogate":
+            rr = np.linalg.norm(r["pos"], axis=1)
+            near = rr <= surr.r_on
+            r["roof_force"] = force_roofline(eng, r["pos"][near], r["quat"][near], prec) if near.any() else None
+            r["roof_mlp"] = mlp_roofline(surr, r["pos"][~near], r["quat"][~near])
+            log(f"path {p}: mlp {r['roof_mlp']['ms']:.3f} ms for {r['roof_mlp']['n']} poses "
+                f"({r['roof_mlp']['tflops']:.1f} TFLOP/s)")
+        else:
+            r["roof_force"] = force_roofline(eng, r["pos"], r["quat"], prec)
+            log(f"path {p}: force kernel {r['roof_force']['ms']:.3f} ms, "
+                f"{r['roof_force']['achieved_gbs']:.1f} GB/s algorithmic")
+        results[p] = r
+    kon = {}
+    if not args.no_kon:
+        for p in ("fp64", "surrogate"):
+            if p in paths:
+                kon[p] = full_run(sysm, engines["fp32" if p == "surrogate" else p], args,
+                                  surr if p == "surrogate" else None)
+                log(f"k_on {p}: P={kon[p]['P']:.5f} CI95 {kon[p]['P_ci95']}")
+        if "fp64" in kon and "surrogate" in kon:
+            lo, hi = kon["fp64"]["P_ci95"]
+            kon["surrogate_in_plain_ci95"] = bool(lo <= kon["surrogate"]["P"] <= hi)
+    head = "surrogate" if "surrogate" in results else max(results, key=lambda k: results[k]["value"])
+    best = results[head]
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline(sysm, grids, best["pos"], best["quat"])
         log(f"cpu baseline {cpu['value']:.4g} traj-steps/s")
-    rf = best["roof"]
+    # dominant kernel of the headline path: MFMA roofline of the MLP when it
+    # takes longer than the force kernel on the near trajectories, else HBM
+    if head == "surrogate" and (best["roof_force"] is None
+                                or best["roof_mlp"]["ms"] >= best["roof_force"]["ms"]):
+        rm = best["roof_mlp"]
+        roof = {"bound": "mfma", "achieved": rm["tflops"], "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": rm["tflops"] / BF16_PEAK_TFLOPS, "traffic": None,
+                "kernel": "k_gemm (MLP forward, 5 layers)", "ms_per_launch": rm["ms"],
+                "rows": rm["n"]}
+    else:
+        rf = best["roof_force"]
+        roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_forces", "ms_per_launch": rf["ms"], "census": rf["census"]}
+    plain = results.get("fp64")
     out = {
         "metric": METRIC,
         "value": best["value"],
@@ -206,20 +314,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 state, " + ("f64" if best_key == "fp64" else "f32") + " forces",
+        "dtype": {"surrogate": "f64 state, f32 grid forces, bf16 MLP (f32 accumulate)",
+                  "fp32": "f64 state, f32 forces", "fp64": "f64"}[head],
         "data": "synthetic",
         "config": {"workload": "protein-protein 1700/1400 atoms, 193^3 FP32 grids (receptor "
                                "potential+Born, ligand Born), bidirectional Born, scalar HI, "
                                "b=60 A, 2-of-5 contacts < 5.5 A",
-                   "path": best_key, "slots": args.slots, "graph_steps": args.graph_steps},
-        "roofline": {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_forces", "ms_per_launch": rf["ms"],
-                     "census": rf["census"]},
+                   "path": head, "slots": args.slots, "graph_steps": args.graph_steps},
+        "speedup_vs_plain_fp64": (best["value"] / plain["value"]) if plain else None,
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "k_on": kon,
+        "surrogate": None if surr is None else {
+            "validation_force_rel_rms": surr.val_error, "audit_tol": args.audit_tol,
+            "r_on": surr.r_on, "train_poses": args.train_poses, "epochs": args.train_epochs},
         "paths": {k: {"value": v["value"], "device_ms_per_step": v["device_ms_per_step"],
-                      "force_ms": v["roof"]["ms"], "force_gbs": v["roof"]["achieved_gbs"],
-                      "counters": v["counters"]} for k, v in results.items()},
+                      "audit_rel_rms": v["audit_rel_rms"],
+                      "force_ms": v["roof_force"]["ms"] if v.get("roof_force") else None,
+                      "force_gbs": v["roof_force"]["achieved_gbs"] if v.get("roof_force") else None,
+                      "mlp": v.get("roof_mlp")}
+                  for k, v in results.items()},
     }
     print(json.dumps(out), flush=True)
 

@@ -135,6 +135,31 @@ int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
 /* Mean device time per step (ms) of the last pst_prod_run, HIP events. */
 int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);
 
+/* ---- learned surrogate ---------------------------------------------------
+ * No reference counterpart: a 9 -> 4x256 (SiLU) -> 13 MLP standing in for
+ * the grid + Born + far-field interaction outside a contact shell, with
+ * forward and backward on BF16 MFMA (FP32 accumulate).  Parameters are one
+ * device FP32 vector of pst_mlp_param_count() floats, [out][in] per layer
+ * (W0 256x9, b0, W1..W3 256x256 + biases, W4 13x256, b4).                */
+typedef struct pst_mlp pst_mlp;
+int pst_mlp_param_count(void);
+int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out);
+int pst_mlp_destroy(pst_mlp* mlp);
+/* Casts device FP32 parameters to the kernels' padded BF16 weights. */
+int pst_mlp_load(pst_mlp* mlp, const float* params);
+/* x (B,9) f32 -> y (B,13) f32, device pointers; train != 0 keeps the
+ * activations for pst_mlp_backward. */
+int pst_mlp_forward(pst_mlp* mlp, int B, const float* x, float* y, int train);
+/* dL/dy (B,13) f32 -> dL/dparams (flat f32), for the last train forward. */
+int pst_mlp_backward(pst_mlp* mlp, int B, const float* gy, float* gparams);
+/* Attaches (mlp != NULL) or detaches the surrogate branch of the production
+ * step.  p9 = {r_on, r0, lambda_env, 1/b, scale_F, scale_T, scale_E,
+ * use_torque, audit_n}: trajectories with centre distance > r_on take the
+ * MLP force; the grid path re-evaluates audit_n of them per step and the
+ * squared error / reference sums come back in counters16[12..14]. */
+int pst_sim_set_surrogate(pst_sim* sim, pst_mlp* mlp, const double* p9);
+int pst_sim_reset_audit(pst_sim* sim);
+
 /* ---- synthetic grids -----------------------------------------------------
  * Fills a host array (nx,ny,nz) C-order with a Debye-Hueckel potential
  * (kind 0) or an analytic Born desolvation field (kind 1) of `atoms`

@@ -83,6 +83,14 @@ _SIGS = {
     "pst_prod_last_ms_per_step": ([_vp, _dp], C.c_int),
     "pst_make_grid": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double,
                        C.c_double, C.c_double, C.c_int, _vp], C.c_int),
+    "pst_mlp_param_count": ([], C.c_int),
+    "pst_mlp_create": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
+    "pst_mlp_destroy": ([_vp], C.c_int),
+    "pst_mlp_load": ([_vp, _vp], C.c_int),
+    "pst_mlp_forward": ([_vp, C.c_int, _vp, _vp, C.c_int], C.c_int),
+    "pst_mlp_backward": ([_vp, C.c_int, _vp, _vp], C.c_int),
+    "pst_sim_set_surrogate": ([_vp, _vp, _vp], C.c_int),
+    "pst_sim_reset_audit": ([_vp], C.c_int),
 }
 
 

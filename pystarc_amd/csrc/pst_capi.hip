@@ -10,6 +10,8 @@
 
 #include "../../include/pystarc_hip.h"
 #include "pst_kernels.hpp"
+#include "pst_mlp.hpp"
+#include "pst_surrogate.hpp"
 
 using namespace pst;
 
@@ -48,6 +50,12 @@ struct pst_ctx {
   int chunk_cap = 0;
 };
 
+struct pst_mlp {
+  pst_ctx* ctx = nullptr;
+  MlpBufs m;
+  std::vector<void*> allocs;
+};
+
 struct pst_sim {
   pst_ctx* ctx = nullptr;
   StepParams P;
@@ -79,6 +87,11 @@ struct pst_sim {
   int* out_steps = nullptr;
   double last_ms_per_step = 0;
   bool prod_ready = false;
+  // surrogate branch
+  pst_mlp* mlp = nullptr;
+  SurrParams sp;
+  double* audit_F = nullptr;
+  int audit_cap = 0;
 };
 
 #define CK(expr)                                                                      \
@@ -548,7 +561,8 @@ int pst_sim_destroy(pst_sim* s) {
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
-                  s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps};
+                  s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
+                  s->audit_F};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete s;
@@ -813,6 +827,23 @@ static int enqueue_step(pst_sim* s, int par) {
   ForceSys S;
   fill_sys(ctx, S);
   CK(launch_forces(S, W, s->precision, group_for(ctx, S), s->S, 2048, ctx->stream));
+  if (s->mlp && s->P.surrogate_r > 0) {
+    CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
+                 ctx->stream));
+    if (s->sp.audit_n > 0) {
+      // the plain path judges the surrogate on the first audit_n surrogate trajectories
+      ForceWork A = W;
+      A.list = s->mlp_list;
+      A.n_dev = &s->ctr->n_mlp;
+      A.n_cap = s->sp.audit_n;
+      A.out_compact = 1;
+      A.F = s->audit_F;
+      A.T = nullptr;
+      CK(launch_forces(S, A, s->precision, group_for(ctx, S), s->sp.audit_n, 2048, ctx->stream));
+      CK(surr_audit(s->mlp_list, &s->ctr->n_mlp, s->sp.audit_n, s->F, s->audit_F, s->ctr,
+                    ctx->stream));
+    }
+  }
   CK(prod_step_post(s->P, S, B, par, ctx->stream));
   CK(prod_step_bridge(s->P, B, par, ctx->stream));
   return 0;
@@ -879,7 +910,8 @@ int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) 
     double v[16] = {(double)h.n_reacted, (double)h.n_escaped, (double)h.n_maxsteps,
                     (double)h.total_steps, (double)h.next_traj, (double)h.n_returns,
                     (double)h.n_overlap, (double)h.n_bridge, (double)steps, (double)h.stop,
-                    (double)h.n_sr, (double)h.n_mlp, 0, 0, 0, 0};
+                    (double)h.n_sr, (double)h.n_mlp, h.audit_err2, h.audit_ref2,
+                    (double)h.audit_n, 0};
     memcpy(c16, v, sizeof(v));
   }
   return 0;
@@ -925,6 +957,131 @@ int pst_make_grid(pst_ctx* ctx, int kind, int nx, int ny, int nz, const double* 
   CK(hipMemcpy(host_out, d_out, count * (dtype == PST_F64 ? 8 : 4), hipMemcpyDeviceToHost));
   hipFree(d_atoms);
   hipFree(d_out);
+  return 0;
+}
+
+// ---- surrogate MLP ------------------------------------------------------------
+int pst_mlp_param_count(void) { return kMlpParams; }
+
+int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out) {
+  if (!ctx) return 1;
+  if (max_rows <= 0 || !out) FAIL("pst_mlp_create: bad arguments");
+  CK(set_dev(ctx));
+  pst_mlp* p = new pst_mlp();
+  p->ctx = ctx;
+  *out = p;
+  MlpBufs& m = p->m;
+  const size_t R = (size_t)(max_rows + 63) / 64 * 64;
+  m.max_rows = (int)R;
+  const size_t H = kMlpWidth;
+  auto al = [&](void** dst, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e == hipSuccess) p->allocs.push_back(*dst);
+    return e;
+  };
+  CK(al((void**)&m.w0, H * 64 * 2));
+  CK(al((void**)&m.w1, H * H * 2));
+  CK(al((void**)&m.w2, H * H * 2));
+  CK(al((void**)&m.w3, H * H * 2));
+  CK(al((void**)&m.w4, 64 * H * 2));
+  CK(al((void**)&m.b0, H * 4));
+  CK(al((void**)&m.b1, H * 4));
+  CK(al((void**)&m.b2, H * 4));
+  CK(al((void**)&m.b3, H * 4));
+  CK(al((void**)&m.b4, 64 * 4));
+  CK(al((void**)&m.x0, R * 64 * 2));
+  __bf16** hs[4] = {&m.h1, &m.h2, &m.h3, &m.h4};
+  float** zs[4] = {&m.z1, &m.z2, &m.z3, &m.z4};
+  __bf16** gs[4] = {&m.g1, &m.g2, &m.g3, &m.g4};
+  for (int l = 0; l < 4; ++l) {
+    CK(al((void**)hs[l], R * H * 2));
+    CK(al((void**)zs[l], R * H * 4));
+    CK(al((void**)gs[l], R * H * 2));
+  }
+  CK(al((void**)&m.y, R * 64 * 4));
+  CK(al((void**)&m.gy, R * 64 * 2));
+  CK(al((void**)&m.gw, (H * 64 * 2 + H * H * 3) * 4));
+  CK(hipMemsetAsync(m.x0, 0, R * 64 * 2, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pst_mlp_destroy(pst_mlp* p) {
+  if (!p) return 0;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  for (void* a : p->allocs) hipFree(a);
+  delete p;
+  return 0;
+}
+
+int pst_mlp_load(pst_mlp* p, const float* params) {
+  if (!p) return 1;
+  pst_ctx* ctx = p->ctx;
+  if (!params) FAIL("pst_mlp_load: null parameters");
+  CK(set_dev(ctx));
+  CK(mlp_cast(p->m, params, ctx->stream));
+  return 0;
+}
+
+int pst_mlp_forward(pst_mlp* p, int B, const float* x, float* y, int train) {
+  if (!p) return 1;
+  pst_ctx* ctx = p->ctx;
+  if (B <= 0 || B > p->m.max_rows || !x || !y) FAIL("pst_mlp_forward: bad batch or pointers");
+  CK(set_dev(ctx));
+  CK(mlp_forward(p->m, B, x, y, train != 0, ctx->stream));
+  return 0;
+}
+
+int pst_mlp_backward(pst_mlp* p, int B, const float* gy, float* gparams) {
+  if (!p) return 1;
+  pst_ctx* ctx = p->ctx;
+  if (B <= 0 || B > p->m.max_rows || !gy || !gparams) FAIL("pst_mlp_backward: bad batch or pointers");
+  CK(set_dev(ctx));
+  CK(mlp_backward(p->m, B, gy, gparams, ctx->stream));
+  return 0;
+}
+
+int pst_sim_set_surrogate(pst_sim* s, pst_mlp* mlp, const double* p9) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  CK(set_dev(ctx));
+  if (!mlp) {
+    s->mlp = nullptr;
+    s->P.surrogate_r = -1.0;
+    return 0;
+  }
+  if (!p9) FAIL("pst_sim_set_surrogate: null parameters");
+  if (mlp->m.max_rows < s->S) FAIL("pst_sim_set_surrogate: MLP buffers smaller than the slot pool");
+  SurrParams sp;
+  sp.r_on = p9[0]; sp.r0 = p9[1]; sp.lam = p9[2]; sp.inv_b = p9[3];
+  sp.scale_F = p9[4]; sp.scale_T = p9[5]; sp.scale_E = p9[6];
+  sp.use_torque = p9[7] != 0.0; sp.audit_n = (int)p9[8];
+  if (!(sp.r_on > 0) || !(sp.r0 > 0) || !(sp.lam > 0)) FAIL("pst_sim_set_surrogate: bad parameters");
+  if (sp.audit_n < 0 || sp.audit_n > 65536) FAIL("pst_sim_set_surrogate: audit_n out of range");
+  if (sp.audit_n > s->audit_cap) {
+    if (s->audit_F) hipFree(s->audit_F);
+    s->audit_F = nullptr;
+    s->audit_cap = 0;
+    if (sp.audit_n > 0) {
+      CK(hipMalloc(&s->audit_F, (size_t)sp.audit_n * 3 * 8));
+      s->audit_cap = sp.audit_n;
+    }
+  }
+  s->sp = sp;
+  s->mlp = mlp;
+  s->P.surrogate_r = sp.r_on;
+  return 0;
+}
+
+int pst_sim_reset_audit(pst_sim* s) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->ctr) FAIL("pst_sim_reset_audit: call pst_prod_init first");
+  CK(set_dev(ctx));
+  CK(hipMemsetAsync(&s->ctr->audit_n, 0, sizeof(unsigned long long), ctx->stream));
+  CK(hipMemsetAsync(&s->ctr->audit_err2, 0, 2 * sizeof(double), ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

@@ -95,7 +95,8 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
     }
     __syncthreads();
   }
-  const int n = W.n_dev ? *W.n_dev : W.n;
+  int n = W.n_dev ? *W.n_dev : W.n;
+  if (W.n_cap > 0 && n > W.n_cap) n = W.n_cap;
   const int lane = threadIdx.x % G;
   const int grp = threadIdx.x / G;
   const int gpb = kForceBlock / G;

@@ -6,8 +6,10 @@
 //   phi(x) = sum_j q_j exp(-(d - a_j)/lambda) / ((1 + a_j/lambda) 4 pi eps0 eps_out d),
 //   d = max(|x - r_j|, a_j, 0.5 A); with a_j = 0 this is the reference's
 //   Yukawa monopole V_fac e^{-r/lambda}/r (gpu_batch_engine.py:407-410).
-// kind 1, analytic Born desolvation field:
-//   phi_B(x) = (1/eps_in - 1/eps_out) / (8 pi eps0) sum_j a_j^3 / max(|x - r_j|, a_j)^4,
+// kind 1, analytic Born desolvation field (image-charge strength of a low-
+// dielectric atom of radius a_j in solvent):
+//   phi_B(x) = (eps_out - eps_in)/(eps_out + eps_in) / (8 pi eps0 eps_out)
+//              * sum_j a_j^3 / max(|x - r_j|, a_j, 0.5)^4,
 //   multiplied by alpha q^2 in the force kernel (gpu_batch_engine.py:116-121).
 #include <hip/hip_runtime.h>
 #include "pst_kernels.hpp"
@@ -30,7 +32,7 @@ __global__ void __launch_bounds__(256) k_grid(float* out32, double* out64, int n
   const double x = ox + i * sx, y = oy + j * sy, z = oz + k * sz;
   double acc = 0.0;
   const double pref0 = 1.0 / (4.0 * M_PI * kEps0 * eps_out);
-  const double pref1 = (1.0 / eps_in - 1.0 / eps_out) / (8.0 * M_PI * kEps0);
+  const double pref1 = (eps_out - eps_in) / (eps_out + eps_in) / (8.0 * M_PI * kEps0 * eps_out);
   for (int base = 0; base < n; base += 256) {
     __syncthreads();
     const int a = base + threadIdx.x;

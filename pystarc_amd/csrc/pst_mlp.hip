@@ -1,0 +1,326 @@
+// Learned surrogate of the grid + Born + far-field interaction: a
+// 9 -> 256 -> 256 -> 256 -> 256 -> 13 SiLU MLP, forward and backward on
+// gfx950 matrix cores (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+// accumulate).  No reference counterpart.
+//
+// Layout: parameters arrive as one FP32 vector in [out][in] order
+// (W0 256x9, b0, W1 256x256, b1, W2, b2, W3, b3, W4 13x256, b4).  The kernels
+// use padded BF16 copies: W0p [256][64] (inputs padded 9 -> 64), W1..W3
+// [256][256], W4p [64][256] (outputs padded 13 -> 64); activations are BF16
+// [rows][256] (rows padded to 64), pre-activations FP32.
+//
+// One GEMM kernel serves every product: a 256-thread workgroup owns a 64x64
+// output tile, stages 64x32 slices of A and B in LDS as [row][k] with k
+// contiguous, and each wave runs a 2x2 block of 16x16x32 MFMAs.  Transposed
+// operands are transposed on their way into LDS.  Epilogues fuse bias,
+// SiLU, SiLU' and the FP32 weight-gradient accumulation.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "pst_device.hpp"
+#include "pst_kernels.hpp"
+#include "pst_mlp.hpp"
+
+namespace pst {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+__device__ inline float dsilu_f(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+enum { LAY_MK = 0, LAY_KM = 1 };  // A: [m][k] or [k][m];  B: [n][k] (=MK) or [k][n] (=KM)
+enum { EPI_HIDDEN = 0, EPI_HIDDEN_INFER = 1, EPI_OUT = 2, EPI_DATA = 3, EPI_WGRAD = 4 };
+
+struct GemmArgs {
+  const __bf16* A; int lda;
+  const __bf16* B; int ldb;
+  int M, N, K;          // M, N multiples of 64; K multiple of 32
+  const int* m_dev;     // live rows (rows >= *m_dev are skipped by 64-row tiles), or null
+  int k_split;          // K range split across blockIdx.z (weight gradients)
+  // epilogue operands
+  const float* bias;    // [N]
+  float* Z; int ldz;    // pre-activation out (EPI_HIDDEN) / in (EPI_DATA)
+  __bf16* H; int ldh;   // bf16 activation / gradient out
+  float* Y; int ldy;    // fp32 out (EPI_OUT, EPI_WGRAD accumulate)
+};
+
+constexpr int kPad = 8;
+
+template <int AL, int BL, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[64][32 + kPad];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][32 + kPad];
+  const int tile_n = blockIdx.x * 64;
+  const int tile_m = blockIdx.y * 64;
+  if (g.m_dev && tile_m >= *g.m_dev) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  int kbeg = 0, kend = g.K;
+  if (g.k_split > 1) {
+    const int chunk = ((g.K / 32 + g.k_split - 1) / g.k_split) * 32;
+    kbeg = blockIdx.z * chunk;
+    kend = min(g.K, kbeg + chunk);
+  }
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+    if (AL == LAY_MK) {
+      const int row = tid >> 2, kc = (tid & 3) * 8;
+      *(bf16x8*)&As[row][kc] = *(const bf16x8*)&g.A[(size_t)(tile_m + row) * g.lda + k0 + kc];
+    } else {
+      const int kr = tid >> 3, mc = (tid & 7) * 8;
+      const bf16x8 v = *(const bf16x8*)&g.A[(size_t)(k0 + kr) * g.lda + tile_m + mc];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) As[mc + j][kr] = v[j];
+    }
+    if (BL == LAY_MK) {
+      const int row = tid >> 2, kc = (tid & 3) * 8;
+      *(bf16x8*)&Bs[row][kc] = *(const bf16x8*)&g.B[(size_t)(tile_n + row) * g.ldb + k0 + kc];
+    } else {
+      const int kr = tid >> 3, nc = (tid & 7) * 8;
+      const bf16x8 v = *(const bf16x8*)&g.B[(size_t)(k0 + kr) * g.ldb + tile_n + nc];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bs[nc + j][kr] = v[j];
+    }
+    __syncthreads();
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    bf16x8 a0 = *(const bf16x8*)&As[wm + r][kq];
+    bf16x8 a1 = *(const bf16x8*)&As[wm + 16 + r][kq];
+    bf16x8 b0 = *(const bf16x8*)&Bs[wn + r][kq];
+    bf16x8 b1 = *(const bf16x8*)&Bs[wn + 16 + r][kq];
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+    __syncthreads();
+  }
+  // D layout of the 16x16 MFMA tile: lane holds rows 4*(lane/16)+i, column lane%16
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = tile_m + wm + mt * 16 + (lane >> 4) * 4 + i;
+        const int n = tile_n + wn + nt * 16 + (lane & 15);
+        const float v = acc[mt][nt][i];
+        if (EPI == EPI_HIDDEN) {
+          const float z = v + g.bias[n];
+          g.Z[(size_t)m * g.ldz + n] = z;
+          g.H[(size_t)m * g.ldh + n] = (__bf16)silu_f(z);
+        } else if (EPI == EPI_HIDDEN_INFER) {
+          g.H[(size_t)m * g.ldh + n] = (__bf16)silu_f(v + g.bias[n]);
+        } else if (EPI == EPI_OUT) {
+          g.Y[(size_t)m * g.ldy + n] = v + g.bias[n];
+        } else if (EPI == EPI_DATA) {
+          g.H[(size_t)m * g.ldh + n] = (__bf16)(v * dsilu_f(g.Z[(size_t)m * g.ldz + n]));
+        } else {
+          atomicAdd(&g.Y[(size_t)m * g.ldy + n], v);
+        }
+      }
+}
+
+template <int AL, int BL, int EPI>
+static hipError_t gemm(const GemmArgs& g, hipStream_t st, int splits = 1) {
+  GemmArgs a = g;
+  a.k_split = splits;
+  hipLaunchKernelGGL((k_gemm<AL, BL, EPI>), dim3(g.N / 64, g.M / 64, splits), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---- small element-wise kernels --------------------------------------------------
+// FP32 parameters -> padded BF16 weights + FP32 biases.
+__global__ void k_mlp_cast(const float* p, __bf16* w0, __bf16* w1, __bf16* w2, __bf16* w3,
+                           __bf16* w4, float* b0, float* b1, float* b2, float* b3, float* b4) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int H = kMlpWidth;
+  // W0p [256][64]
+  if (t < H * 64) {
+    const int o = t / 64, i = t % 64;
+    w0[t] = (__bf16)(i < kMlpIn ? p[kOffW0 + o * kMlpIn + i] : 0.0f);
+  }
+  if (t < H * H) {
+    w1[t] = (__bf16)p[kOffW1 + t];
+    w2[t] = (__bf16)p[kOffW2 + t];
+    w3[t] = (__bf16)p[kOffW3 + t];
+  }
+  if (t < 64 * H) {
+    const int o = t / H, i = t % H;
+    w4[t] = (__bf16)(o < kMlpOut ? p[kOffW4 + o * H + i] : 0.0f);
+  }
+  if (t < H) {
+    b0[t] = p[kOffB0 + t];
+    b1[t] = p[kOffB1 + t];
+    b2[t] = p[kOffB2 + t];
+    b3[t] = p[kOffB3 + t];
+  }
+  if (t < 64) b4[t] = t < kMlpOut ? p[kOffB4 + t] : 0.0f;
+}
+
+// (B,9) FP32 features -> [Bp][64] BF16, zero padded.
+__global__ void k_mlp_pack_x(const float* x, int B, int Bp, __bf16* x0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Bp * 64) return;
+  const int r = t / 64, c = t % 64;
+  x0[t] = (__bf16)((r < B && c < kMlpIn) ? x[r * kMlpIn + c] : 0.0f);
+}
+
+// [Bp][64] FP32 outputs -> (B,13).
+__global__ void k_mlp_unpack_y(const float* y, int B, float* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * kMlpOut) return;
+  out[t] = y[(t / kMlpOut) * 64 + t % kMlpOut];
+}
+
+// (B,13) FP32 output gradient -> [Bp][64] BF16, zero padded.
+__global__ void k_mlp_pack_gy(const float* gy, int B, int Bp, __bf16* g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Bp * 64) return;
+  const int r = t / 64, c = t % 64;
+  g[t] = (__bf16)((r < B && c < kMlpOut) ? gy[r * kMlpOut + c] : 0.0f);
+}
+
+// Bias gradient: column sums of a BF16 [Bp][ld] gradient into grad[off + n].
+__global__ void k_colsum(const __bf16* g, int rows, int ld, int n_cols, float* out) {
+  const int n = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) s += (float)g[(size_t)r * ld + n];
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && n < n_cols) out[n] = red[0];
+}
+
+// Padded weight gradients -> the flat FP32 [out][in] layout.
+__global__ void k_mlp_gather_grads(const float* gw0, const float* gw1, const float* gw2,
+                                   const float* gw3, const float* gw4, float* gp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int H = kMlpWidth;
+  if (t < H * kMlpIn) gp[kOffW0 + t] = gw0[(t / kMlpIn) * 64 + t % kMlpIn];
+  if (t < H * H) {
+    gp[kOffW1 + t] = gw1[t];
+    gp[kOffW2 + t] = gw2[t];
+    gp[kOffW3 + t] = gw3[t];
+  }
+  if (t < kMlpOut * H) gp[kOffW4 + t] = gw4[t];
+}
+
+__global__ void k_zero(float* p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.f;
+}
+
+static inline unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ---- host-side orchestration ----------------------------------------------------------
+hipError_t mlp_cast(MlpBufs& m, const float* params, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_cast, dim3(nblk(kMlpWidth * kMlpWidth)), dim3(256), 0, st, params,
+                     m.w0, m.w1, m.w2, m.w3, m.w4, m.b0, m.b1, m.b2, m.b3, m.b4);
+  return hipGetLastError();
+}
+
+// Forward on padded BF16 input x0 [Bp][64]; rows >= *m_dev skipped when m_dev is set.
+hipError_t mlp_forward_packed(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
+                              hipStream_t st) {
+  const int H = kMlpWidth;
+  __bf16* hs[4] = {m.h1, m.h2, m.h3, m.h4};
+  float* zs[4] = {m.z1, m.z2, m.z3, m.z4};
+  const __bf16* ws[4] = {m.w0, m.w1, m.w2, m.w3};
+  const float* bs[4] = {m.b0, m.b1, m.b2, m.b3};
+  const __bf16* in = m.x0;
+  int ld_in = 64;
+  hipError_t e;
+  for (int l = 0; l < 4; ++l) {
+    GemmArgs g = {};
+    g.A = in; g.lda = ld_in; g.B = ws[l]; g.ldb = ld_in;
+    g.M = Bp; g.N = H; g.K = ld_in; g.m_dev = m_dev;
+    g.bias = bs[l]; g.Z = zs[l]; g.ldz = H; g.H = hs[l]; g.ldh = H;
+    e = train ? gemm<LAY_MK, LAY_MK, EPI_HIDDEN>(g, st) : gemm<LAY_MK, LAY_MK, EPI_HIDDEN_INFER>(g, st);
+    if (e != hipSuccess) return e;
+    in = hs[l];
+    ld_in = H;
+  }
+  GemmArgs g = {};
+  g.A = m.h4; g.lda = H; g.B = m.w4; g.ldb = H;
+  g.M = Bp; g.N = 64; g.K = H; g.m_dev = m_dev;
+  g.bias = m.b4; g.Y = y; g.ldy = 64;
+  return gemm<LAY_MK, LAY_MK, EPI_OUT>(g, st);
+}
+
+hipError_t mlp_forward(const MlpBufs& m, int B, const float* x, float* y_out, bool train,
+                       hipStream_t st) {
+  const int Bp = (B + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_mlp_pack_x, dim3(nblk((size_t)Bp * 64)), dim3(256), 0, st, x, B, Bp, m.x0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = mlp_forward_packed(m, Bp, nullptr, train, m.y, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mlp_unpack_y, dim3(nblk((size_t)B * kMlpOut)), dim3(256), 0, st, m.y, B,
+                     y_out);
+  return hipGetLastError();
+}
+
+// Backward of the last train-mode forward: gy (B,13) FP32 -> grads (flat FP32).
+hipError_t mlp_backward(const MlpBufs& m, int B, const float* gy, float* gparams, hipStream_t st) {
+  const int H = kMlpWidth;
+  const int Bp = (B + 63) / 64 * 64;
+  hipError_t e;
+  hipLaunchKernelGGL(k_mlp_pack_gy, dim3(nblk((size_t)Bp * 64)), dim3(256), 0, st, gy, B, Bp, m.gy);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const size_t nw = (size_t)H * 64 * 2 + (size_t)H * H * 3;
+  hipLaunchKernelGGL(k_zero, dim3(nblk(nw)), dim3(256), 0, st, m.gw, nw);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  float* gw0 = m.gw;                    // [256][64]
+  float* gw1 = gw0 + (size_t)H * 64;    // [256][256]
+  float* gw2 = gw1 + (size_t)H * H;
+  float* gw3 = gw2 + (size_t)H * H;
+  float* gw4 = gw3 + (size_t)H * H;     // [64][256]
+  const int splits = max(1, min(64, Bp / 256));
+  // layer 4: dW4 = gy^T h4, db4 = sum gy, g_h4 = gy W4, g_z4 = g_h4 * silu'(z4)
+  {
+    GemmArgs g = {};
+    g.A = m.gy; g.lda = 64; g.B = m.h4; g.ldb = H; g.M = 64; g.N = H; g.K = Bp;
+    g.Y = gw4; g.ldy = H;
+    if ((e = gemm<LAY_KM, LAY_KM, EPI_WGRAD>(g, st, splits)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, st, m.gy, Bp, 64, kMlpOut, gparams + kOffB4);
+    GemmArgs d = {};
+    d.A = m.gy; d.lda = 64; d.B = m.w4; d.ldb = H; d.M = Bp; d.N = H; d.K = 64;
+    d.Z = m.z4; d.ldz = H; d.H = m.g4; d.ldh = H;
+    if ((e = gemm<LAY_MK, LAY_KM, EPI_DATA>(d, st)) != hipSuccess) return e;
+  }
+  __bf16* gz[4] = {m.g1, m.g2, m.g3, m.g4};
+  __bf16* hin[4] = {m.x0, m.h1, m.h2, m.h3};
+  float* zin[4] = {nullptr, m.z1, m.z2, m.z3};
+  const __bf16* ws[4] = {m.w0, m.w1, m.w2, m.w3};
+  float* gws[4] = {gw0, gw1, gw2, gw3};
+  const int boff[4] = {kOffB0, kOffB1, kOffB2, kOffB3};
+  for (int l = 3; l >= 0; --l) {
+    const int nin = l == 0 ? 64 : H;
+    GemmArgs g = {};
+    g.A = gz[l]; g.lda = H; g.B = hin[l]; g.ldb = nin; g.M = H; g.N = nin; g.K = Bp;
+    g.Y = gws[l]; g.ldy = nin;
+    if ((e = gemm<LAY_KM, LAY_KM, EPI_WGRAD>(g, st, splits)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_colsum, dim3(H), dim3(256), 0, st, gz[l], Bp, H, H, gparams + boff[l]);
+    if (l > 0) {
+      GemmArgs d = {};
+      d.A = gz[l]; d.lda = H; d.B = ws[l]; d.ldb = H; d.M = Bp; d.N = H; d.K = H;
+      d.Z = zin[l]; d.ldz = H; d.H = gz[l - 1]; d.ldh = H;
+      if ((e = gemm<LAY_MK, LAY_KM, EPI_DATA>(d, st)) != hipSuccess) return e;
+    }
+  }
+  hipLaunchKernelGGL(k_mlp_gather_grads, dim3(nblk((size_t)H * H)), dim3(256), 0, st, gw0, gw1,
+                     gw2, gw3, gw4, gparams);
+  return hipGetLastError();
+}
+
+}  // namespace pst

@@ -1,0 +1,43 @@
+// Surrogate MLP: sizes, flat FP32 parameter offsets ([out][in] per layer)
+// and device buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pst {
+
+constexpr int kMlpIn = 9;
+constexpr int kMlpWidth = 256;
+constexpr int kMlpOut = 13;
+constexpr int kOffW0 = 0;
+constexpr int kOffB0 = kOffW0 + kMlpWidth * kMlpIn;
+constexpr int kOffW1 = kOffB0 + kMlpWidth;
+constexpr int kOffB1 = kOffW1 + kMlpWidth * kMlpWidth;
+constexpr int kOffW2 = kOffB1 + kMlpWidth;
+constexpr int kOffB2 = kOffW2 + kMlpWidth * kMlpWidth;
+constexpr int kOffW3 = kOffB2 + kMlpWidth;
+constexpr int kOffB3 = kOffW3 + kMlpWidth * kMlpWidth;
+constexpr int kOffW4 = kOffB3 + kMlpWidth;
+constexpr int kOffB4 = kOffW4 + kMlpOut * kMlpWidth;
+constexpr int kMlpParams = kOffB4 + kMlpOut;  // 203,277
+
+struct MlpBufs {
+  int max_rows;  // padded to 64
+  __bf16 *w0, *w1, *w2, *w3, *w4;
+  float *b0, *b1, *b2, *b3, *b4;
+  __bf16* x0;                 // [rows][64]
+  __bf16 *h1, *h2, *h3, *h4;  // [rows][256]
+  float *z1, *z2, *z3, *z4;   // [rows][256] (train mode)
+  float* y;                   // [rows][64]
+  __bf16* gy;                 // [rows][64]
+  __bf16 *g1, *g2, *g3, *g4;  // [rows][256]
+  float* gw;                  // padded weight gradients
+};
+
+hipError_t mlp_cast(MlpBufs& m, const float* params, hipStream_t st);
+hipError_t mlp_forward(const MlpBufs& m, int B, const float* x, float* y_out, bool train,
+                       hipStream_t st);
+hipError_t mlp_forward_packed(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
+                              hipStream_t st);
+hipError_t mlp_backward(const MlpBufs& m, int B, const float* gy, float* gparams, hipStream_t st);
+
+}  // namespace pst

@@ -1,0 +1,107 @@
+// Surrogate branch of the production step.  Trajectories whose centre
+// distance exceeds r_on are listed by k_prod_pre in the surrogate list; for
+// them the MLP replaces the grid + Born + far-field force evaluation.
+//
+// Features (9): centroid in the receptor frame divided by b, and the first
+// two columns of R(q) (a 6-D rotation representation).  Outputs (13):
+// ligand force and torque, receptor force and torque, energy, each divided
+// by a per-group scale and by the envelope
+//   env(r) = exp(-(r - r0)/lam) (r0/r)^2 (1 + r/lam) / (1 + r0/lam),
+// the shape of a screened-Coulomb force normalised to 1 at r0, so the
+// network learns O(1) targets over the whole shell.  An audit evaluates the
+// plain grid path on the first `audit_n` surrogate trajectories of every step
+// and accumulates the squared force error and the squared reference force.
+#include <hip/hip_runtime.h>
+#include "pst_device.hpp"
+#include "pst_kernels.hpp"
+#include "pst_mlp.hpp"
+#include "pst_surrogate.hpp"
+
+namespace pst {
+
+__device__ inline double envelope(const SurrParams& S, double r) {
+  const double x = S.r0 / r;
+  return exp(-(r - S.r0) / S.lam) * x * x * (1.0 + r / S.lam) / (1.0 + S.r0 / S.lam);
+}
+
+__global__ void k_surr_features(SurrParams S, const double* pos, const double* quat,
+                                 const int* list, const int* n_dev, __bf16* x0) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= *n_dev) return;
+  const int s = list[w];
+  double R[9];
+  quat_to_rot(quat + 4 * (size_t)s, R);
+  __bf16* row = x0 + (size_t)w * 64;
+  row[0] = (__bf16)(float)(pos[3 * s] * S.inv_b);
+  row[1] = (__bf16)(float)(pos[3 * s + 1] * S.inv_b);
+  row[2] = (__bf16)(float)(pos[3 * s + 2] * S.inv_b);
+  row[3] = (__bf16)(float)R[0]; row[4] = (__bf16)(float)R[3]; row[5] = (__bf16)(float)R[6];
+  row[6] = (__bf16)(float)R[1]; row[7] = (__bf16)(float)R[4]; row[8] = (__bf16)(float)R[7];
+  for (int c = 9; c < 64; ++c) row[c] = (__bf16)0.0f;
+}
+
+__global__ void k_surr_apply(SurrParams S, const double* pos, const int* list, const int* n_dev,
+                             const float* y, double* F, double* T) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= *n_dev) return;
+  const int s = list[w];
+  const double r = norm3(pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]);
+  const double env = envelope(S, r);
+  const float* o = y + (size_t)w * 64;
+  const double fs = S.scale_F * env, ts = S.scale_T * env;
+  F[3 * s] = o[0] * fs; F[3 * s + 1] = o[1] * fs; F[3 * s + 2] = o[2] * fs;
+  if (S.use_torque) {
+    T[3 * s] = o[3] * ts; T[3 * s + 1] = o[4] * ts; T[3 * s + 2] = o[5] * ts;
+  } else {
+    T[3 * s] = 0.0; T[3 * s + 1] = 0.0; T[3 * s + 2] = 0.0;
+  }
+}
+
+__global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const double* F,
+                             const double* Fref, RunCounters* C) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = min(*n_dev, cap);
+  double e2 = 0, r2 = 0;
+  if (w < n) {
+    const int s = list[w];
+    for (int d = 0; d < 3; ++d) {
+      const double a = F[3 * s + d], b = Fref[3 * w + d];
+      e2 += (a - b) * (a - b);
+      r2 += b * b;
+    }
+  }
+  // wave reduction, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    e2 += __shfl_xor(e2, off);
+    r2 += __shfl_xor(r2, off);
+  }
+  if (__lane_id() == 0 && (e2 > 0 || r2 > 0)) {
+    atomicAdd(&C->audit_err2, e2);
+    atomicAdd(&C->audit_ref2, r2);
+  }
+  if (w == 0) atomicAdd(&C->audit_n, (unsigned long long)n);
+}
+
+hipError_t surr_step(const SurrParams& S, const MlpBufs& m, const double* pos, const double* quat,
+                     const int* list, const int* n_dev, int n_slots, double* F, double* T,
+                     hipStream_t st) {
+  const int blocks = (n_slots + 255) / 256;
+  hipLaunchKernelGGL(k_surr_features, dim3(blocks), dim3(256), 0, st, S, pos, quat, list, n_dev,
+                     m.x0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int Sp = (n_slots + 63) / 64 * 64;
+  e = mlp_forward_packed(m, Sp, n_dev, false, m.y, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_surr_apply, dim3(blocks), dim3(256), 0, st, S, pos, list, n_dev, m.y, F, T);
+  return hipGetLastError();
+}
+
+hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* F,
+                      const double* Fref, RunCounters* C, hipStream_t st) {
+  hipLaunchKernelGGL(k_surr_audit, dim3((cap + 255) / 256), dim3(256), 0, st, list, n_dev, cap, F,
+                     Fref, C);
+  return hipGetLastError();
+}
+
+}  // namespace pst

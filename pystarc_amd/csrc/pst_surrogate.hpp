@@ -1,0 +1,27 @@
+// Surrogate branch of the production step (see pst_surrogate.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pst_mlp.hpp"
+#include "pst_types.hpp"
+
+namespace pst {
+
+struct SurrParams {
+  double r_on;      // centre distance above which the surrogate replaces the grid path
+  double r0;        // envelope reference distance
+  double lam;       // envelope screening length
+  double inv_b;     // feature scale 1/b
+  double scale_F;   // force scale (kBT/A) at r0
+  double scale_T;   // torque scale
+  double scale_E;   // energy scale
+  int use_torque;   // apply the predicted ligand torque (physical torque mode)
+  int audit_n;      // surrogate trajectories re-evaluated on the grid path per step
+};
+
+hipError_t surr_step(const SurrParams& S, const MlpBufs& m, const double* pos, const double* quat,
+                     const int* list, const int* n_dev, int n_slots, double* F, double* T,
+                     hipStream_t st);
+hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* F,
+                      const double* Fref, RunCounters* C, hipStream_t st);
+
+}  // namespace pst

@@ -82,6 +82,7 @@ struct ForceWork {
   int chunk_cs;             // chunk size within a force batch
   int chunks_per_fb;
   int out_compact;          // 1: write outputs at list position, 0: at trajectory index
+  int n_cap;                // > 0: process at most n_cap list entries
 };
 
 // Step parameters (gpu_batch_simulator.py:480-781).

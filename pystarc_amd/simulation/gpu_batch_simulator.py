@@ -20,6 +20,11 @@ Keyword-only extensions (defaults keep the reference's physics):
   precision    force precision "fp64" / "fp32" (default: the engine's).
   graph_steps  steps per captured hipGraph launch in philox mode.
   record       keep per-trajectory outcome and step counts (sim_data).
+  surrogate    a trained pystarc_amd.surrogate.Surrogate: trajectories beyond
+               its contact shell take the MLP force (philox mode only).
+  audit_n      surrogate trajectories re-evaluated on the grid path per step.
+  audit_tol    relative RMS force error above which the run falls back to
+               the grid path for every trajectory.
 """
 from __future__ import annotations
 
@@ -132,12 +137,18 @@ def _shoemake_host(u):
 
 class GPUBatchSimulator:
     def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
-                 n_slots=None, precision=None, graph_steps=16, record=False):
+                 n_slots=None, precision=None, graph_steps=16, record=False, surrogate=None,
+                 audit_n=64, audit_tol=0.05):
         if rng not in ("philox", "numpy"):
             raise ValueError("rng must be 'philox' or 'numpy'")
         self.mol1, self.mol2, self.mob = mol1, mol2, mob
         self.pathway_set, self.params, self.engine = pathway_set, params, batch_engine
         self.rng_mode = rng
+        if surrogate is not None and rng != "philox":
+            raise ValueError("the surrogate runs in philox mode only")
+        self.surrogate = surrogate
+        self.audit_n = int(audit_n)
+        self.audit_tol = float(audit_tol)
         self.graph_steps = int(graph_steps)
         self.record = bool(record)
         prec = precision if precision is not None else getattr(batch_engine, "precision", "fp64")
@@ -305,6 +316,13 @@ class GPUBatchSimulator:
         h = self._make_sim(P)
         try:
             ctx.check(lib.pst_prod_init(h, int(self.record)), "pst_prod_init")
+            surr_on = False
+            if self.surrogate is not None:
+                p9 = np.ascontiguousarray(self.surrogate.sim_params(self.audit_n), dtype=np.float64)
+                ctx.check(lib.pst_sim_set_surrogate(h, self.surrogate.h, ptr(p9)),
+                          "pst_sim_set_surrogate")
+                surr_on = True
+            audit = {"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None}
             c16 = np.zeros(16)
             t0 = time.time()
             steps = 0
@@ -313,6 +331,18 @@ class GPUBatchSimulator:
                 todo = int(min(chunk, P.max_steps - steps))
                 ctx.check(lib.pst_prod_run(h, todo, self.graph_steps, ptr(c16)), "pst_prod_run")
                 steps += int(c16[8])
+                if surr_on and c16[13] > 0:
+                    err = math.sqrt(c16[12] / c16[13])
+                    audit["err2"] += c16[12]
+                    audit["ref2"] += c16[13]
+                    audit["n"] += int(c16[14])
+                    audit["worst"] = max(audit["worst"], err)
+                    ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
+                    if err > self.audit_tol:
+                        # the grid path takes over every trajectory from here on
+                        ctx.check(lib.pst_sim_set_surrogate(h, None, None), "pst_sim_set_surrogate")
+                        surr_on = False
+                        audit["fallback_step"] = steps
                 if c16[9] or (c16[10] + c16[11]) == 0 or int(c16[8]) < todo:
                     break
             elapsed = time.time() - t0
@@ -321,6 +351,9 @@ class GPUBatchSimulator:
             n_m = N - n_r - n_e
             sim_data = {"n_returns": int(c16[5]), "n_overlap_rejected": int(c16[6]),
                         "n_bridge": int(c16[7]), "bd_steps": steps}
+            if self.surrogate is not None:
+                audit["rel_rms"] = math.sqrt(audit["err2"] / audit["ref2"]) if audit["ref2"] > 0 else 0.0
+                sim_data["surrogate_audit"] = audit
             if self.record:
                 st = np.zeros(N, dtype=np.int8)
                 ns = np.zeros(N, dtype=np.int32)

@@ -140,10 +140,29 @@ def two_spheres(n_traj=10_000, seed=11111113, grid_n=97, grid_h=0.5, max_steps=1
                     {"config": "two charged spheres", "a_react": 12.5})
 
 
-def _contact_pairs(rec_pos, lig_pos, n_pairs, cutoff):
-    rec_top = np.argsort(-rec_pos[:, 2])[:n_pairs]
-    lig_bot = np.argsort(lig_pos[:, 2])[:n_pairs]
-    return [ContactPair(int(i), int(j), cutoff) for i, j in zip(rec_top, lig_bot)]
+def _contact_pairs(rec_pos, lig_pos, n_pairs, cutoff, gap=3.0):
+    """Contact pairs of a docked pose, as reaction criteria are taken from a
+    complex structure: the ligand (identity orientation) sits on the receptor's
+    +z face with its lowest atom `gap` A above the receptor's highest atom; the
+    n_pairs closest receptor/ligand atom pairs across that interface (each atom
+    used once) become the pairs."""
+    zc = rec_pos[:, 2].max() - lig_pos[:, 2].min() + gap
+    docked = lig_pos + np.array([0.0, 0.0, zc])
+    rec_face = np.argsort(-rec_pos[:, 2])[: max(4 * n_pairs, 20)]
+    lig_face = np.argsort(lig_pos[:, 2])[: max(4 * n_pairs, 20)]
+    d = np.linalg.norm(rec_pos[rec_face][:, None, :] - docked[lig_face][None, :, :], axis=2)
+    pairs, used_r, used_l = [], set(), set()
+    for flat in np.argsort(d, axis=None):
+        a, b = np.unravel_index(flat, d.shape)
+        i, j = int(rec_face[a]), int(lig_face[b])
+        if i in used_r or j in used_l:
+            continue
+        pairs.append(ContactPair(i, j, cutoff))
+        used_r.add(i)
+        used_l.add(j)
+        if len(pairs) == n_pairs:
+            break
+    return pairs
 
 
 def protein_ligand(n_traj=1_000_000, seed=2012, geom_seed=2002, n_rec=3220,
