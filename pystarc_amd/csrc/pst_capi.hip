@@ -30,6 +30,7 @@ struct pst_ctx {
   bool own_stream = false;
   std::string err;
   std::vector<DevGrid> elec, born, lig_born;
+  DevGrid pair;  // interleaved elec + Born copy when both share a geometry
   double receptor_charge = 0, debye = 7.858, sdie = 78.0;
   bool has_mp = false;
   double dip[3] = {0, 0, 0}, quad[9] = {0};
@@ -82,6 +83,8 @@ struct pst_sim {
   long long* traj = nullptr;
   int* plain_list = nullptr;
   int* mlp_list = nullptr;
+  int* key = nullptr;
+  int* bins = nullptr;
   RunCounters* ctr = nullptr;
   signed char* out_status = nullptr;
   int* out_steps = nullptr;
@@ -119,6 +122,8 @@ static void fill_sys(pst_ctx* ctx, ForceSys& S) {
   for (int i = 0; i < S.n_elec; ++i) S.elec[i] = grid_record(ctx->elec[i]);
   S.n_born = (int)ctx->born.size();
   for (int i = 0; i < S.n_born; ++i) S.born[i] = grid_record(ctx->born[i]);
+  S.paired = ctx->pair.mem != nullptr ? 1 : 0;
+  if (S.paired) S.pair = ctx->pair.g;
   S.has_rev = (!ctx->lig_born.empty() && ctx->n_rec > 0) ? 1 : 0;
   if (S.has_rev) {
     S.lig_born = ctx->lig_born[0].g;
@@ -223,6 +228,7 @@ int pst_ctx_destroy(pst_ctx* ctx) {
   free_grids(ctx->elec);
   free_grids(ctx->born);
   free_grids(ctx->lig_born);
+  if (ctx->pair.mem) hipFree(ctx->pair.mem);
   if (ctx->d_body) hipFree(ctx->d_body);
   if (ctx->d_q) hipFree(ctx->d_q);
   if (ctx->d_rec) hipFree(ctx->d_rec);
@@ -234,6 +240,30 @@ int pst_ctx_destroy(pst_ctx* ctx) {
 }
 
 const char* pst_last_error(const pst_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// Keep an interleaved float2 copy of the receptor potential and Born grids
+// when there is exactly one of each, both FP32, with identical geometry.
+static int update_pair(pst_ctx* ctx) {
+  if (ctx->pair.mem) {
+    hipFree(ctx->pair.mem);
+    ctx->pair.mem = nullptr;
+  }
+  if (ctx->elec.size() != 1 || ctx->born.size() != 1) return 0;
+  const GridDev& a = ctx->elec[0].g;
+  const GridDev& b = ctx->born[0].g;
+  if (a.dtype != 0 || b.dtype != 0 || a.nx != b.nx || a.ny != b.ny || a.nz != b.nz) return 0;
+  for (int i = 0; i < 3; ++i)
+    if (a.o[i] != b.o[i] || a.sp[i] != b.sp[i]) return 0;
+  const long n = (long)a.nx * a.ny * a.nz;
+  CK(hipMalloc(&ctx->pair.mem, (size_t)n * sizeof(float2)));
+  CK(launch_interleave((const float*)a.data, (const float*)b.data, (float2*)ctx->pair.mem, n,
+                       ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->pair.g = a;
+  ctx->pair.g.data = ctx->pair.mem;
+  ctx->pair.g.dtype = 2;
+  return 0;
+}
 
 int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int nx, int ny, int nz,
                  const double* origin3, const double* spacing3) {
@@ -269,7 +299,7 @@ int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int n
     d.g.hi[i] = origin3[i] + (n[i] - 1 - 3.0) * spacing3[i];
   }
   dst.push_back(d);
-  return 0;
+  return update_pair(ctx);
 }
 
 int pst_clear_grids(pst_ctx* ctx) {
@@ -279,7 +309,7 @@ int pst_clear_grids(pst_ctx* ctx) {
   free_grids(ctx->elec);
   free_grids(ctx->born);
   free_grids(ctx->lig_born);
-  return 0;
+  return update_pair(ctx);
 }
 
 int pst_set_farfield(pst_ctx* ctx, double receptor_charge, double debye_length, double sdie,
@@ -562,7 +592,7 @@ int pst_sim_destroy(pst_sim* s) {
   void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
                   s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
-                  s->audit_F};
+                  s->audit_F, s->key, s->bins};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete s;
@@ -779,6 +809,7 @@ static ProdBufs prod_bufs(pst_sim* s) {
   ProdBufs B;
   B.pos = s->pos; B.quat = s->quat; B.status = s->status; B.n_steps = s->n_steps32;
   B.traj = s->traj; B.plain_list = s->plain_list; B.mlp_list = s->mlp_list;
+  B.key = s->key; B.bins = s->bins;
   B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
   B.ctr = s->ctr; B.out_status = s->out_status; B.out_steps = s->out_steps;
   B.n_slots = s->S;
@@ -796,7 +827,10 @@ int pst_prod_init(pst_sim* s, int record) {
     CK(hipMalloc(&s->plain_list, S * 4));
     CK(hipMalloc(&s->mlp_list, S * 4));
     CK(hipMalloc(&s->ctr, sizeof(RunCounters)));
+    CK(hipMalloc(&s->key, S * 4));
+    CK(hipMalloc(&s->bins, 512 * 4));
   }
+  CK(hipMemsetAsync(s->bins, 0, 512 * 4, ctx->stream));
   if (s->out_status) { hipFree(s->out_status); s->out_status = nullptr; }
   if (s->out_steps) { hipFree(s->out_steps); s->out_steps = nullptr; }
   if (record) {
@@ -816,10 +850,12 @@ static int enqueue_step(pst_sim* s, int par) {
   pst_ctx* ctx = s->ctx;
   ProdBufs B = prod_bufs(s);
   CK(prod_step_pre(s->P, B, par, ctx->stream));
+  CK(prod_step_sort(B, ctx->stream));
   ForceWork W;
   memset(&W, 0, sizeof(W));
   W.list = s->plain_list;
   W.n_dev = &s->ctr->n_sr;
+  W.xcd_chunks = 1;
   W.pos = s->pos;
   W.quat = s->quat;
   W.F = s->F; W.T = s->T; W.E = nullptr;
@@ -837,6 +873,7 @@ static int enqueue_step(pst_sim* s, int par) {
       A.n_dev = &s->ctr->n_mlp;
       A.n_cap = s->sp.audit_n;
       A.out_compact = 1;
+      A.xcd_chunks = 0;
       A.F = s->audit_F;
       A.T = nullptr;
       CK(launch_forces(S, A, s->precision, group_for(ctx, S), s->sp.audit_n, 2048, ctx->stream));

@@ -103,24 +103,48 @@ __device__ inline double norm3(double x, double y, double z) { return sqrt(x * x
 // Trilinear gather, batch_force_kernel (gpu_batch_engine.py:95-110): cell
 // index floor((p - o) / sp); value 0 when the cell is outside [0, n-2].
 // Index arithmetic is FP64; weights and sums are in TC (float or double).
+// The value type V is TC for one grid, or Pair<TC> when the receptor
+// potential and Born grids share a geometry and are stored interleaved
+// (float2 per node): one stencil then fetches both fields from the same lines.
 // ---------------------------------------------------------------------------
-template <typename TG, typename TC>
-__device__ inline TC grid_interp(const GridDev& g, double x, double y, double z) {
+template <typename T> struct Pair { T a, b; };
+template <typename T> __device__ inline Pair<T> operator+(Pair<T> x, Pair<T> y) { return {x.a + y.a, x.b + y.b}; }
+template <typename T> __device__ inline Pair<T> operator-(Pair<T> x, Pair<T> y) { return {x.a - y.a, x.b - y.b}; }
+template <typename T> __device__ inline Pair<T> operator*(Pair<T> x, T s) { return {x.a * s, x.b * s}; }
+template <typename T> __device__ inline Pair<T> operator*(T s, Pair<T> x) { return {x.a * s, x.b * s}; }
+template <typename T> __device__ inline Pair<T> operator/(Pair<T> x, T s) { return {x.a / s, x.b / s}; }
+
+template <typename V, typename TG> __device__ inline V gld(const TG* d, long i);
+template <> __device__ inline double gld<double, double>(const double* d, long i) { return d[i]; }
+template <> __device__ inline double gld<double, float>(const float* d, long i) { return (double)d[i]; }
+template <> __device__ inline float gld<float, float>(const float* d, long i) { return d[i]; }
+template <> __device__ inline float gld<float, double>(const double* d, long i) { return (float)d[i]; }
+template <> __device__ inline Pair<float> gld<Pair<float>, float2>(const float2* d, long i) {
+  const float2 v = d[i];
+  return {v.x, v.y};
+}
+template <> __device__ inline Pair<double> gld<Pair<double>, float2>(const float2* d, long i) {
+  const float2 v = d[i];
+  return {(double)v.x, (double)v.y};
+}
+
+template <typename TG, typename TC, typename V>
+__device__ inline V grid_interp(const GridDev& g, double x, double y, double z) {
   const double ix = (x - g.o[0]) * g.inv[0];
   const double iy = (y - g.o[1]) * g.inv[1];
   const double iz = (z - g.o[2]) * g.inv[2];
   const double fi = floor(ix), fj = floor(iy), fk = floor(iz);
   const int i0 = (int)fi, j0 = (int)fj, k0 = (int)fk;
   if (i0 < 0 || i0 >= g.nx - 1 || j0 < 0 || j0 >= g.ny - 1 || k0 < 0 || k0 >= g.nz - 1)
-    return (TC)0;
+    return V{};
   const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
   const TG* __restrict__ d = (const TG*)g.data;
-  const size_t sxy = (size_t)g.ny * g.nz;
-  const size_t base = (size_t)i0 * sxy + (size_t)j0 * g.nz + k0;
-  const TC c000 = (TC)d[base], c001 = (TC)d[base + 1];
-  const TC c010 = (TC)d[base + g.nz], c011 = (TC)d[base + g.nz + 1];
-  const TC c100 = (TC)d[base + sxy], c101 = (TC)d[base + sxy + 1];
-  const TC c110 = (TC)d[base + sxy + g.nz], c111 = (TC)d[base + sxy + g.nz + 1];
+  const long sxy = (long)g.ny * g.nz;
+  const long base = (long)i0 * sxy + (long)j0 * g.nz + k0;
+  const V c000 = gld<V>(d, base), c001 = gld<V>(d, base + 1);
+  const V c010 = gld<V>(d, base + g.nz), c011 = gld<V>(d, base + g.nz + 1);
+  const V c100 = gld<V>(d, base + sxy), c101 = gld<V>(d, base + sxy + 1);
+  const V c110 = gld<V>(d, base + sxy + g.nz), c111 = gld<V>(d, base + sxy + g.nz + 1);
   const TC gx = (TC)1 - fx, gy = (TC)1 - fy, gz = (TC)1 - fz;
   return c000 * gx * gy * gz + c100 * fx * gy * gz + c010 * gx * fy * gz + c001 * gx * gy * fz +
          c110 * fx * fy * gz + c101 * fx * gy * fz + c011 * gx * fy * fz + c111 * fx * fy * fz;
@@ -131,18 +155,19 @@ __device__ inline bool grid_inside(const GridDev& g, double x, double y, double 
   return x > g.lo[0] && y > g.lo[1] && z > g.lo[2] && x < g.hi[0] && y < g.hi[1] && z < g.hi[2];
 }
 
-// phi and grad phi by central differences at +-sp/2 (gpu_batch_engine.py:111-115).
-template <typename TG, typename TC>
-__device__ inline void grid_phi_grad(const GridDev& g, double px, double py, double pz, TC& phi,
-                                     TC& gx, TC& gy, TC& gz) {
+// phi and grad phi by central differences at +-sp/2 (gpu_batch_engine.py:111-115),
+// seven interpolations exactly as the reference evaluates them.
+template <typename TG, typename TC, typename V>
+__device__ inline void grid_phi_grad(const GridDev& g, double px, double py, double pz, V& phi,
+                                     V& gx, V& gy, V& gz) {
   const double h0 = g.sp[0] * 0.5, h1 = g.sp[1] * 0.5, h2 = g.sp[2] * 0.5;
-  gx = (grid_interp<TG, TC>(g, px + h0, py, pz) - grid_interp<TG, TC>(g, px - h0, py, pz)) /
+  gx = (grid_interp<TG, TC, V>(g, px + h0, py, pz) - grid_interp<TG, TC, V>(g, px - h0, py, pz)) /
        (TC)(2 * h0);
-  gy = (grid_interp<TG, TC>(g, px, py + h1, pz) - grid_interp<TG, TC>(g, px, py - h1, pz)) /
+  gy = (grid_interp<TG, TC, V>(g, px, py + h1, pz) - grid_interp<TG, TC, V>(g, px, py - h1, pz)) /
        (TC)(2 * h1);
-  gz = (grid_interp<TG, TC>(g, px, py, pz + h2) - grid_interp<TG, TC>(g, px, py, pz - h2)) /
+  gz = (grid_interp<TG, TC, V>(g, px, py, pz + h2) - grid_interp<TG, TC, V>(g, px, py, pz - h2)) /
        (TC)(2 * h2);
-  phi = grid_interp<TG, TC>(g, px, py, pz);
+  phi = grid_interp<TG, TC, V>(g, px, py, pz);
 }
 
 // Same phi and central-difference gradient, evaluated from the 20 distinct
@@ -153,9 +178,9 @@ __device__ inline void grid_phi_grad(const GridDev& g, double px, double py, dou
 // are weighted, which keeps FP32 free of the cancellation of two nearly
 // equal interpolated values, and every node is loaded once.  Callers
 // guarantee the whole stencil is inside the grid (the 3-spacing valid box).
-template <typename TG, typename TC>
+template <typename TG, typename TC, typename V>
 __device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double py, double pz,
-                                             TC& phi, TC& gx, TC& gy, TC& gz) {
+                                             V& phi, V& gx, V& gy, V& gz) {
   const double ix = (px - g.o[0]) * g.inv[0];
   const double iy = (py - g.o[1]) * g.inv[1];
   const double iz = (pz - g.o[2]) * g.inv[2];
@@ -169,60 +194,60 @@ __device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double
   const long sy = g.nz, sx = (long)g.ny * g.nz;
   const long b = (long)i0 * sx + (long)j0 * sy + k0;
   // centre cell
-  const TC c000 = (TC)d[b], c001 = (TC)d[b + 1], c010 = (TC)d[b + sy], c011 = (TC)d[b + sy + 1];
-  const TC c100 = (TC)d[b + sx], c101 = (TC)d[b + sx + 1], c110 = (TC)d[b + sx + sy],
-           c111 = (TC)d[b + sx + sy + 1];
+  const V c000 = gld<V>(d, b), c001 = gld<V>(d, b + 1), c010 = gld<V>(d, b + sy),
+          c011 = gld<V>(d, b + sy + 1);
+  const V c100 = gld<V>(d, b + sx), c101 = gld<V>(d, b + sx + 1), c110 = gld<V>(d, b + sx + sy),
+          c111 = gld<V>(d, b + sx + sy + 1);
   // extra plane per axis: index -1 (a = -1) or +2 (a = 0)
   const long xo = ai < 0 ? -sx : 2 * sx, yo = aj < 0 ? -sy : 2 * sy, zo = ak < 0 ? -1 : 2;
-  const TC x00 = (TC)d[b + xo], x01 = (TC)d[b + xo + 1], x10 = (TC)d[b + xo + sy],
-           x11 = (TC)d[b + xo + sy + 1];
-  const TC y00 = (TC)d[b + yo], y01 = (TC)d[b + yo + 1], y10 = (TC)d[b + yo + sx],
-           y11 = (TC)d[b + yo + sx + 1];
-  const TC z00 = (TC)d[b + zo], z01 = (TC)d[b + zo + sy], z10 = (TC)d[b + zo + sx],
-           z11 = (TC)d[b + zo + sx + sy];
+  const V x00 = gld<V>(d, b + xo), x01 = gld<V>(d, b + xo + 1), x10 = gld<V>(d, b + xo + sy),
+          x11 = gld<V>(d, b + xo + sy + 1);
+  const V y00 = gld<V>(d, b + yo), y01 = gld<V>(d, b + yo + 1), y10 = gld<V>(d, b + yo + sx),
+          y11 = gld<V>(d, b + yo + sx + 1);
+  const V z00 = gld<V>(d, b + zo), z01 = gld<V>(d, b + zo + sy), z10 = gld<V>(d, b + zo + sx),
+          z11 = gld<V>(d, b + zo + sx + sy);
   const TC one = (TC)1;
   const TC gxw = one - fx, gyw = one - fy, gzw = one - fz;
   phi = c000 * gxw * gyw * gzw + c100 * fx * gyw * gzw + c010 * gxw * fy * gzw +
         c001 * gxw * gyw * fz + c110 * fx * fy * gzw + c101 * fx * gyw * fz +
         c011 * gxw * fy * fz + c111 * fx * fy * fz;
-  // x: nodes along x at (m, m+1, m+2) for the four (y,z) corners of the centre cell
   {
-    // value at x offset o in {-1,0,1,2} relative to i0, per (jj,kk) corner
-    const TC n0_00 = ai < 0 ? x00 : c000, n1_00 = ai < 0 ? c000 : c100, n2_00 = ai < 0 ? c100 : x00;
-    const TC n0_01 = ai < 0 ? x01 : c001, n1_01 = ai < 0 ? c001 : c101, n2_01 = ai < 0 ? c101 : x01;
-    const TC n0_10 = ai < 0 ? x10 : c010, n1_10 = ai < 0 ? c010 : c110, n2_10 = ai < 0 ? c110 : x10;
-    const TC n0_11 = ai < 0 ? x11 : c011, n1_11 = ai < 0 ? c011 : c111, n2_11 = ai < 0 ? c111 : x11;
+    // x: nodes at (m, m+1, m+2) for the four (y,z) corners of the centre cell
+    const V n0_00 = ai < 0 ? x00 : c000, n1_00 = ai < 0 ? c000 : c100, n2_00 = ai < 0 ? c100 : x00;
+    const V n0_01 = ai < 0 ? x01 : c001, n1_01 = ai < 0 ? c001 : c101, n2_01 = ai < 0 ? c101 : x01;
+    const V n0_10 = ai < 0 ? x10 : c010, n1_10 = ai < 0 ? c010 : c110, n2_10 = ai < 0 ? c110 : x10;
+    const V n0_11 = ai < 0 ? x11 : c011, n1_11 = ai < 0 ? c011 : c111, n2_11 = ai < 0 ? c111 : x11;
     const TC e1 = one - ex;
-    const TC d00 = e1 * (n1_00 - n0_00) + ex * (n2_00 - n1_00);
-    const TC d01 = e1 * (n1_01 - n0_01) + ex * (n2_01 - n1_01);
-    const TC d10 = e1 * (n1_10 - n0_10) + ex * (n2_10 - n1_10);
-    const TC d11 = e1 * (n1_11 - n0_11) + ex * (n2_11 - n1_11);
+    const V d00 = e1 * (n1_00 - n0_00) + ex * (n2_00 - n1_00);
+    const V d01 = e1 * (n1_01 - n0_01) + ex * (n2_01 - n1_01);
+    const V d10 = e1 * (n1_10 - n0_10) + ex * (n2_10 - n1_10);
+    const V d11 = e1 * (n1_11 - n0_11) + ex * (n2_11 - n1_11);
     gx = (d00 * gyw * gzw + d10 * fy * gzw + d01 * gyw * fz + d11 * fy * fz) * (TC)g.inv[0];
   }
   {
     // along y for the four (x,z) corners; y-extra loads are ordered (x0 z0, x0 z1, x1 z0, x1 z1)
-    const TC n0_00 = aj < 0 ? y00 : c000, n1_00 = aj < 0 ? c000 : c010, n2_00 = aj < 0 ? c010 : y00;
-    const TC n0_01 = aj < 0 ? y01 : c001, n1_01 = aj < 0 ? c001 : c011, n2_01 = aj < 0 ? c011 : y01;
-    const TC n0_10 = aj < 0 ? y10 : c100, n1_10 = aj < 0 ? c100 : c110, n2_10 = aj < 0 ? c110 : y10;
-    const TC n0_11 = aj < 0 ? y11 : c101, n1_11 = aj < 0 ? c101 : c111, n2_11 = aj < 0 ? c111 : y11;
+    const V n0_00 = aj < 0 ? y00 : c000, n1_00 = aj < 0 ? c000 : c010, n2_00 = aj < 0 ? c010 : y00;
+    const V n0_01 = aj < 0 ? y01 : c001, n1_01 = aj < 0 ? c001 : c011, n2_01 = aj < 0 ? c011 : y01;
+    const V n0_10 = aj < 0 ? y10 : c100, n1_10 = aj < 0 ? c100 : c110, n2_10 = aj < 0 ? c110 : y10;
+    const V n0_11 = aj < 0 ? y11 : c101, n1_11 = aj < 0 ? c101 : c111, n2_11 = aj < 0 ? c111 : y11;
     const TC e1 = one - ey;
-    const TC d00 = e1 * (n1_00 - n0_00) + ey * (n2_00 - n1_00);
-    const TC d01 = e1 * (n1_01 - n0_01) + ey * (n2_01 - n1_01);
-    const TC d10 = e1 * (n1_10 - n0_10) + ey * (n2_10 - n1_10);
-    const TC d11 = e1 * (n1_11 - n0_11) + ey * (n2_11 - n1_11);
+    const V d00 = e1 * (n1_00 - n0_00) + ey * (n2_00 - n1_00);
+    const V d01 = e1 * (n1_01 - n0_01) + ey * (n2_01 - n1_01);
+    const V d10 = e1 * (n1_10 - n0_10) + ey * (n2_10 - n1_10);
+    const V d11 = e1 * (n1_11 - n0_11) + ey * (n2_11 - n1_11);
     gy = (d00 * gxw * gzw + d10 * fx * gzw + d01 * gxw * fz + d11 * fx * fz) * (TC)g.inv[1];
   }
   {
     // along z for the four (x,y) corners; z-extra loads are ordered (x0 y0, x0 y1, x1 y0, x1 y1)
-    const TC n0_00 = ak < 0 ? z00 : c000, n1_00 = ak < 0 ? c000 : c001, n2_00 = ak < 0 ? c001 : z00;
-    const TC n0_01 = ak < 0 ? z01 : c010, n1_01 = ak < 0 ? c010 : c011, n2_01 = ak < 0 ? c011 : z01;
-    const TC n0_10 = ak < 0 ? z10 : c100, n1_10 = ak < 0 ? c100 : c101, n2_10 = ak < 0 ? c101 : z10;
-    const TC n0_11 = ak < 0 ? z11 : c110, n1_11 = ak < 0 ? c110 : c111, n2_11 = ak < 0 ? c111 : z11;
+    const V n0_00 = ak < 0 ? z00 : c000, n1_00 = ak < 0 ? c000 : c001, n2_00 = ak < 0 ? c001 : z00;
+    const V n0_01 = ak < 0 ? z01 : c010, n1_01 = ak < 0 ? c010 : c011, n2_01 = ak < 0 ? c011 : z01;
+    const V n0_10 = ak < 0 ? z10 : c100, n1_10 = ak < 0 ? c100 : c101, n2_10 = ak < 0 ? c101 : z10;
+    const V n0_11 = ak < 0 ? z11 : c110, n1_11 = ak < 0 ? c110 : c111, n2_11 = ak < 0 ? c111 : z11;
     const TC e1 = one - ez;
-    const TC d00 = e1 * (n1_00 - n0_00) + ez * (n2_00 - n1_00);
-    const TC d01 = e1 * (n1_01 - n0_01) + ez * (n2_01 - n1_01);
-    const TC d10 = e1 * (n1_10 - n0_10) + ez * (n2_10 - n1_10);
-    const TC d11 = e1 * (n1_11 - n0_11) + ez * (n2_11 - n1_11);
+    const V d00 = e1 * (n1_00 - n0_00) + ez * (n2_00 - n1_00);
+    const V d01 = e1 * (n1_01 - n0_01) + ez * (n2_01 - n1_01);
+    const V d10 = e1 * (n1_10 - n0_10) + ez * (n2_10 - n1_10);
+    const V d11 = e1 * (n1_11 - n0_11) + ez * (n2_11 - n1_11);
     gz = (d00 * gxw * gyw + d10 * fx * gyw + d01 * gxw * fy + d11 * fx * fy) * (TC)g.inv[2];
   }
 }
@@ -236,17 +261,34 @@ template <>
 __device__ inline void grid_phi_grad_any<double>(const GridDev& g, double px, double py, double pz,
                                                  double& phi, double& gx, double& gy, double& gz) {
   if (g.dtype == 1)
-    grid_phi_grad<double, double>(g, px, py, pz, phi, gx, gy, gz);
+    grid_phi_grad<double, double, double>(g, px, py, pz, phi, gx, gy, gz);
   else
-    grid_phi_grad<float, double>(g, px, py, pz, phi, gx, gy, gz);
+    grid_phi_grad<float, double, double>(g, px, py, pz, phi, gx, gy, gz);
 }
 template <>
 __device__ inline void grid_phi_grad_any<float>(const GridDev& g, double px, double py, double pz,
                                                 float& phi, float& gx, float& gy, float& gz) {
   if (g.dtype == 1)
-    grid_phi_grad_stencil<double, float>(g, px, py, pz, phi, gx, gy, gz);
+    grid_phi_grad_stencil<double, float, float>(g, px, py, pz, phi, gx, gy, gz);
   else
-    grid_phi_grad_stencil<float, float>(g, px, py, pz, phi, gx, gy, gz);
+    grid_phi_grad_stencil<float, float, float>(g, px, py, pz, phi, gx, gy, gz);
+}
+
+// Interleaved potential + Born grid (float2 per node): both fields at once.
+template <typename TC>
+__device__ inline void grid_phi_grad_pair(const GridDev& g, double px, double py, double pz,
+                                          Pair<TC>& phi, Pair<TC>& gx, Pair<TC>& gy, Pair<TC>& gz);
+template <>
+__device__ inline void grid_phi_grad_pair<double>(const GridDev& g, double px, double py, double pz,
+                                                  Pair<double>& phi, Pair<double>& gx,
+                                                  Pair<double>& gy, Pair<double>& gz) {
+  grid_phi_grad<float2, double, Pair<double>>(g, px, py, pz, phi, gx, gy, gz);
+}
+template <>
+__device__ inline void grid_phi_grad_pair<float>(const GridDev& g, double px, double py, double pz,
+                                                 Pair<float>& phi, Pair<float>& gx,
+                                                 Pair<float>& gy, Pair<float>& gz) {
+  grid_phi_grad_stencil<float2, float, Pair<float>>(g, px, py, pz, phi, gx, gy, gz);
 }
 
 template <typename TC> __device__ inline TC pst_exp(TC x);

@@ -28,6 +28,23 @@ template <> struct Vec4<float> { typedef float4 type; };
 template <typename TC>
 __device__ inline void atom_direct(const ForceSys& S, double px, double py, double pz, double q,
                                    bool grids, TC& fx, TC& fy, TC& fz, TC& e) {
+  if (S.paired) {
+    // single elec + Born grid pair: same box, one stencil for both fields
+    if (grids && grid_inside(S.pair, px, py, pz)) {
+      if (fabs(q) >= 1e-9) {
+        Pair<TC> phi, gx, gy, gz;
+        grid_phi_grad_pair<TC>(S.pair, px, py, pz, phi, gx, gy, gz);
+        const TC qq = (TC)q, c = (TC)(S.alpha * q * q);
+        fx -= qq * gx.a + c * gx.b;
+        fy -= qq * gy.a + c * gy.b;
+        fz -= qq * gz.a + c * gz.b;
+        e += qq * phi.a + c * phi.b;
+      }
+    } else if (S.has_yukawa && q != 0.0) {
+      yukawa_atom<TC>(S, (TC)px, (TC)py, (TC)pz, (TC)q, fx, fy, fz, e);
+    }
+    return;
+  }
   bool assigned = false;
   const int n_elec = grids ? S.n_elec : 0;
   const int n_born = grids ? S.n_born : 0;
@@ -102,9 +119,22 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
   const int gpb = kForceBlock / G;
   const int M = S.M;
   const double* lq = W.lig_q ? W.lig_q : S.lig_q;
-  for (int base = blockIdx.x * gpb; base < n; base += gridDim.x * gpb) {
+  // Work mapping.  Workgroups are dispatched to the 8 XCDs round-robin
+  // (workgroup b runs on XCD b % 8).  With xcd_chunks the list, sorted by
+  // direction, is cut into 8 contiguous chunks and XCD x walks chunk x, so
+  // the trajectories in flight on one XCD gather from one region of the grids
+  // and share its L2.
+  int lo = 0, hi = n, first = blockIdx.x * gpb, stride = gridDim.x * gpb;
+  if (W.xcd_chunks) {
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+    lo = (int)(((long)n * xcd) >> 3);
+    hi = (int)(((long)n * (xcd + 1)) >> 3);
+    first = lo + (blockIdx.x >> 3) * gpb;
+    stride = per * gpb;
+  }
+  for (int base = first; base < hi; base += stride) {
     const int w = base + grp;
-    const bool valid = w < n;
+    const bool valid = w < hi;
     const int t = valid ? (W.list ? W.list[w] : w) : 0;
     double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     double cx = 0, cy = 0, cz = 0;
@@ -262,6 +292,7 @@ static hipError_t launch_g(const ForceSys& S, const ForceWork& W, int max_n, int
   int blocks = (max_n + gpb - 1) / gpb;
   if (blocks > blocks_cap) blocks = blocks_cap;
   if (blocks < 1) blocks = 1;
+  if (W.xcd_chunks) blocks = (blocks + 7) / 8 * 8;
   const bool pose = W.quat != nullptr && W.lig_pos == nullptr;
   size_t lds = 0;
   if (pose && S.M <= kLdsAtomsMax) lds = (size_t)S.M * sizeof(typename Vec4<TC>::type);

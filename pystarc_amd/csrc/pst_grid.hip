@@ -61,6 +61,17 @@ __global__ void __launch_bounds__(256) k_grid(float* out32, double* out64, int n
   }
 }
 
+__global__ void k_interleave(const float* a, const float* b, float2* out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(a[i], b[i]);
+}
+
+hipError_t launch_interleave(const float* a, const float* b, float2* out, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_interleave, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out,
+                     n);
+  return hipGetLastError();
+}
+
 hipError_t launch_grid_dh(float* out, double* out64, int nx, int ny, int nz, const double* origin,
                           const double* sp, const double* atoms, int n_atoms, double lam,
                           double eps_out, double eps_in, int kind, hipStream_t st) {

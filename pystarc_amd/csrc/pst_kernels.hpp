@@ -40,6 +40,8 @@ hipError_t replay_bridge(const StepParams& P, const ReplayBufs& B, const double*
 struct ProdBufs {
   double* pos; double* quat; int* status; unsigned int* n_steps; long long* traj;
   int* plain_list; int* mlp_list;
+  int* key;                    // per-slot direction key of the grid-path list (-1: none)
+  int* bins;                   // [256] key histogram, [256..511] scatter cursors
   double* F; double* T; double* dt; double* x0; double* x1;
   RunCounters* ctr;
   signed char* out_status;     // per trajectory id (nullable)
@@ -48,9 +50,13 @@ struct ProdBufs {
 };
 hipError_t prod_init(const StepParams& P, const ProdBufs& B, hipStream_t st);
 hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
+hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st);
 hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs& B, int par,
                           hipStream_t st);
 hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
+
+// Interleave two FP32 grids of identical geometry into float2 nodes.
+hipError_t launch_interleave(const float* a, const float* b, float2* out, long n, hipStream_t st);
 
 // Synthetic grids.
 hipError_t launch_grid_dh(float* out, double* out64, int nx, int ny, int nz, const double* origin,

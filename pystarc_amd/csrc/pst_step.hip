@@ -459,8 +459,73 @@ __global__ void k_prod_pre(StepParams P, ProdBufs B, int par) {
     }
   }
   RunCounters* Cw = B.ctr;
-  wave_append(B.plain_list, &Cw->n_sr, s, live && !far);
   wave_append(B.mlp_list, &Cw->n_mlp, s, live && far);
+  // Grid-path trajectories get a direction key for the locality sort: lab
+  // octant of the centroid (bits 5-7, the receptor-grid region it gathers
+  // from) and the octant + dominant axis of the receptor direction in the
+  // ligand frame (bits 0-4, the ligand-grid region of its reverse Born).
+  __shared__ int hist[256];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  if (s < B.n_slots) {
+    int k = -1;
+    if (live && !far) {
+      const double* pos = B.pos + 3 * (size_t)s;
+      double R[9];
+      quat_to_rot(B.quat + 4 * (size_t)s, R);
+      const double cx = pos[0], cy = pos[1], cz = pos[2];
+      // d = R^T (-c)
+      const double dx = -(R[0] * cx + R[3] * cy + R[6] * cz);
+      const double dy = -(R[1] * cx + R[4] * cy + R[7] * cz);
+      const double dz = -(R[2] * cx + R[5] * cy + R[8] * cz);
+      const int lab = (cx > 0) | ((cy > 0) << 1) | ((cz > 0) << 2);
+      const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+      const int dom = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+      const int lig = ((dx > 0) | ((dy > 0) << 1) | ((dz > 0) << 2)) * 3 + dom;
+      k = lab * 32 + lig;
+      atomicAdd(&hist[k], 1);
+    }
+    B.key[s] = k;
+  }
+  __syncthreads();
+  if (hist[threadIdx.x]) atomicAdd(&B.bins[threadIdx.x], hist[threadIdx.x]);
+}
+
+// Exclusive scan of the key histogram into scatter cursors; sets the length
+// of the grid-path list and clears the histogram for the next step.
+__global__ void __launch_bounds__(256) k_bins_scan(ProdBufs B) {
+  __shared__ int sc[256];
+  const int t = threadIdx.x;
+  const int v = B.bins[t];
+  sc[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int add = t >= off ? sc[t - off] : 0;
+    __syncthreads();
+    sc[t] += add;
+    __syncthreads();
+  }
+  B.bins[256 + t] = sc[t] - v;
+  B.bins[t] = 0;
+  if (t == 255) B.ctr->n_sr = sc[255];
+}
+
+__global__ void k_bins_scatter(ProdBufs B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.n_slots) return;
+  const int k = B.key[s];
+  if (k < 0) return;
+  const int p = atomicAdd(&B.bins[256 + k], 1);
+  B.plain_list[p] = s;
+}
+
+hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_bins_scan, dim3(1), dim3(256), 0, st, B);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bins_scatter, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, B);
+  return hipGetLastError();
 }
 
 // Move every still-running slot (both lists).

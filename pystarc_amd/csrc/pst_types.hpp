@@ -28,6 +28,10 @@ struct ForceSys {
   GridDev elec[PST_MAX_GRIDS];
   GridDev born[PST_MAX_GRIDS];
   GridDev lig_born;
+  // one elec and one Born grid of identical geometry, interleaved float2
+  // (potential, Born) per node; the kernels then fetch both with one stencil
+  GridDev pair;
+  int paired;
   int n_elec, n_born, has_rev;
   double alpha;
   // Yukawa multipole far-field (gpu_batch_engine.py:161-175, :393-472)
@@ -83,6 +87,7 @@ struct ForceWork {
   int chunks_per_fb;
   int out_compact;          // 1: write outputs at list position, 0: at trajectory index
   int n_cap;                // > 0: process at most n_cap list entries
+  int xcd_chunks;           // 1: XCD x (= blockIdx % 8) takes the x-th eighth of the list
 };
 
 // Step parameters (gpu_batch_simulator.py:480-781).
