@@ -183,12 +183,12 @@ def cpu_baseline(sysm, grids, pos, quat, seconds=15.0, n_traj=128):
                       f"({steps} trajectory-steps, {wall:.1f}s, single-threaded NumPy FP64 oracle)"}
 
 
-def train_surrogate(sysm, eng64, args):
+def train_surrogate(sysm, eng64, args, extra_poses=None):
     from pystarc_amd.surrogate import build_surrogate
     t0 = time.time()
     s = build_surrogate(eng64, sysm, n_train=args.train_poses, epochs=args.train_epochs,
                         batch=args.train_batch, max_rows=max(args.slots, args.train_batch),
-                        log=log)
+                        log=log, extra_poses=extra_poses)
     log(f"surrogate trained in {time.time() - t0:.1f}s, validation force error {s.val_error:.3f}")
     return s
 
@@ -220,7 +220,7 @@ def main():
     ap.add_argument("--trajectories", type=int, default=10_000_000)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--paths", default="fp64,fp32,surrogate")
-    ap.add_argument("--kon-trajectories", type=int, default=100_000)
+    ap.add_argument("--kon-trajectories", type=int, default=200_000)
     ap.add_argument("--train-poses", type=int, default=4_000_000)
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--train-batch", type=int, default=4096)
@@ -249,10 +249,16 @@ def main():
     engines = {p: S.make_engine(sysm, grids, precision="fp64" if p == "fp64" else "fp32")
                for p in ("fp64", "fp32")}
     surr = None
-    if "surrogate" in paths:
-        surr = train_surrogate(sysm, engines["fp64"], args)
     results = {}
-    for p in paths:
+    # plain paths first: their live poses join the surrogate's training set
+    order = [p for p in paths if p != "surrogate"] + [p for p in paths if p == "surrogate"]
+    for p in order:
+        if p == "surrogate":
+            extra = None
+            if results:
+                extra = (np.concatenate([r["pos"] for r in results.values()]),
+                         np.concatenate([r["quat"] for r in results.values()]))
+            surr = train_surrogate(sysm, engines["fp64"], args, extra)
         eng = engines["fp32" if p == "surrogate" else p]
         log(f"path {p}: warmup {args.warmup}, timed {args.steps} steps, {args.slots} slots")
         r = run_window(sysm, eng, args, surr if p == "surrogate" else None)

@@ -92,12 +92,18 @@ class Surrogate:
         self.ctx.check(self.lib.pst_mlp_load(self.h, ptr(self.params.data)), "pst_mlp_load")
 
     # -- data -----------------------------------------------------------------------
-    def sample_poses(self, n, rng, r_lo=None, r_hi=None):
+    def sample_poses(self, n, rng, r_lo=None, r_hi=None, near_frac=0.0, near_width=20.0):
+        """Uniform directions and orientations; centre distance uniform in
+        [r_lo, r_hi], with a fraction `near_frac` drawn from the first
+        `near_width` A beyond r_lo, where the force (and its effect on k_on)
+        is largest."""
         r_lo = self.r_on if r_lo is None else r_lo
         r_hi = self.r_max if r_hi is None else r_hi
         d = rng.normal(size=(n, 3))
         d /= np.linalg.norm(d, axis=1, keepdims=True)
         r = rng.uniform(r_lo, r_hi, size=(n, 1))
+        near = rng.random(n) < near_frac
+        r[near, 0] = rng.uniform(r_lo, min(r_hi, r_lo + near_width), size=int(near.sum()))
         return d * r, _shoemake(rng.random((n, 3)))
 
     def features(self, c, q):
@@ -127,9 +133,15 @@ class Surrogate:
         _, parts = self.raw_targets(c, F, T, E)
         return np.concatenate([a / self.scale[k] for k, a in enumerate(parts)], axis=1).astype(np.float32)
 
-    def make_dataset(self, n, seed=0, extra_poses=None):
+    def sample_weights(self, c):
+        """Per-pose loss weight: the force envelope, normalised to mean 1, so
+        the fit is weighted like the physical force (as the step's audit is)."""
+        env = envelope(np.linalg.norm(c, axis=1), self.r0, self.lam)
+        return (env / env.mean()).astype(np.float32)
+
+    def make_dataset(self, n, seed=0, extra_poses=None, near_frac=0.5):
         rng = np.random.default_rng(seed)
-        c, q = self.sample_poses(n, rng)
+        c, q = self.sample_poses(n, rng, near_frac=near_frac)
         if extra_poses is not None:
             ce, qe = extra_poses
             keep = np.linalg.norm(ce, axis=1) > self.r_on
@@ -168,8 +180,9 @@ class Surrogate:
                      + [0.05] * 3 + [0.05])
         return w / w.sum() * 13.0
 
-    def train(self, X, Y, epochs=4, batch=8192, lr=2e-3, lr_end=5e-5, log=None):
-        """Adam on weighted MSE of normalised targets; X, Y device float32 tensors."""
+    def train(self, X, Y, epochs=4, batch=8192, lr=2e-3, lr_end=5e-5, log=None, W=None):
+        """Adam on weighted MSE of normalised targets; X, Y (and per-sample
+        weights W) are device float32 tensors."""
         torch = self.torch
         n = X.shape[0]
         opt = torch.optim.Adam([self.params], lr=lr)
@@ -186,6 +199,8 @@ class Surrogate:
                 y = self.forward(xb, train=True)
                 diff = y - yb
                 gy = (2.0 / (batch * 13)) * diff * wts
+                if W is not None:
+                    gy = gy * W[idx][:, None]
                 self.params.grad = self.backward(gy.contiguous())
                 for gparam in opt.param_groups:
                     gparam["lr"] = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * k / steps))
@@ -215,7 +230,7 @@ class Surrogate:
 
 
 def build_surrogate(engine, system, n_train=2_000_000, epochs=4, batch=8192, contact=None,
-                    shell=8.0, seed=0, log=None, max_rows=65536, extra_poses=None):
+                    shell=8.0, seed=0, log=None, max_rows=65536, extra_poses=None, lr=2e-3):
     """Train a surrogate for `system`: active beyond contact + shell (the
     contact distance defaults to the sum of the hydrodynamic radii)."""
     import torch
@@ -231,10 +246,11 @@ def build_surrogate(engine, system, n_train=2_000_000, epochs=4, batch=8192, con
     s.fit_scales(c, F, T, E)
     X = torch.as_tensor(s.features(c, q), device=engine.device)
     Y = torch.as_tensor(s.targets(c, F, T, E), device=engine.device)
+    Wt = torch.as_tensor(s.sample_weights(c), device=engine.device)
     if log:
         log(f"surrogate data: {len(c)} poses labelled by the FP64 grid path in {time.time() - t0:.1f}s")
-    s.train(X, Y, epochs=epochs, batch=batch, log=log)
-    cv, qv, Fv, _, _ = s.make_dataset(20000, seed=seed + 1)
+    s.train(X, Y, epochs=epochs, batch=batch, log=log, W=Wt, lr=lr)
+    cv, qv, Fv, _, _ = s.make_dataset(20000, seed=seed + 1, near_frac=0.0)
     s.val_error, _ = s.force_error(cv, qv, Fv)
     if log:
         log(f"surrogate validation force RMS relative error {s.val_error:.3e}")
