@@ -95,6 +95,28 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
                          double* T, double* E, int precision, int reps, double* ms_per_launch,
                          double* census4);
 
+/* ---- two-body RPY hydrodynamics ------------------------------------------
+ * Extends MobilityTensor / rpy_offdiagonal
+ * (pystarc/hydrodynamics/rotne_prager.py:67-124, :127-215), whose
+ * translational coupling feeds the scalar D_parallel of
+ * gpu_batch_simulator.py:1074-1084, to the full 12x12 grand mobility of the
+ * two spheres (Zuk et al. 2014 three regimes for tt, rr and rt couplings),
+ * order [x_rec, x_lig, theta_rec, theta_lig], 1/eta units, receptor at the
+ * origin.  The production step uses it when StepParams.hydro_mode == 1.
+ * pst_rpy12_eval writes, per pose (device pos (n,3)), the mobility M and the
+ * noise factor G (G G^T = M), each (n,12,12) row-major; either may be NULL.
+ * pst_rpy12_move applies one move with the step's own code: receptor load
+ * from F / T of the ligand (torque_mode 0: zero torques as the reference,
+ * 1: angular-momentum balance), dX = kT_over_eta dt M f
+ * + sqrt(2 kT_over_eta dt) G xi (xi (n,12) standard normals), then pos /
+ * quat (device, in-out) re-expressed in the turned receptor frame; dX (n,12)
+ * may be NULL. */
+int pst_rpy12_eval(pst_ctx* ctx, int n, const double* pos, double a_rec, double a_lig,
+                   double* mobility, double* factor);
+int pst_rpy12_move(pst_ctx* ctx, int n, double a_rec, double a_lig, double kT_over_eta,
+                   int torque_mode, const double* dt, double* pos, double* quat, const double* F,
+                   const double* T, const double* xi, double* dX);
+
 /* ---- simulator -----------------------------------------------------------
  * Replaces GPUBatchSimulator (pystarc/simulation/gpu_batch_simulator.py:143-1751).
  * `params` points to a pst_step_params struct whose layout is StepParams
@@ -134,6 +156,20 @@ int pst_prod_run(pst_sim* sim, long long max_steps, int graph_steps, double* cou
 int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
 /* Mean device time per step (ms) of the last pst_prod_run, HIP events. */
 int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);
+/* Autotuner of the production force launch (no reference counterpart: the
+ * reference launches batch_force_kernel with a fixed (16,16) block,
+ * gpu_batch_engine.py:328-332).  Candidates: lanes per trajectory
+ * G in {g0/4, g0/2, g0} (g0 = smallest power of two >= max(ligand atoms,
+ * reverse-Born receptor atoms), at most 64), resident-workgroup cap in
+ * {512 ... 8192} and the XCD-chunked list mapping on/off.  Each candidate runs
+ * `steps_per_cand` real production steps per round, `rounds` round-robin
+ * passes, force kernel timed by HIP events; the fastest becomes the launch of
+ * later steps.  table (max_rows,4) = {G, cap, xcd, mean ms} per candidate;
+ * best4 = {G, cap, xcd, steps run}.  Both may be NULL. */
+int pst_prod_autotune(pst_sim* sim, int rounds, int steps_per_cand, double* table, int max_rows,
+                      int* n_rows, int* best4);
+/* The same launch configuration set by hand (G = 0: heuristic). */
+int pst_prod_set_launch(pst_sim* sim, int G, int blocks_cap, int xcd);
 
 /* ---- learned surrogate ---------------------------------------------------
  * No reference counterpart: a 9 -> 4x256 (SiLU) -> 13 MLP standing in for

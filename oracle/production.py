@@ -1,0 +1,194 @@
+"""Per-trajectory restatement of the production step (oracle, test
+infrastructure only).
+
+The production kernels (refilling slot pool, Philox noise keyed by
+(trajectory, its own step, stream)) make every trajectory's fate a function
+of its id alone.  This oracle advances all trajectories in lock-step by their
+own step counter with the same physics as the reference's flattened-pairs
+step (pystarc/simulation/gpu_batch_simulator.py):
+
+  start          b-sphere position from normals, Shoemake quaternion (:509-516)
+  max steps      finish when the step counter reached max_steps (:1486)
+  reaction       pair distances at the current pose (:855-871)
+  escape/return  r >= r_esc: u < return_prob -> back to b with a new
+                 orientation, else escaped (:879-911)
+  force          oracle engine at the pose (:931-943)
+  dt, D          adaptive dt (:961-1067), D_parallel with HI (:1074-1084)
+  move           Ermak-McCammon + right-multiplied rotation (:1085-1101), or
+                 the two-body RPY move (oracle/rpy12.py) in hydro mode "rpy12"
+  overlap        centroid within 1.5 A of a receptor atom -> restore (:1106-1128)
+  bridge         p = exp(-x0 x1 / (D_eff dt)) for pairs above on both ends;
+                 fired = u < p or new distance < cutoff (:1142-1252)
+
+Draws, per trajectory id t at its step n (oracle/philox.py):
+  start   normal2(INIT, step 0xFFFFFFFF, sub 0, 1), uniform2(INIT, sub 2, 3)
+  escape  uniform2(ESC, n, 0)[0];  return quaternion uniform2(RETQ, n, 0), (.., 1)[0]
+  noise   normal2(NOISE, n, sub 0..2) -> (t0 t1) (t2 r0) (r1 r2); rpy12: sub 0..5 -> xi[0..11]
+  bridge  uniform2(BRIDGE, n, sub p >> 1) -> pairs p, p + 1
+
+The only coupling between trajectories in the device run is the batch-wide
+"any pair above on both ends" gate of the bridge (:1161); a trajectory that
+the gate skips and that reached its cutoffs is finished at the start of its
+next step instead, with the same step count, unless that next step hits
+max_steps first.  This oracle always evaluates the bridge.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import forces as F_
+from . import philox as PH
+from . import rpy12
+from .physics import apply_rotation, quats_to_rotmats, return_prob_and_kb, shoemake
+from .simulator import OracleSim
+
+INIT_STEP = 0xFFFFFFFF
+
+
+class ProductionOracle(OracleSim):
+    """``system`` / ``params`` as OracleSim; hydro "scalar" or "rpy12"."""
+
+    def __init__(self, system, params, torque_mode="reference", hydro="scalar",
+                 kT_over_eta=1.0 / 0.243):
+        super().__init__(system, params, torque_mode)
+        self.hydro = hydro
+        self.Df = float(kT_over_eta)
+        self.a0, self.a1 = float(system["a_rec"]), float(system["a_lig"])
+        self.seed = int(params["seed"]) & ((1 << 64) - 1)
+        self.max_steps = int(params.get("max_steps", 10 ** 9))
+        N = self.N
+        t = np.arange(N)
+        z0, z1 = PH.normal2(self.seed, t, INIT_STEP, PH.RNG_INIT, 0)
+        z2, _ = PH.normal2(self.seed, t, INIT_STEP, PH.RNG_INIT, 1)
+        u0, u1 = PH.uniform2(self.seed, t, INIT_STEP, PH.RNG_INIT, 2)
+        u2, _ = PH.uniform2(self.seed, t, INIT_STEP, PH.RNG_INIT, 3)
+        v = np.stack([z0, z1, z2], axis=1)
+        self.pos = v / np.linalg.norm(v, axis=1, keepdims=True) * self.r_b
+        self.q = _shoemake(u0, u1, u2)
+        self.status = np.zeros(N, dtype=np.int32)
+        self.n_steps = np.zeros(N, dtype=np.int64)
+        self.out_steps = np.full(N, -1, dtype=np.int64)
+        self.total_steps = 0
+        self.n_returns = 0
+        self.n_overlap_rejected = 0
+
+    def _finish(self, idx, code, steps):
+        self.status[idx] = code
+        self.out_steps[idx] = steps
+
+    def step(self):
+        st = self.status
+        run = np.where(st == 0)[0]
+        if len(run) == 0:
+            return False
+        ns = self.n_steps[run]
+        # max steps
+        mx = ns >= self.max_steps
+        self._finish(run[mx], 3, ns[mx])
+        run, ns = run[~mx], ns[~mx]
+        P = len(self.cut)
+        if P and len(run):
+            R = quats_to_rotmats(self.q[run])
+            d = self.pair_dists(R, self.pos[run])
+            re = (d < self.cut[None, :]).sum(axis=1) >= self.n_needed
+            self._finish(run[re], 1, ns[re])
+            run, ns = run[~re], ns[~re]
+        # escape / return
+        r = np.linalg.norm(self.pos[run], axis=1)
+        at = r >= self.r_esc
+        if at.any():
+            ia, na = run[at], ns[at]
+            u, _ = PH.uniform2(self.seed, ia, na, PH.RNG_ESC, 0)
+            ret = u < self.return_prob
+            ir, nr_ = ia[ret], na[ret]
+            if len(ir):
+                p = self.pos[ir]
+                self.pos[ir] = p / np.linalg.norm(p, axis=1, keepdims=True) * self.r_b
+                a0, a1 = PH.uniform2(self.seed, ir, nr_, PH.RNG_RETQ, 0)
+                a2, _ = PH.uniform2(self.seed, ir, nr_, PH.RNG_RETQ, 1)
+                self.q[ir] = _shoemake(a0, a1, a2)
+                self.n_returns += len(ir)
+            self._finish(ia[~ret], 2, na[~ret])
+            keep = np.ones(len(run), dtype=bool)
+            keep[np.where(at)[0][~ret]] = False
+            run, ns = run[keep], ns[keep]
+        n = len(run)
+        if n == 0:
+            return True
+        pos0 = self.pos[run].copy()
+        q0 = self.q[run].copy()
+        R = quats_to_rotmats(q0)
+        Fo, To = self.forces(R, pos0)
+        old_d = self.pair_dists(R, pos0) if P else None
+        r = np.linalg.norm(pos0, axis=1)
+        dt = self.adaptive_dt(r)
+        if self.hydro == "rpy12":
+            xi = np.empty((n, 12))
+            for k in range(6):
+                xi[:, 2 * k], xi[:, 2 * k + 1] = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, k)
+            tm = 0 if self.torque_mode == "reference" else 1
+            FR, TR = rpy12.receptor_load_v(pos0, Fo, To, tm)
+            dX = rpy12.displacement_v(pos0, Fo, To, FR, TR, xi, dt, self.Df, self.a0, self.a1)
+            newp, newq = rpy12.move_v(pos0, q0, dX)
+        else:
+            D = self.diffusion(r)
+            a, b = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, 0)
+            c, e = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, 1)
+            f, g = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, 2)
+            nt = np.stack([a, b, c], axis=1)
+            nrot = np.stack([e, f, g], axis=1)
+            newp = pos0 + (D * dt)[:, None] * Fo + np.sqrt(2.0 * D * dt)[:, None] * nt
+            sr = np.sqrt(2.0 * self.D_r * dt)[:, None]
+            newq = apply_rotation(q0, self.D_r * dt[:, None] * To + sr * nrot)
+        if self.use_overlap:
+            dmin = np.full(n, np.inf)
+            for c0 in range(0, self.rec_pos.shape[0], 4096):
+                dd = np.linalg.norm(newp[:, None, :] - self.rec_pos[None, c0:c0 + 4096, :], axis=2)
+                dmin = np.minimum(dmin, dd.min(axis=1))
+            bad = dmin < 1.5
+            newp[bad] = pos0[bad]
+            newq[bad] = q0[bad]
+            self.n_overlap_rejected += int(bad.sum())
+        self.pos[run] = newp
+        self.q[run] = newq
+        self.total_steps += n
+        if P:
+            new_d = self.pair_dists(quats_to_rotmats(newq), newp)
+            x0 = old_d - self.cut[None, :]
+            x1 = new_d - self.cut[None, :]
+            u = np.empty((n, P))
+            for p in range(0, P, 2):
+                ua, ub = PH.uniform2(self.seed, run, ns, PH.RNG_BRIDGE, p >> 1)
+                u[:, p] = ua
+                if p + 1 < P:
+                    u[:, p + 1] = ub
+            Dv = self.bb_D_eff if self.bb_D_eff is not None else np.full(P, self.D_t)
+            Ddt = np.maximum(Dv[None, :] * dt[:, None], 1e-30)
+            both = (x0 > 0) & (x1 > 0)
+            with np.errstate(over="ignore"):
+                pc = np.where(both, np.exp(-x0 * x1 / Ddt), 0.0)
+            fired = (u < pc) | (x1 < 0.0)
+            bb = fired.sum(axis=1) >= self.n_needed
+            self._finish(run[bb], 1, ns[bb] + 1)
+        self.n_steps[run] = ns + 1
+        return True
+
+    def run(self, max_loop=None):
+        k = 0
+        while self.step():
+            k += 1
+            if max_loop is not None and k >= max_loop:
+                break
+        return self.counts()
+
+    def counts(self):
+        return {"n_reacted": int((self.status == 1).sum()),
+                "n_escaped": int((self.status == 2).sum()),
+                "n_max_steps": int((self.status == 3).sum()),
+                "total_steps": int(self.total_steps)}
+
+
+def _shoemake(u0, u1, u2):
+    return shoemake(np.stack([u0, u1, u2], axis=1))

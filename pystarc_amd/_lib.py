@@ -68,6 +68,9 @@ _SIGS = {
     "pst_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int], C.c_int),
     "pst_time_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp],
                              C.c_int),
+    "pst_rpy12_eval": ([_vp, C.c_int, _vp, C.c_double, C.c_double, _vp, _vp], C.c_int),
+    "pst_rpy12_move": ([_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp, _vp, _vp,
+                        _vp, _vp, _vp, _vp], C.c_int),
     "pst_sim_create": ([_vp, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
     "pst_sim_destroy": ([_vp], C.c_int),
     "pst_sim_set_state": ([_vp, C.c_int, _vp, _vp], C.c_int),
@@ -80,6 +83,8 @@ _SIGS = {
     "pst_prod_init": ([_vp, C.c_int], C.c_int),
     "pst_prod_run": ([_vp, C.c_longlong, C.c_int, _vp], C.c_int),
     "pst_prod_outcomes": ([_vp, _vp, _vp], C.c_int),
+    "pst_prod_autotune": ([_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp], C.c_int),
+    "pst_prod_set_launch": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
     "pst_prod_last_ms_per_step": ([_vp, _dp], C.c_int),
     "pst_make_grid": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double,
                        C.c_double, C.c_double, C.c_int, _vp], C.c_int),

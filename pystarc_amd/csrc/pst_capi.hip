@@ -95,6 +95,11 @@ struct pst_sim {
   SurrParams sp;
   double* audit_F = nullptr;
   int audit_cap = 0;
+  // force-launch configuration of the production step (autotuned or default)
+  int tune_G = 0;          // lanes per trajectory; 0 = pick_group heuristic
+  int tune_cap = 2048;     // cap on resident workgroups (grid-stride beyond)
+  int tune_xcd = 1;        // XCD-chunked mapping of the sorted grid-path list
+  hipEvent_t tev0 = nullptr, tev1 = nullptr;  // brackets the force launch while tuning
 };
 
 #define CK(expr)                                                                      \
@@ -553,6 +558,35 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
 }
 
 // ----------------------------------------------------------------------------
+// two-body RPY mobility
+// ----------------------------------------------------------------------------
+int pst_rpy12_eval(pst_ctx* ctx, int n, const double* pos, double a_rec, double a_lig,
+                   double* mobility, double* factor) {
+  if (!ctx) return 1;
+  if (n < 0 || !(a_rec > 0) || !(a_lig > 0)) FAIL("pst_rpy12_eval: bad arguments");
+  if (n == 0) return 0;
+  if (!pos || (!mobility && !factor)) FAIL("pst_rpy12_eval: null pointer");
+  CK(set_dev(ctx));
+  CK(launch_rpy_eval(n, pos, a_rec, a_lig, mobility, factor, ctx->stream));
+  return 0;
+}
+
+int pst_rpy12_move(pst_ctx* ctx, int n, double a_rec, double a_lig, double kT_over_eta,
+                   int torque_mode, const double* dt, double* pos, double* quat, const double* F,
+                   const double* T, const double* xi, double* dX) {
+  if (!ctx) return 1;
+  if (n < 0 || !(a_rec > 0) || !(a_lig > 0) || !(kT_over_eta > 0))
+    FAIL("pst_rpy12_move: bad arguments");
+  if (torque_mode != 0 && torque_mode != 1) FAIL("pst_rpy12_move: torque_mode must be 0 or 1");
+  if (n == 0) return 0;
+  if (!dt || !pos || !quat || !F || !T || !xi) FAIL("pst_rpy12_move: null pointer");
+  CK(set_dev(ctx));
+  CK(launch_rpy_move(n, a_rec, a_lig, kT_over_eta, torque_mode, dt, pos, quat, F, T, xi, dX,
+                     ctx->stream));
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
 // simulator
 // ----------------------------------------------------------------------------
 int pst_sim_create(pst_ctx* ctx, const void* params, int precision, pst_sim** out) {
@@ -563,6 +597,9 @@ int pst_sim_create(pst_ctx* ctx, const void* params, int precision, pst_sim** ou
   if (P->n_slots <= 0 || P->n_slots > (1ll << 30)) FAIL("pst_sim_create: n_slots out of range");
   if (P->n_pairs < 0 || P->n_pairs > PST_MAX_PAIRS) FAIL("pst_sim_create: too many reaction pairs");
   if (ctx->M <= 0) FAIL("pst_sim_create: ligand not set");
+  if (P->hydro_mode != 0 && P->hydro_mode != 1) FAIL("pst_sim_create: hydro_mode must be 0 or 1");
+  if (P->hydro_mode == 1 && !(P->a_rec > 0 && P->a_lig > 0 && P->kT_over_eta > 0))
+    FAIL("pst_sim_create: hydro_mode 1 needs a_rec, a_lig and kT_over_eta > 0");
   CK(set_dev(ctx));
   pst_sim* s = new pst_sim();
   s->ctx = ctx;
@@ -679,6 +716,7 @@ int pst_sim_get_state(pst_sim* s, double* pos, double* quat, int* status, long l
 int pst_replay_begin(pst_sim* s, int* n_run, int* n_esc, int* n_react) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
+  if (s->P.hydro_mode != 0) FAIL("pst_replay_begin: replay follows the scalar diffusion (hydro_mode 0)");
   CK(set_dev(ctx));
   if (ensure_replay(s)) return 2;
   ReplayBufs B = replay_bufs(s);
@@ -855,14 +893,17 @@ static int enqueue_step(pst_sim* s, int par) {
   memset(&W, 0, sizeof(W));
   W.list = s->plain_list;
   W.n_dev = &s->ctr->n_sr;
-  W.xcd_chunks = 1;
+  W.xcd_chunks = s->tune_xcd;
   W.pos = s->pos;
   W.quat = s->quat;
   W.F = s->F; W.T = s->T; W.E = nullptr;
   W.rev_skip_mode = 0;
   ForceSys S;
   fill_sys(ctx, S);
-  CK(launch_forces(S, W, s->precision, group_for(ctx, S), s->S, 2048, ctx->stream));
+  const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
+  if (s->tev0) CK(hipEventRecord(s->tev0, ctx->stream));
+  CK(launch_forces(S, W, s->precision, G, s->S, s->tune_cap, ctx->stream));
+  if (s->tev1) CK(hipEventRecord(s->tev1, ctx->stream));
   if (s->mlp && s->P.surrogate_r > 0) {
     CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
                  ctx->stream));
@@ -951,6 +992,102 @@ int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) 
                     (double)h.audit_n, 0};
     memcpy(c16, v, sizeof(v));
   }
+  return 0;
+}
+
+// Autotuner of the production force launch: lanes per trajectory G (atoms
+// per wave slice), the resident-workgroup cap (tile count of the persistent
+// grid-stride launch) and the XCD-chunked list mapping.  Every candidate runs
+// real production steps (plain launches, round-robin over `rounds` so that a
+// slowly drifting population is seen by all), its force kernel bracketed by
+// HIP events; the fastest total becomes the configuration of later steps.
+int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table, int max_rows,
+                      int* n_rows, int* best4) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_autotune: call pst_prod_init first");
+  if (rounds <= 0 || steps_per_cand <= 0) FAIL("pst_prod_autotune: bad arguments");
+  CK(set_dev(ctx));
+  ForceSys S;
+  fill_sys(ctx, S);
+  const int g0 = group_for(ctx, S);
+  std::vector<int> Gs;
+  for (int g = g0 / 4; g <= g0; g *= 2)
+    if (g >= 1) Gs.push_back(g);
+  if (Gs.empty()) Gs.push_back(g0);
+  const int caps[] = {512, 1024, 2048, 4096, 8192};
+  struct Cand { int G, cap, xcd; double ms; int n; };
+  std::vector<Cand> cands;
+  for (int g : Gs)
+    for (int c : caps)
+      for (int x = 0; x < 2; ++x) cands.push_back({g, c, x, 0.0, 0});
+  CK(hipEventCreate(&s->tev0));
+  CK(hipEventCreate(&s->tev1));
+  RunCounters h;
+  memset(&h, 0, sizeof(h));
+  bool stop = false;
+  long long k = 0;
+  int rc = 0;
+  for (int r = 0; r < rounds && !stop && !rc; ++r) {
+    for (auto& c : cands) {
+      if (stop || rc) break;
+      s->tune_G = c.G; s->tune_cap = c.cap; s->tune_xcd = c.xcd;
+      for (int i = 0; i < steps_per_cand && !stop; ++i) {
+        rc = enqueue_step(s, (int)(k++ & 1));
+        if (rc) break;
+        hipError_t e = hipEventSynchronize(s->tev1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->tev0, s->tev1);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = 2; break; }
+        c.ms += ms;
+        c.n += 1;
+        if (h.stop || (h.n_sr + h.n_mlp) == 0) stop = true;
+      }
+    }
+  }
+  hipEventDestroy(s->tev0);
+  hipEventDestroy(s->tev1);
+  s->tev0 = s->tev1 = nullptr;
+  // choose among candidates measured in every round-robin pass
+  int nmax = 0;
+  for (auto& c : cands) nmax = c.n > nmax ? c.n : nmax;
+  const Cand* best = nullptr;
+  for (auto& c : cands)
+    if (c.n == nmax && nmax > 0 && (!best || c.ms < best->ms)) best = &c;
+  if (best) {
+    s->tune_G = best->G; s->tune_cap = best->cap; s->tune_xcd = best->xcd;
+  } else {
+    s->tune_G = 0; s->tune_cap = 2048; s->tune_xcd = 1;
+  }
+  if (rc) return rc;
+  if (table && n_rows) {
+    int m = 0;
+    for (auto& c : cands) {
+      if (m >= max_rows) break;
+      table[4 * m] = c.G; table[4 * m + 1] = c.cap; table[4 * m + 2] = c.xcd;
+      table[4 * m + 3] = c.n ? c.ms / c.n : -1.0;
+      ++m;
+    }
+    *n_rows = m;
+  }
+  if (best4) {
+    best4[0] = s->tune_G > 0 ? s->tune_G : g0; best4[1] = s->tune_cap; best4[2] = s->tune_xcd;
+    best4[3] = (int)k;
+  }
+  return 0;
+}
+
+// Force-launch configuration of the production step, set by hand.
+int pst_prod_set_launch(pst_sim* s, int G, int blocks_cap, int xcd) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!(G == 0 || G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64))
+    FAIL("pst_prod_set_launch: G must be 0 or a power of two up to 64");
+  if (blocks_cap < 1) FAIL("pst_prod_set_launch: blocks_cap must be positive");
+  s->tune_G = G; s->tune_cap = blocks_cap; s->tune_xcd = xcd ? 1 : 0;
   return 0;
 }
 

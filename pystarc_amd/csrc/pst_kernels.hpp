@@ -55,6 +55,13 @@ hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs
                           hipStream_t st);
 hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
 
+// Two-body RPY: 12x12 mobility / noise factor columns, and the move itself.
+hipError_t launch_rpy_eval(int n, const double* pos, double a0, double a1, double* mob,
+                           double* fac, hipStream_t st);
+hipError_t launch_rpy_move(int n, double a0, double a1, double Df, int torque_mode,
+                           const double* dt, double* pos, double* quat, const double* F,
+                           const double* T, const double* xi, double* dX_out, hipStream_t st);
+
 // Interleave two FP32 grids of identical geometry into float2 nodes.
 hipError_t launch_interleave(const float* a, const float* b, float2* out, long n, hipStream_t st);
 

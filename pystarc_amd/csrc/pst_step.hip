@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include "pst_device.hpp"
 #include "pst_kernels.hpp"
+#include "pst_rpy.hpp"
 
 namespace pst {
 
@@ -96,35 +97,33 @@ __device__ inline bool overlaps(const StepParams& P, const ForceSys& S, double c
   return sqrt(m2) < P.overlap_thresh;
 }
 
-// Translate + rotate + overlap + new pair distances.  Returns whether any
-// pair is above its cutoff both before and after the move.
-__device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* pos, double* q,
-                                const double* Fv, const double* Tv, const double* nt,
-                                const double* nr, double dt, double D, double* x0, double* x1,
-                                bool* rejected) {
+// Pair distances before a move; keeps the pose for an overlap rejection.
+struct MoveSave { double p[3], q[4]; };
+
+__device__ inline void move_begin(const StepParams& P, const double* pos, const double* q,
+                                  double* x0, MoveSave& sv) {
   double R[9], d[PST_MAX_PAIRS];
   quat_to_rot(q, R);
   pair_dists(P, R, pos[0], pos[1], pos[2], d);
   for (int p = 0; p < P.n_pairs; ++p) x0[p] = d[p] - P.cutoff[p];
-  const double op0 = pos[0], op1 = pos[1], op2 = pos[2];
-  const double oq0 = q[0], oq1 = q[1], oq2 = q[2], oq3 = q[3];
-  // Ermak-McCammon translation (:1085-1094)
-  const double sig = sqrt(2.0 * D * dt);
-  pos[0] += D * dt * Fv[0] + sig * nt[0];
-  pos[1] += D * dt * Fv[1] + sig * nt[1];
-  pos[2] += D * dt * Fv[2] + sig * nt[2];
-  // rotation (:1095-1101)
-  const double sr = sqrt(2.0 * P.D_r * dt);
-  apply_rotation(q, P.D_r * dt * Tv[0] + sr * nr[0], P.D_r * dt * Tv[1] + sr * nr[1],
-                 P.D_r * dt * Tv[2] + sr * nr[2]);
+  sv.p[0] = pos[0]; sv.p[1] = pos[1]; sv.p[2] = pos[2];
+  sv.q[0] = q[0]; sv.q[1] = q[1]; sv.q[2] = q[2]; sv.q[3] = q[3];
+}
+
+// Overlap rejection (:1106-1128) and new pair distances.  Returns whether any
+// pair is above its cutoff both before and after the move.
+__device__ inline bool move_end(const StepParams& P, const ForceSys& S, double* pos, double* q,
+                                const MoveSave& sv, const double* x0, double* x1,
+                                bool* rejected) {
   *rejected = false;
   if (P.use_overlap && S.n_rec > 0 && overlaps(P, S, pos[0], pos[1], pos[2])) {
-    pos[0] = op0; pos[1] = op1; pos[2] = op2;
-    q[0] = oq0; q[1] = oq1; q[2] = oq2; q[3] = oq3;
+    pos[0] = sv.p[0]; pos[1] = sv.p[1]; pos[2] = sv.p[2];
+    q[0] = sv.q[0]; q[1] = sv.q[1]; q[2] = sv.q[2]; q[3] = sv.q[3];
     *rejected = true;
   }
   bool any = false;
   if (P.n_pairs) {
+    double R[9], d[PST_MAX_PAIRS];
     quat_to_rot(q, R);
     pair_dists(P, R, pos[0], pos[1], pos[2], d);
     for (int p = 0; p < P.n_pairs; ++p) {
@@ -133,6 +132,40 @@ __device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* 
     }
   }
   return any;
+}
+
+// Scalar-diffusion move: Ermak-McCammon translation (:1085-1094) and the
+// right-multiplied rotation (:1095-1101).
+__device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* pos, double* q,
+                                const double* Fv, const double* Tv, const double* nt,
+                                const double* nr, double dt, double D, double* x0, double* x1,
+                                bool* rejected) {
+  MoveSave sv;
+  move_begin(P, pos, q, x0, sv);
+  const double sig = sqrt(2.0 * D * dt);
+  pos[0] += D * dt * Fv[0] + sig * nt[0];
+  pos[1] += D * dt * Fv[1] + sig * nt[1];
+  pos[2] += D * dt * Fv[2] + sig * nt[2];
+  const double sr = sqrt(2.0 * P.D_r * dt);
+  apply_rotation(q, P.D_r * dt * Tv[0] + sr * nr[0], P.D_r * dt * Tv[1] + sr * nr[1],
+                 P.D_r * dt * Tv[2] + sr * nr[2]);
+  return move_end(P, S, pos, q, sv, x0, x1, rejected);
+}
+
+// Two-body RPY move: both bodies translate and turn with the 12x12 mobility
+// and its Cholesky noise (xi: 12 standard normals), then the pose is
+// re-expressed in the turned receptor frame.
+__device__ inline bool move_one_rpy(const StepParams& P, const ForceSys& S, double* pos,
+                                    double* q, const double* FL, const double* TL,
+                                    const double* xi, double dt, double* x0, double* x1,
+                                    bool* rejected) {
+  MoveSave sv;
+  move_begin(P, pos, q, x0, sv);
+  double FR[3], TR[3], dX[12];
+  rpy_receptor_load(S.torque_mode, pos, FL, TL, FR, TR);
+  rpy_displacement(P.a_rec, P.a_lig, P.kT_over_eta, dt, pos, FR, FL, TR, TL, xi, dX);
+  rpy_apply(dX, pos, q);
+  return move_end(P, S, pos, q, sv, x0, x1, rejected);
 }
 
 // Brownian-bridge pair count (:1157-1184, :1236-1239).
@@ -542,15 +575,24 @@ __global__ void k_prod_post(StepParams P, ForceSys S, ProdBufs B, int par) {
   const double dt = adaptive_dt(P, r);
   const double D = diffusion(P, r);
   B.dt[s] = dt;
-  double nt[3], nr[3], extra;
-  philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 0, nt[0], nt[1]);
-  philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 1, nt[2], nr[0]);
-  philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 2, nr[1], nr[2]);
-  (void)extra;
-  bool rej;
-  const bool any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt,
-                            D, B.x0 + (size_t)s * PST_MAX_PAIRS, B.x1 + (size_t)s * PST_MAX_PAIRS,
-                            &rej);
+  bool rej, any;
+  double* x0 = B.x0 + (size_t)s * PST_MAX_PAIRS;
+  double* x1 = B.x1 + (size_t)s * PST_MAX_PAIRS;
+  if (P.hydro_mode == 1) {
+    double xi[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, (uint32_t)k, xi[2 * k], xi[2 * k + 1]);
+    any = move_one_rpy(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, xi, dt, x0, x1,
+                       &rej);
+  } else {
+    double nt[3], nr[3];
+    philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 0, nt[0], nt[1]);
+    philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 1, nt[2], nr[0]);
+    philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 2, nr[1], nr[2]);
+    any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt, D, x0, x1,
+                   &rej);
+  }
   if (rej) atomicAdd(&B.ctr->n_overlap, 1ull);
   const unsigned long long m = __ballot(any);
   if (m && __lane_id() == __ffsll((long long)m) - 1) atomicOr(&B.ctr->any_above[par], 1);

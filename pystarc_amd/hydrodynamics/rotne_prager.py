@@ -78,3 +78,23 @@ class MobilityTensor:
 
     def relative_rotational_diffusion(self):
         return self.D_rot1 + self.D_rot2
+
+
+def rpy12_mobility(pos, a_rec, a_lig, ctx=None):
+    """12x12 two-sphere grand mobility M and noise factor G (G G^T = M) per
+    ligand position, evaluated on the GPU (``pst_rpy12_eval``), 1/eta units,
+    order [x_rec, x_lig, theta_rec, theta_lig], receptor at the origin.
+    ``pos`` (n,3) host array or device tensor; returns two (n,12,12) float64
+    device tensors."""
+    import torch
+
+    from .._lib import Context, ptr
+    if ctx is None:
+        ctx = Context(0)
+    p = torch.as_tensor(pos, dtype=torch.float64, device=f"cuda:{ctx.device}").contiguous()
+    n = p.shape[0]
+    M = torch.empty((n, 12, 12), dtype=torch.float64, device=p.device)
+    G = torch.empty_like(M)
+    ctx.check(ctx.lib.pst_rpy12_eval(ctx.h, n, ptr(p), float(a_rec), float(a_lig), ptr(M), ptr(G)),
+              "pst_rpy12_eval")
+    return M, G

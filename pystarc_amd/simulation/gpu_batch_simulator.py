@@ -25,6 +25,16 @@ Keyword-only extensions (defaults keep the reference's physics):
   audit_n      surrogate trajectories re-evaluated on the grid path per step.
   audit_tol    relative RMS force error above which the run falls back to
                the grid path for every trajectory.
+  autotune     False (default) or the number of warm-up steps (True: 64)
+               after which the production force launch (lanes per trajectory,
+               workgroup cap, XCD mapping) is tuned on live steps.
+  hydro        "scalar" (default): the reference's scalar relative diffusion
+               (D_t, or D_parallel(r) with hydrodynamic_interactions).
+               "rpy12": both bodies translate and turn with the two-sphere
+               12x12 RPY grand mobility (radii mob.radius1 / mob.radius2,
+               kT/eta = 1/0.243 A^3/ps as the reference's HI constant) and its
+               Cholesky noise; the pose is then re-expressed in the turned
+               receptor frame (philox mode only).
 """
 from __future__ import annotations
 
@@ -128,6 +138,20 @@ class GPUBatchResult:
         return "\n".join(lines)
 
 
+def autotune_sim(lib, ctx, h, rounds=2, steps_per_cand=1):
+    """Runs pst_prod_autotune on a production sim handle; returns the table of
+    candidates {G, cap, xcd, ms} and the chosen launch."""
+    table = np.zeros((64, 4))
+    n = C.c_int()
+    best = (C.c_int * 4)()
+    ctx.check(lib.pst_prod_autotune(h, int(rounds), int(steps_per_cand), ptr(table), 64,
+                                    C.byref(n), best), "pst_prod_autotune")
+    rows = [{"G": int(r[0]), "cap": int(r[1]), "xcd": int(r[2]), "ms": float(r[3])}
+            for r in table[:n.value]]
+    return {"best": {"G": best[0], "cap": best[1], "xcd": best[2]}, "table": rows,
+            "steps": int(best[3])}
+
+
 def _shoemake_host(u):
     return np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]),
                      np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
@@ -138,9 +162,17 @@ def _shoemake_host(u):
 class GPUBatchSimulator:
     def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
                  n_slots=None, precision=None, graph_steps=16, record=False, surrogate=None,
-                 audit_n=64, audit_tol=0.05):
+                 audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False):
         if rng not in ("philox", "numpy"):
             raise ValueError("rng must be 'philox' or 'numpy'")
+        if hydro not in ("scalar", "rpy12"):
+            raise ValueError("hydro must be 'scalar' or 'rpy12'")
+        if hydro == "rpy12" and rng != "philox":
+            raise ValueError("replay follows the reference's scalar diffusion; rpy12 needs rng='philox'")
+        if hydro == "rpy12" and not (getattr(mob, "radius1", 0) > 0 and getattr(mob, "radius2", 0) > 0):
+            raise ValueError("rpy12 needs both hydrodynamic radii (mob.radius1, mob.radius2)")
+        self.hydro = hydro
+        self.autotune = int(autotune) if autotune is not True else 64
         self.mol1, self.mol2, self.mob = mol1, mol2, mob
         self.pathway_set, self.params, self.engine = pathway_set, params, batch_engine
         self.rng_mode = rng
@@ -278,9 +310,10 @@ class GPUBatchSimulator:
                 P.cutoff[k] = float(self._rxn_cutoffs[k])
                 P.bb_deff[k] = float(deff[k])
         P.max_steps = int(p.max_steps)
-        P.hydro_mode = 0
+        P.hydro_mode = 1 if self.hydro == "rpy12" else 0
         P.a_rec = float(getattr(self.mob, "radius1", 0.0))
         P.a_lig = float(getattr(self.mob, "radius2", 0.0))
+        P.kT_over_eta = 1.0 / MU_REF
         P.surrogate_r = -1.0
         P.stop_done = int(N * 0.995)
         return P
@@ -326,6 +359,13 @@ class GPUBatchSimulator:
             c16 = np.zeros(16)
             t0 = time.time()
             steps = 0
+            tuned = None
+            if self.autotune:
+                ctx.check(lib.pst_prod_run(h, self.autotune, self.graph_steps, ptr(c16)),
+                          "pst_prod_run")
+                steps += int(c16[8])
+                tuned = autotune_sim(lib, ctx, h)
+                steps += tuned["steps"]
             chunk = max(self.graph_steps, 1) * 64
             while steps < P.max_steps:
                 todo = int(min(chunk, P.max_steps - steps))
@@ -351,6 +391,8 @@ class GPUBatchSimulator:
             n_m = N - n_r - n_e
             sim_data = {"n_returns": int(c16[5]), "n_overlap_rejected": int(c16[6]),
                         "n_bridge": int(c16[7]), "bd_steps": steps}
+            if tuned is not None:
+                sim_data["autotune"] = tuned
             if self.surrogate is not None:
                 audit["rel_rms"] = math.sqrt(audit["err2"] / audit["ref2"]) if audit["ref2"] > 0 else 0.0
                 sim_data["surrogate_audit"] = audit
@@ -368,6 +410,15 @@ class GPUBatchSimulator:
         """Step-by-step replay handle (run() drives it to completion)."""
         self._return_prob = self._compute_return_prob()
         return ReplayDriver(self, self._step_params())
+
+    def production_driver(self, record=True) -> "ProductionDriver":
+        """Step-by-step handle on the production kernels (philox mode, no
+        surrogate): advance(k) runs k device steps, state() reads the slots."""
+        if self.rng_mode != "philox":
+            raise ValueError("production_driver needs rng='philox'")
+        self._return_prob = self._compute_return_prob()
+        return ProductionDriver(self, self._step_params(), record)
+
 
     def _run_replay(self, P) -> GPUBatchResult:
         drv = ReplayDriver(self, P)
@@ -482,3 +533,68 @@ class ReplayDriver:
         if self.h:
             self.lib.pst_sim_destroy(self.h)
             self.h = None
+
+
+class ProductionDriver:
+    """Production step in host-chosen increments.  With n_slots >= N slot s
+    holds trajectory s for the whole run (no refill), so the slot state is
+    the trajectory state."""
+
+    def __init__(self, sim: GPUBatchSimulator, P: StepParams, record: bool):
+        self.sim, self.P = sim, P
+        self.lib, self.ctx = sim.engine.ctx.lib, sim.engine.ctx
+        self.h = sim._make_sim(P)
+        self.record = bool(record)
+        self.ctx.check(self.lib.pst_prod_init(self.h, int(self.record)), "pst_prod_init")
+        self.c16 = np.zeros(16)
+        self.steps = 0
+
+    def advance(self, k: int, graph_steps: int = 0) -> int:
+        """Runs up to k steps; returns the number run (fewer once the run stops)."""
+        self.ctx.check(self.lib.pst_prod_run(self.h, int(k), int(graph_steps), ptr(self.c16)),
+                       "pst_prod_run")
+        n = int(self.c16[8])
+        self.steps += n
+        return n
+
+    def autotune(self, rounds=2, steps_per_cand=1) -> Dict:
+        """Tunes the force launch on live steps (they count as run steps)."""
+        out = autotune_sim(self.lib, self.ctx, self.h, rounds, steps_per_cand)
+        self.steps += out["steps"]
+        return out
+
+    def counters(self) -> Dict:
+        c = self.c16
+        return {"n_reacted": int(c[0]), "n_escaped": int(c[1]), "n_max_steps": int(c[2]),
+                "total_steps": int(c[3]), "n_returns": int(c[5]), "n_overlap": int(c[6]),
+                "n_bridge": int(c[7]), "stopped": bool(c[9])}
+
+    def state(self):
+        """(pos (S,3), quat (S,4), status (S,), n_steps (S,)) of the slots."""
+        S = int(self.P.n_slots)
+        pos = np.zeros((S, 3))
+        q = np.zeros((S, 4))
+        st = np.zeros(S, dtype=np.int32)
+        ns = np.zeros(S, dtype=np.int64)
+        self.ctx.check(self.lib.pst_sim_get_state(self.h, ptr(pos), ptr(q), ptr(st), ptr(ns)),
+                       "pst_sim_get_state")
+        return pos, q, st, ns
+
+    def outcomes(self):
+        """Per-trajectory (outcome, steps): 0 running, 1 reacted, 2 escaped, 3 max steps."""
+        N = int(self.P.n_total)
+        st = np.zeros(N, dtype=np.int8)
+        ns = np.zeros(N, dtype=np.int32)
+        self.ctx.check(self.lib.pst_prod_outcomes(self.h, ptr(st), ptr(ns)), "pst_prod_outcomes")
+        return st, ns
+
+    def close(self):
+        if self.h is not None:
+            self.lib.pst_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

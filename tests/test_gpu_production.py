@@ -65,3 +65,23 @@ def test_fp32_forces_keep_k_on():
     r32 = run(sysm, S.make_engine(sysm, g, precision="fp32"))
     lo, hi = wilson_ci(r64.n_reacted, r64.n_escaped, 0.999)
     assert lo - 0.01 <= r32.reaction_probability <= hi + 0.01
+
+
+def test_autotune_measures_every_candidate_and_keeps_outcomes():
+    """The tuner runs real steps under each launch candidate; the launch only
+    changes the order of FP64 force sums, so trajectories keep their fate."""
+    sysm = small_protein_ligand(n_traj=1500)
+    sysm.params.max_steps = 20000
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+    a = run(sysm, eng, record=True)
+    b = run(sysm, eng, record=True, autotune=16)
+    tune = b.sim_data["autotune"]
+    ms = [r["ms"] for r in tune["table"]]
+    assert len(ms) >= 10 and min(ms) > 0
+    best = tune["best"]
+    row = [r for r in tune["table"] if (r["G"], r["cap"], r["xcd"]) == (best["G"], best["cap"], best["xcd"])]
+    assert row and row[0]["ms"] == min(ms)
+    o1, o2 = a.sim_data["outcome"], b.sim_data["outcome"]
+    both = (o1 > 0) & (o2 > 0)
+    agree = (o1[both] == o2[both]) & (a.sim_data["n_steps"][both] == b.sim_data["n_steps"][both])
+    assert both.sum() > 1000 and agree.mean() > 0.99
