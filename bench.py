@@ -3,8 +3,10 @@ surrogate vs plain, with k_on of both paths from the same run.
 
 Workload: the 1,700 / 1,400-atom protein-protein pair (193^3 FP32 receptor
 potential + Born grids and ligand Born grid at 0.5 A, bidirectional Born,
-scalar HI, overlap rejection, 2-of-5 contact reaction < 5.5 A, b = 60 A,
-r_esc = 120 A).  A "step" is one BD step of every live trajectory slot;
+two-body RPY 12x12 mobility with hydrodynamic radii 17 / 15 A and FP64
+Cholesky noise (--hydro scalar: the reference's scalar D_parallel), overlap
+rejection, 2-of-5 contact reaction < 5.5 A, b = 60 A, r_esc = 120 A).  The
+last TUNE_STEPS warm-up steps run the force-launch autotuner.  A "step" is one BD step of every live trajectory slot;
 value = trajectory-steps per second over the timed steps of the surrogate +
 FP32 path (inputs resident in HBM).  Paths timed on identical seeds:
 
@@ -37,6 +39,8 @@ METRIC = "trajectory-steps/sec on 1 MI355X, surrogate vs plain at equal k_on (Wi
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_TRAJ = 2 * (9 * 256 + 3 * 256 * 256 + 256 * 13)   # 404,480
+TUNE_STEPS = 60   # 2 rounds x 30 launch candidates x 1 step
+FORCE_TIMING_STEPS = 20
 
 
 def log(*a):
@@ -55,17 +59,19 @@ def build_system(args, n_traj=None):
     return sysm, grids
 
 
-def make_sim(sysm, eng, args, surrogate=None, n_slots=None):
+def make_sim(sysm, eng, args, surrogate=None, n_slots=None, autotune=False):
     from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchSimulator
     return GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
                              n_slots=n_slots or args.slots, graph_steps=args.graph_steps,
-                             surrogate=surrogate, audit_n=args.audit_n, audit_tol=args.audit_tol)
+                             surrogate=surrogate, audit_n=args.audit_n, audit_tol=args.audit_tol,
+                             hydro=args.hydro, autotune=autotune)
 
 
 def run_window(sysm, eng, args, surrogate=None):
     """Production run: warm-up steps, then exactly `steps` timed steps."""
     import torch
     from pystarc_amd._lib import ptr
+    from pystarc_amd.simulation.gpu_batch_simulator import autotune_sim
     sim = make_sim(sysm, eng, args, surrogate)
     sim._return_prob = sim._compute_return_prob()
     P = sim._step_params()
@@ -76,8 +82,15 @@ def run_window(sysm, eng, args, surrogate=None):
     if surrogate is not None:
         p9 = np.ascontiguousarray(surrogate.sim_params(args.audit_n), dtype=np.float64)
         ctx.check(lib.pst_sim_set_surrogate(h, surrogate.h, ptr(p9)), "pst_sim_set_surrogate")
-    if args.warmup:
-        ctx.check(lib.pst_prod_run(h, args.warmup, args.graph_steps, ptr(c16)), "warmup")
+    # warm-up: plain steps, then (with --autotune) the launch tuner's steps,
+    # which are real steps under each candidate launch
+    tune = None
+    n_tune = TUNE_STEPS if args.autotune else 0
+    if args.warmup > n_tune:
+        ctx.check(lib.pst_prod_run(h, args.warmup - n_tune, args.graph_steps, ptr(c16)), "warmup")
+    if n_tune:
+        tune = autotune_sim(lib, ctx, h)
+        log(f"autotune: {tune['best']} after {tune['steps']} tuning steps")
     if surrogate is not None:
         ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
     base_steps = c16[3]
@@ -91,37 +104,23 @@ def run_window(sysm, eng, args, surrogate=None):
     S_ = sim.n_slots
     pos = np.zeros((S_, 3)); q = np.zeros((S_, 4)); st = np.zeros(S_, dtype=np.int32)
     ctx.check(lib.pst_sim_get_state(h, ptr(pos), ptr(q), ptr(st), None), "get_state")
+    # the force kernel as the step launches it: FORCE_TIMING_STEPS further steps,
+    # each force launch timed by HIP events on the library's stream
+    m6 = np.zeros(6)
+    ctx.check(lib.pst_prod_time_forces(h, FORCE_TIMING_STEPS, ptr(m6)), "pst_prod_time_forces")
     lib.pst_sim_destroy(h)
+    alg = m6[2] * (56 + 48) + (m6[3] + m6[4] + m6[5]) * 20 * 4
+    roof_force = {"ms": m6[0], "bytes": alg, "launches": int(m6[1]),
+                  "census": [m6[2], m6[3], m6[4], m6[5]],
+                  "achieved_gbs": alg / (m6[0] * 1e-3) / 1e9 if m6[0] > 0 else 0.0}
     live = st == 0
     traj_steps = c16[3] - base_steps
     audit = math.sqrt(c16[12] / c16[13]) if c16[13] > 0 else None
     return {"wall_s": wall, "device_ms_per_step": ms.value, "steps": int(c16[8]),
             "traj_steps": traj_steps, "value": traj_steps / wall if wall > 0 else 0.0,
             "pos": pos[live], "quat": q[live], "counters": c16.tolist(), "audit_rel_rms": audit,
-            "n_surrogate_last_step": int(c16[11])}
-
-
-def force_roofline(eng, pos, quat, precision, reps=20):
-    """Average duration of the fused force kernel on given poses (HIP events
-    on the context stream) and its algorithmic bytes from an atom census."""
-    import torch
-    from pystarc_amd._lib import ptr
-    n = len(pos)
-    dp = torch.as_tensor(np.ascontiguousarray(pos), device=eng.device)
-    dq = torch.as_tensor(np.ascontiguousarray(quat), device=eng.device)
-    F = torch.zeros((n, 3), dtype=torch.float64, device=eng.device)
-    T = torch.zeros_like(F)
-    ms = C.c_double()
-    census = np.zeros(4)
-    eng.ctx.check(eng.ctx.lib.pst_time_forces_pose(
-        eng.ctx.h, n, ptr(dp), ptr(dq), ptr(F), ptr(T), None, 0 if precision == "fp64" else 1,
-        reps, C.byref(ms), ptr(census)), "pst_time_forces_pose")
-    # per trajectory: pose in (24 + 32 B), force + torque out (48 B); per
-    # in-grid atom and grid: 20 distinct FP32 stencil nodes
-    alg = census[0] * (56 + 48) + (census[1] + census[2] + census[3]) * 20 * 4
-    sec = ms.value * 1e-3
-    return {"ms": ms.value, "bytes": alg, "census": census.tolist(),
-            "achieved_gbs": alg / sec / 1e9 if sec > 0 else 0.0, "n": n}
+            "n_surrogate_last_step": int(c16[11]),
+            "launch": tune["best"] if tune else None, "roof_force": roof_force}
 
 
 def mlp_roofline(surr, pos, quat, reps=20):
@@ -144,7 +143,8 @@ def mlp_roofline(surr, pos, quat, reps=20):
 def full_run(sysm, eng, args, surrogate=None):
     """Complete trajectories for k_on (99.5 % rule), returns P and Wilson CI."""
     sysm.params.n_trajectories = args.kon_trajectories
-    res = make_sim(sysm, eng, args, surrogate, n_slots=min(args.kon_trajectories, args.slots)).run()
+    res = make_sim(sysm, eng, args, surrogate, n_slots=min(args.kon_trajectories, args.slots),
+                   autotune=64 if args.autotune else False).run()
     lo, hi = res.reaction_probability_ci(0.95)
     out = {"n": res.n_trajectories, "reacted": res.n_reacted, "escaped": res.n_escaped,
            "P": res.reaction_probability, "P_ci95": [lo, hi],
@@ -158,29 +158,45 @@ def full_run(sysm, eng, args, surrogate=None):
     return out
 
 
-def cpu_baseline(sysm, grids, pos, quat, seconds=15.0, n_traj=128):
-    """The NumPy FP64 oracle of the same step on a bounded sample: n_traj live
-    poses taken from the GPU run, stepped for about `seconds` of wall time."""
-    from oracle.simulator import OracleSim
+def cpu_baseline(sysm, grids, pos, quat, hydro, seconds=15.0, n_traj=128):
+    """The NumPy FP64 oracle of the same production step (same hydrodynamics)
+    on a bounded sample: n_traj live poses taken from the GPU run, stepped for
+    about `seconds` of wall time."""
+    from oracle.production import ProductionOracle
     from tests._systems import oracle_params, oracle_system
     np_grids = {k: [(np.asarray(g.data), np.asarray(g.origin), float(g.delta[0, 0])) for g in v]
                 for k, v in grids.items()}
     sel = np.random.default_rng(0).choice(len(pos), size=min(n_traj, len(pos)), replace=False)
-    ora = OracleSim(oracle_system(sysm, np_grids), oracle_params(sysm, n_trajectories=len(sel)))
+    ora = ProductionOracle(oracle_system(sysm, np_grids, chunk=0),
+                           oracle_params(sysm, n_trajectories=len(sel)), hydro=hydro)
     ora.pos[:] = pos[sel]
     ora.q[:] = quat[sel]
     w0 = time.perf_counter()
-    steps = 0
+    steps = loops = 0
     while time.perf_counter() - w0 < seconds:
         before = ora.total_steps
         if not ora.step():
             break
+        loops += 1
         steps += ora.total_steps - before
     wall = time.perf_counter() - w0
     return {"value": steps / wall if wall > 0 else 0.0, "unit": "trajectory-steps/s", "cores": 1,
             "kind": "port",
-            "sample": f"{len(sel)} live poses from the GPU run stepped {ora.step_index} times "
-                      f"({steps} trajectory-steps, {wall:.1f}s, single-threaded NumPy FP64 oracle)"}
+            "sample": f"{len(sel)} live poses from the GPU run stepped {loops} times "
+                      f"({steps} trajectory-steps, {wall:.1f}s, single-threaded NumPy FP64 oracle, "
+                      f"hydro {hydro})"}
+
+
+def measured_traffic(path):
+    """HBM bytes per force launch of `path` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated), or None."""
+    f = os.path.join(ROOT, "profiles", "k_forces_traffic.json")
+    try:
+        with open(f) as fh:
+            d = json.load(fh)
+        return d["paths"][path]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def train_surrogate(sysm, eng64, args, extra_poses=None):
@@ -194,16 +210,22 @@ def train_surrogate(sysm, eng64, args, extra_poses=None):
 
 
 def pmc_mode(args):
-    """For rocprofv3 --pmc runs: a production window of the first path, three
-    fused force launches on the live poses, then one plain device copy of
-    256 MiB whose FETCH_SIZE / WRITE_SIZE calibrate the counters' byte scale."""
+    """For rocprofv3 --pmc runs: the production window of one path (its
+    force-timing steps sit between two k_timing_window_mark dispatches), then
+    one plain device copy of 256 MiB whose FETCH_SIZE / WRITE_SIZE calibrate
+    the counters' byte scale.  The surrogate path trains on the plain path's
+    FP64 forces first (--train-poses / --train-epochs)."""
     import torch
     from pystarc_amd import systems as S
     sysm, grids = build_system(args)
-    prec = args.paths.split(",")[0]
-    eng = S.make_engine(sysm, grids, precision=prec)
-    r = run_window(sysm, eng, args)
-    force_roofline(eng, r["pos"], r["quat"], prec, reps=args.pmc_launches)
+    p = args.paths.split(",")[0]
+    eng = S.make_engine(sysm, grids, precision="fp64" if p == "fp64" else "fp32")
+    surr = None
+    if p == "surrogate":
+        surr = train_surrogate(sysm, S.make_engine(sysm, grids, precision="fp64"), args)
+    r = run_window(sysm, eng, args, surr)
+    log(f"pmc mode: path {p}, force kernel in the step {r['roof_force']['ms']:.3f} ms, "
+        f"census {r['roof_force']['census']}, bytes {r['roof_force']['bytes']:.4g}")
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -229,7 +251,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kon", action="store_true")
     ap.add_argument("--pmc", action="store_true", help="counter-collection mode (no JSON line)")
-    ap.add_argument("--pmc-launches", type=int, default=3)
+    ap.add_argument("--hydro", default="rpy12", choices=("rpy12", "scalar"),
+                    help="two-body RPY 12x12 mobility (config default) or scalar D_parallel")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the heuristic force launch instead of tuning it in the warm-up")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU in this build (asked for {args.gpus}); "
@@ -264,18 +289,14 @@ def main():
         r = run_window(sysm, eng, args, surr if p == "surrogate" else None)
         log(f"path {p}: {r['value']:.4g} traj-steps/s, {r['device_ms_per_step']:.3f} ms/step device"
             + (f", audit rel rms {r['audit_rel_rms']:.3f}" if r["audit_rel_rms"] is not None else ""))
-        prec = "fp64" if p == "fp64" else "fp32"
+        log(f"path {p}: force kernel in the step {r['roof_force']['ms']:.3f} ms over "
+            f"{r['roof_force']['census'][0]:.0f} trajectories, "
+            f"{r['roof_force']['achieved_gbs']:.1f} GB/s algorithmic")
         if p == "surrogate":
             rr = np.linalg.norm(r["pos"], axis=1)
-            near = rr <= surr.r_on
-            r["roof_force"] = force_roofline(eng, r["pos"][near], r["quat"][near], prec) if near.any() else None
-            r["roof_mlp"] = mlp_roofline(surr, r["pos"][~near], r["quat"][~near])
+            r["roof_mlp"] = mlp_roofline(surr, r["pos"][rr > surr.r_on], r["quat"][rr > surr.r_on])
             log(f"path {p}: mlp {r['roof_mlp']['ms']:.3f} ms for {r['roof_mlp']['n']} poses "
                 f"({r['roof_mlp']['tflops']:.1f} TFLOP/s)")
-        else:
-            r["roof_force"] = force_roofline(eng, r["pos"], r["quat"], prec)
-            log(f"path {p}: force kernel {r['roof_force']['ms']:.3f} ms, "
-                f"{r['roof_force']['achieved_gbs']:.1f} GB/s algorithmic")
         results[p] = r
     kon = {}
     if not args.no_kon:
@@ -292,7 +313,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(sysm, grids, best["pos"], best["quat"])
+        cpu = cpu_baseline(sysm, grids, best["pos"], best["quat"], args.hydro)
         log(f"cpu baseline {cpu['value']:.4g} traj-steps/s")
     # dominant kernel of the headline path: MFMA roofline of the MLP when it
     # takes longer than the force kernel on the near trajectories, else HBM
@@ -305,9 +326,14 @@ def main():
                 "rows": rm["n"]}
     else:
         rf = best["roof_force"]
+        lc = best.get("launch")
+        tc = "double" if head == "fp64" else "float"
+        kname = f"k_forces<{tc}, {lc['G']}, true>" if lc else "k_forces"
         roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_forces", "ms_per_launch": rf["ms"], "census": rf["census"]}
+                "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS,
+                "traffic": measured_traffic(head),
+                "kernel": kname, "ms_per_launch": rf["ms"], "launches": rf["launches"],
+                "census": rf["census"], "algorithmic_bytes": rf["bytes"]}
     plain = results.get("fp64")
     out = {
         "metric": METRIC,
@@ -324,9 +350,12 @@ def main():
                   "fp32": "f64 state, f32 forces", "fp64": "f64"}[head],
         "data": "synthetic",
         "config": {"workload": "protein-protein 1700/1400 atoms, 193^3 FP32 grids (receptor "
-                               "potential+Born, ligand Born), bidirectional Born, scalar HI, "
-                               "b=60 A, 2-of-5 contacts < 5.5 A",
-                   "path": head, "slots": args.slots, "graph_steps": args.graph_steps},
+                               "potential+Born, ligand Born), bidirectional Born, "
+                               + ("two-body RPY 12x12 mobility (radii 17/15 A) with FP64 "
+                                  "Cholesky noise" if args.hydro == "rpy12" else "scalar HI")
+                               + ", b=60 A, 2-of-5 contacts < 5.5 A",
+                   "path": head, "slots": args.slots, "graph_steps": args.graph_steps,
+                   "hydro": args.hydro, "launch": best.get("launch")},
         "speedup_vs_plain_fp64": (best["value"] / plain["value"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -338,7 +367,7 @@ def main():
                       "audit_rel_rms": v["audit_rel_rms"],
                       "force_ms": v["roof_force"]["ms"] if v.get("roof_force") else None,
                       "force_gbs": v["roof_force"]["achieved_gbs"] if v.get("roof_force") else None,
-                      "mlp": v.get("roof_mlp")}
+                      "mlp": v.get("roof_mlp"), "launch": v.get("launch")}
                   for k, v in results.items()},
     }
     print(json.dumps(out), flush=True)

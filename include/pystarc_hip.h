@@ -170,6 +170,14 @@ int pst_prod_autotune(pst_sim* sim, int rounds, int steps_per_cand, double* tabl
                       int* n_rows, int* best4);
 /* The same launch configuration set by hand (G = 0: heuristic). */
 int pst_prod_set_launch(pst_sim* sim, int G, int blocks_cap, int xcd);
+/* Times the force kernel inside the production step (no reference
+ * counterpart): `steps` real steps with plain launches, each force launch
+ * bracketed by HIP events on the context stream and preceded by a gather
+ * census of its trajectory list; empty k_timing_window_mark dispatches bracket
+ * the window in a kernel trace.  out6 = {mean ms per force launch, steps run,
+ * and per launch the mean trajectories, receptor-potential-grid atoms,
+ * receptor-Born-grid atoms, ligand-Born-grid (reverse) atoms}. */
+int pst_prod_time_forces(pst_sim* sim, int steps, double* out6);
 
 /* ---- learned surrogate ---------------------------------------------------
  * No reference counterpart: a 9 -> 4x256 (SiLU) -> 13 MLP standing in for

@@ -85,6 +85,7 @@ _SIGS = {
     "pst_prod_outcomes": ([_vp, _vp, _vp], C.c_int),
     "pst_prod_autotune": ([_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp], C.c_int),
     "pst_prod_set_launch": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "pst_prod_time_forces": ([_vp, C.c_int, _vp], C.c_int),
     "pst_prod_last_ms_per_step": ([_vp, _dp], C.c_int),
     "pst_make_grid": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double,
                        C.c_double, C.c_double, C.c_int, _vp], C.c_int),

@@ -100,6 +100,8 @@ struct pst_sim {
   int tune_cap = 2048;     // cap on resident workgroups (grid-stride beyond)
   int tune_xcd = 1;        // XCD-chunked mapping of the sorted grid-path list
   hipEvent_t tev0 = nullptr, tev1 = nullptr;  // brackets the force launch while tuning
+  unsigned long long* census = nullptr;       // gather census of timed force launches
+  bool census_on = false;
 };
 
 #define CK(expr)                                                                      \
@@ -629,7 +631,7 @@ int pst_sim_destroy(pst_sim* s) {
   void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
                   s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
-                  s->audit_F, s->key, s->bins};
+                  s->audit_F, s->key, s->bins, s->census};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete s;
@@ -901,6 +903,7 @@ static int enqueue_step(pst_sim* s, int par) {
   ForceSys S;
   fill_sys(ctx, S);
   const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
+  if (s->census_on) CK(launch_census(S, W, s->census, ctx->stream));
   if (s->tev0) CK(hipEventRecord(s->tev0, ctx->stream));
   CK(launch_forces(S, W, s->precision, G, s->S, s->tune_cap, ctx->stream));
   if (s->tev1) CK(hipEventRecord(s->tev1, ctx->stream));
@@ -1077,6 +1080,56 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
     best4[0] = s->tune_G > 0 ? s->tune_G : g0; best4[1] = s->tune_cap; best4[2] = s->tune_xcd;
     best4[3] = (int)k;
   }
+  return 0;
+}
+
+// Times the production force kernel in its own step: `steps` production steps
+// (plain launches, current launch configuration), each force launch bracketed
+// by HIP events on the context stream and preceded by a gather census of its
+// list; marker kernels bracket the window in a kernel trace.
+// out6 = {mean ms per force launch, steps run, mean trajectories, mean elec-grid
+// atoms, mean born-grid atoms, mean reverse-grid atoms} per launch.
+int pst_prod_time_forces(pst_sim* s, int steps, double* out6) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_time_forces: call pst_prod_init first");
+  if (steps <= 0 || !out6) FAIL("pst_prod_time_forces: bad arguments");
+  CK(set_dev(ctx));
+  if (!s->census) CK(hipMalloc(&s->census, 4 * sizeof(unsigned long long)));
+  CK(hipMemsetAsync(s->census, 0, 4 * sizeof(unsigned long long), ctx->stream));
+  CK(hipEventCreate(&s->tev0));
+  CK(hipEventCreate(&s->tev1));
+  s->census_on = true;
+  CK(launch_window_mark(ctx->stream));
+  double total = 0;
+  int done = 0, rc = 0;
+  RunCounters h;
+  memset(&h, 0, sizeof(h));
+  for (int k = 0; k < steps; ++k) {
+    rc = enqueue_step(s, k & 1);
+    if (rc) break;
+    hipError_t e = hipEventSynchronize(s->tev1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->tev0, s->tev1);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = 2; break; }
+    total += ms;
+    ++done;
+    if (h.stop || (h.n_sr + h.n_mlp) == 0) break;
+  }
+  s->census_on = false;
+  hipEventDestroy(s->tev0);
+  hipEventDestroy(s->tev1);
+  s->tev0 = s->tev1 = nullptr;
+  if (rc) return rc;
+  CK(launch_window_mark(ctx->stream));
+  unsigned long long c[4];
+  CK(hipMemcpyAsync(c, s->census, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  const double d = done > 0 ? (double)done : 1.0;
+  out6[0] = total / d; out6[1] = done;
+  out6[2] = c[0] / d; out6[3] = c[1] / d; out6[4] = c[2] / d; out6[5] = c[3] / d;
   return 0;
 }
 

@@ -10,6 +10,12 @@ hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, i
 hipError_t launch_chunk_min_r(const double* pos, const int* list, int n, int fb, int cs, int cpf,
                               double* out, int n_chunks, hipStream_t st);
 int pick_group(int M, int n_rec_active);
+// Gathers of one force launch over W's list (pose source): out4 += {trajectories,
+// elec-grid atoms, born-grid atoms, reverse-grid atoms}.
+hipError_t launch_census(const ForceSys& S, const ForceWork& W, unsigned long long* out4,
+                         hipStream_t st);
+// Empty marker kernel bracketing a timing window in a kernel trace.
+hipError_t launch_window_mark(hipStream_t st);
 
 // Ordered (stable) compaction of in_list (or 0..n-1) by flag[slot].
 hipError_t launch_compact_ordered(const int* in_list, int n_in, const int* n_in_dev,
