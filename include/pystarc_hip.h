@@ -70,6 +70,17 @@ int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charge
 int pst_set_torque_mode(pst_ctx* ctx, int mode);
 /* Lanes per trajectory for the force kernel (1..64, power of two; 0 = automatic). */
 int pst_set_force_group(pst_ctx* ctx, int lanes);
+/* Force evaluation of the pose entry points and of new simulators: 0 = the
+ * direct kernel (one trajectory per lane group, gathers from the grids), 1 =
+ * brick-binned (gather requests sorted by 8^3-cell brick, each brick staged
+ * once in LDS; same results bit for bit; needs the float2 receptor pair grid,
+ * else the direct kernel runs).  Mode 1 sizes its buffers for max_traj
+ * trajectories (~32 B x (ligand + receptor atoms) each).  No reference
+ * counterpart (the reference has one kernel, gpu_batch_engine.py:65-123). */
+int pst_set_gather_mode(pst_ctx* ctx, int mode, int max_traj);
+/* What the context's last force evaluation launched: mode 0 direct, 1
+ * binned (-1: none yet), and its lanes per trajectory. */
+int pst_last_force_eval(const pst_ctx* ctx, int* mode, int* G);
 
 /* ---- force operator ------------------------------------------------------
  * Replaces GPUBatchForceEngine.__call__(lig_positions, lig_charges,
@@ -158,18 +169,22 @@ int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
 int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);
 /* Autotuner of the production force launch (no reference counterpart: the
  * reference launches batch_force_kernel with a fixed (16,16) block,
- * gpu_batch_engine.py:328-332).  Candidates: lanes per trajectory
- * G in {g0/4, g0/2, g0} (g0 = smallest power of two >= max(ligand atoms,
- * reverse-Born receptor atoms), at most 64), resident-workgroup cap in
- * {512 ... 8192} and the XCD-chunked list mapping on/off.  Each candidate runs
- * `steps_per_cand` real production steps per round, `rounds` round-robin
- * passes, force kernel timed by HIP events; the fastest becomes the launch of
- * later steps.  table (max_rows,4) = {G, cap, xcd, mean ms} per candidate;
- * best4 = {G, cap, xcd, steps run}.  Both may be NULL. */
+ * gpu_batch_engine.py:328-332).  Candidates: the direct kernel with lanes per
+ * trajectory G in {g0/4, g0/2, g0} (g0 = smallest power of two >= max(ligand
+ * atoms, reverse-Born receptor atoms), at most 64), resident-workgroup cap in
+ * {1024 ... 8192} and the XCD-chunked list mapping on/off; and, when the
+ * receptor grids form a float2 pair, the brick-binned evaluation with G >= 8
+ * and caps {1024, 4096}.  Each candidate runs `steps_per_cand` real
+ * production steps per round, `rounds` round-robin passes, force evaluation
+ * timed by HIP events; the fastest becomes the launch of later steps.
+ * table (max_rows,5) = {mode, G, cap, xcd, mean ms} per candidate
+ * (mode 0 direct, 1 binned); best5 = {mode, G, cap, xcd, steps run}.
+ * Both may be NULL. */
 int pst_prod_autotune(pst_sim* sim, int rounds, int steps_per_cand, double* table, int max_rows,
-                      int* n_rows, int* best4);
-/* The same launch configuration set by hand (G = 0: heuristic). */
-int pst_prod_set_launch(pst_sim* sim, int G, int blocks_cap, int xcd);
+                      int* n_rows, int* best5);
+/* The same launch configuration set by hand (mode -1: the context's gather
+ * mode, 0 direct, 1 binned; G = 0: heuristic). */
+int pst_prod_set_launch(pst_sim* sim, int mode, int G, int blocks_cap, int xcd);
 /* Times the force kernel inside the production step (no reference
  * counterpart): `steps` real steps with plain launches, each force launch
  * bracketed by HIP events on the context stream and preceded by a gather

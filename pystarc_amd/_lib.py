@@ -84,7 +84,9 @@ _SIGS = {
     "pst_prod_run": ([_vp, C.c_longlong, C.c_int, _vp], C.c_int),
     "pst_prod_outcomes": ([_vp, _vp, _vp], C.c_int),
     "pst_prod_autotune": ([_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp], C.c_int),
-    "pst_prod_set_launch": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "pst_prod_set_launch": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "pst_set_gather_mode": ([_vp, C.c_int, C.c_int], C.c_int),
+    "pst_last_force_eval": ([_vp, _ip, _ip], C.c_int),
     "pst_prod_time_forces": ([_vp, C.c_int, _vp], C.c_int),
     "pst_prod_last_ms_per_step": ([_vp, _dp], C.c_int),
     "pst_make_grid": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double,

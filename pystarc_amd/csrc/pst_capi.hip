@@ -49,6 +49,11 @@ struct pst_ctx {
   // scratch for the reference chunk skip
   double* d_chunk = nullptr;
   int chunk_cap = 0;
+  // brick-binned force evaluation (0 = direct kernel, 1 = binned when eligible)
+  int gather_mode = 0;
+  int last_mode = -1, last_G = 0;  // what the last force evaluation launched
+  BinBufs bins{};
+  std::vector<void*> bin_allocs;
 };
 
 struct pst_mlp {
@@ -99,6 +104,7 @@ struct pst_sim {
   int tune_G = 0;          // lanes per trajectory; 0 = pick_group heuristic
   int tune_cap = 2048;     // cap on resident workgroups (grid-stride beyond)
   int tune_xcd = 1;        // XCD-chunked mapping of the sorted grid-path list
+  int tune_mode = -1;      // 0 direct, 1 brick-binned, -1 the context's gather mode
   hipEvent_t tev0 = nullptr, tev1 = nullptr;  // brackets the force launch while tuning
   unsigned long long* census = nullptr;       // gather census of timed force launches
   bool census_on = false;
@@ -241,6 +247,7 @@ int pst_ctx_destroy(pst_ctx* ctx) {
   if (ctx->d_rec) hipFree(ctx->d_rec);
   if (ctx->d_rec_q) hipFree(ctx->d_rec_q);
   if (ctx->d_chunk) hipFree(ctx->d_chunk);
+  for (void* p : ctx->bin_allocs) hipFree(p);
   if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -422,6 +429,97 @@ static void chunk_layout(const pst_ctx* ctx, int n, int fb, int* cs, int* cpf, i
   *n_chunks = nfb * (*cpf);
 }
 
+// Bin buffers for up to max_traj trajectories of the current system (grown,
+// never shrunk; not called while a graph is being captured).  Left empty when
+// the system has no float2 pair grid or the request ids would not fit int32.
+static void free_bins(pst_ctx* ctx) {
+  for (void* p : ctx->bin_allocs) hipFree(p);
+  ctx->bin_allocs.clear();
+  memset(&ctx->bins, 0, sizeof(BinBufs));
+}
+
+static int ensure_bins(pst_ctx* ctx, int max_traj) {
+  if (!ctx->pair.mem || max_traj <= 0 || ctx->M <= 0) return 0;
+  const GridDev& pg = ctx->pair.g;
+  const bool rev = !ctx->lig_born.empty() && ctx->n_rec > 0;
+  int rb[3] = {bricks_per_axis(pg.nx), bricks_per_axis(pg.ny), bricks_per_axis(pg.nz)};
+  int lb[3] = {0, 0, 0};
+  if (rev) {
+    const GridDev& lg = ctx->lig_born[0].g;
+    lb[0] = bricks_per_axis(lg.nx); lb[1] = bricks_per_axis(lg.ny); lb[2] = bricks_per_axis(lg.nz);
+  }
+  BinBufs& B = ctx->bins;
+  const int need_rec = rev ? ctx->n_rec : 0;
+  if (B.hist && B.max_traj >= max_traj && B.max_m >= ctx->M && B.max_rec >= need_rec &&
+      !memcmp(B.rb, rb, sizeof(rb)) && !memcmp(B.lb, lb, sizeof(lb)))
+    return 0;
+  const int mt = B.hist && B.max_traj > max_traj ? B.max_traj : max_traj;
+  const long max_req = (long)mt * ((long)ctx->M + need_rec);
+  free_bins(ctx);
+  if (max_req >= 2147483647L) return 0;
+  const int nb_rec = rb[0] * rb[1] * rb[2], nb_lig = lb[0] * lb[1] * lb[2], nb = nb_rec + nb_lig;
+  const int max_items = bin_items_bound(max_req, nb);
+  auto get = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return nullptr;
+    ctx->bin_allocs.push_back(p);
+    return p;
+  };
+  B.hist = (int*)get((size_t)nb * 4);
+  B.start = (int*)get((size_t)(nb + 1) * 4);
+  B.cursor = (int*)get((size_t)nb * 4);
+  B.items = (int2*)get((size_t)max_items * 8);
+  B.n_items = (int*)get(4);
+  B.sorted = (int*)get((size_t)max_req * 4);
+  B.out = get((size_t)mt * ctx->M * 32);
+  B.out_rev = get((size_t)mt * need_rec * 32);
+  if (!B.hist || !B.start || !B.cursor || !B.items || !B.n_items || !B.sorted || !B.out ||
+      !B.out_rev) {
+    free_bins(ctx);
+    ctx->err = "ensure_bins: out of device memory";
+    return 2;
+  }
+  CK(hipMemsetAsync(B.hist, 0, (size_t)nb * 4, ctx->stream));
+  CK(hipMemsetAsync(B.n_items, 0, 4, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  B.nb_rec = nb_rec; B.nb_lig = nb_lig;
+  memcpy(B.rb, rb, sizeof(rb));
+  memcpy(B.lb, lb, sizeof(lb));
+  B.max_traj = mt; B.max_m = ctx->M; B.max_rec = need_rec;
+  B.rev_base = mt * ctx->M;
+  B.max_items = max_items;
+  B.gather_blocks = 2048;
+  return 0;
+}
+
+// The binned buffers must describe the current grids.
+static bool bins_match(pst_ctx* ctx, const ForceSys& S) {
+  const BinBufs& B = ctx->bins;
+  if (!B.hist || !S.paired) return false;
+  if (B.rb[0] != bricks_per_axis(S.pair.nx) || B.rb[1] != bricks_per_axis(S.pair.ny) ||
+      B.rb[2] != bricks_per_axis(S.pair.nz))
+    return false;
+  if (S.has_rev) {
+    if (B.lb[0] != bricks_per_axis(S.lig_born.nx) || B.lb[1] != bricks_per_axis(S.lig_born.ny) ||
+        B.lb[2] != bricks_per_axis(S.lig_born.nz))
+      return false;
+  }
+  return true;
+}
+
+// One force evaluation: the brick-binned pipeline when asked for and
+// eligible (same results as the direct kernel with the same G), else k_forces.
+static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const ForceWork& W,
+                                    int precision, int G, int max_n, int cap, int mode) {
+  ctx->last_G = G;
+  if (mode == 1 && G >= 8 && bins_match(ctx, S) && binned_eligible(S, W, ctx->bins, max_n)) {
+    ctx->last_mode = 1;
+    return launch_forces_binned(S, W, ctx->bins, precision, G, max_n, cap, ctx->stream);
+  }
+  ctx->last_mode = 0;
+  return launch_forces(S, W, precision, G, max_n, cap, ctx->stream);
+}
+
 static int forces_common(pst_ctx* ctx, ForceWork& W, int n, int precision, int fb) {
   ForceSys S;
   fill_sys(ctx, S);
@@ -435,7 +533,24 @@ static int forces_common(pst_ctx* ctx, ForceWork& W, int n, int precision, int f
     W.chunk_cs = cs;
     W.chunks_per_fb = cpf;
   }
-  CK(launch_forces(S, W, precision, group_for(ctx, S), n, 2048, ctx->stream));
+  CK(launch_force_eval(ctx, S, W, precision, group_for(ctx, S), n, 2048, ctx->gather_mode));
+  return 0;
+}
+
+int pst_last_force_eval(const pst_ctx* ctx, int* mode, int* G) {
+  if (!ctx) return 1;
+  if (mode) *mode = ctx->last_mode;
+  if (G) *G = ctx->last_G;
+  return 0;
+}
+
+int pst_set_gather_mode(pst_ctx* ctx, int mode, int max_traj) {
+  if (!ctx) return 1;
+  if (mode != 0 && mode != 1) FAIL("pst_set_gather_mode: mode must be 0 or 1");
+  if (mode == 1 && max_traj <= 0) FAIL("pst_set_gather_mode: max_traj must be positive");
+  CK(set_dev(ctx));
+  ctx->gather_mode = mode;
+  if (mode == 1) return ensure_bins(ctx, max_traj);
   return 0;
 }
 
@@ -856,6 +971,12 @@ static ProdBufs prod_bufs(pst_sim* s) {
   return B;
 }
 
+// Bin buffers for the sim's slots when its force evaluation is binned.
+static int sim_ensure_bins(pst_sim* s) {
+  const int mode = s->tune_mode >= 0 ? s->tune_mode : s->ctx->gather_mode;
+  return mode == 1 ? ensure_bins(s->ctx, s->S) : 0;
+}
+
 int pst_prod_init(pst_sim* s, int record) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
@@ -883,7 +1004,7 @@ int pst_prod_init(pst_sim* s, int record) {
   CK(prod_init(s->P, prod_bufs(s), ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
   s->prod_ready = true;
-  return 0;
+  return sim_ensure_bins(s);
 }
 
 static int enqueue_step(pst_sim* s, int par) {
@@ -905,7 +1026,8 @@ static int enqueue_step(pst_sim* s, int par) {
   const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
   if (s->census_on) CK(launch_census(S, W, s->census, ctx->stream));
   if (s->tev0) CK(hipEventRecord(s->tev0, ctx->stream));
-  CK(launch_forces(S, W, s->precision, G, s->S, s->tune_cap, ctx->stream));
+  const int mode = s->tune_mode >= 0 ? s->tune_mode : ctx->gather_mode;
+  CK(launch_force_eval(ctx, S, W, s->precision, G, s->S, s->tune_cap, mode));
   if (s->tev1) CK(hipEventRecord(s->tev1, ctx->stream));
   if (s->mlp && s->P.surrogate_r > 0) {
     CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
@@ -936,6 +1058,7 @@ int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) 
   if (!s->prod_ready) FAIL("pst_prod_run: call pst_prod_init first");
   if (max_steps < 0 || graph_steps < 0) FAIL("pst_prod_run: bad arguments");
   CK(set_dev(ctx));
+  if (sim_ensure_bins(s)) return 2;
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
@@ -1005,7 +1128,7 @@ int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) 
 // slowly drifting population is seen by all), its force kernel bracketed by
 // HIP events; the fastest total becomes the configuration of later steps.
 int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table, int max_rows,
-                      int* n_rows, int* best4) {
+                      int* n_rows, int* best5) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
   if (!s->prod_ready) FAIL("pst_prod_autotune: call pst_prod_init first");
@@ -1018,12 +1141,23 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
   for (int g = g0 / 4; g <= g0; g *= 2)
     if (g >= 1) Gs.push_back(g);
   if (Gs.empty()) Gs.push_back(g0);
-  const int caps[] = {512, 1024, 2048, 4096, 8192};
-  struct Cand { int G, cap, xcd; double ms; int n; };
+  // brick-binned candidates when the system has the float2 pair grid
+  bool binned = false;
+  if (S.paired) {
+    if (ensure_bins(ctx, s->S)) return 2;
+    fill_sys(ctx, S);
+    binned = bins_match(ctx, S);
+  }
+  struct Cand { int mode, G, cap, xcd; double ms; int n; };
   std::vector<Cand> cands;
+  const int caps[] = {1024, 2048, 4096, 8192};
   for (int g : Gs)
     for (int c : caps)
-      for (int x = 0; x < 2; ++x) cands.push_back({g, c, x, 0.0, 0});
+      for (int x = 0; x < 2; ++x) cands.push_back({0, g, c, x, 0.0, 0});
+  if (binned)
+    for (int g : Gs)
+      if (g >= 8)
+        for (int c : {1024, 4096}) cands.push_back({1, g, c, 1, 0.0, 0});
   CK(hipEventCreate(&s->tev0));
   CK(hipEventCreate(&s->tev1));
   RunCounters h;
@@ -1034,7 +1168,7 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
   for (int r = 0; r < rounds && !stop && !rc; ++r) {
     for (auto& c : cands) {
       if (stop || rc) break;
-      s->tune_G = c.G; s->tune_cap = c.cap; s->tune_xcd = c.xcd;
+      s->tune_mode = c.mode; s->tune_G = c.G; s->tune_cap = c.cap; s->tune_xcd = c.xcd;
       for (int i = 0; i < steps_per_cand && !stop; ++i) {
         rc = enqueue_step(s, (int)(k++ & 1));
         if (rc) break;
@@ -1061,24 +1195,25 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
   for (auto& c : cands)
     if (c.n == nmax && nmax > 0 && (!best || c.ms < best->ms)) best = &c;
   if (best) {
-    s->tune_G = best->G; s->tune_cap = best->cap; s->tune_xcd = best->xcd;
+    s->tune_mode = best->mode; s->tune_G = best->G; s->tune_cap = best->cap; s->tune_xcd = best->xcd;
   } else {
-    s->tune_G = 0; s->tune_cap = 2048; s->tune_xcd = 1;
+    s->tune_mode = -1; s->tune_G = 0; s->tune_cap = 2048; s->tune_xcd = 1;
   }
   if (rc) return rc;
   if (table && n_rows) {
     int m = 0;
     for (auto& c : cands) {
       if (m >= max_rows) break;
-      table[4 * m] = c.G; table[4 * m + 1] = c.cap; table[4 * m + 2] = c.xcd;
-      table[4 * m + 3] = c.n ? c.ms / c.n : -1.0;
+      table[5 * m] = c.mode; table[5 * m + 1] = c.G; table[5 * m + 2] = c.cap;
+      table[5 * m + 3] = c.xcd; table[5 * m + 4] = c.n ? c.ms / c.n : -1.0;
       ++m;
     }
     *n_rows = m;
   }
-  if (best4) {
-    best4[0] = s->tune_G > 0 ? s->tune_G : g0; best4[1] = s->tune_cap; best4[2] = s->tune_xcd;
-    best4[3] = (int)k;
+  if (best5) {
+    best5[0] = s->tune_mode >= 0 ? s->tune_mode : ctx->gather_mode;
+    best5[1] = s->tune_G > 0 ? s->tune_G : g0; best5[2] = s->tune_cap; best5[3] = s->tune_xcd;
+    best5[4] = (int)k;
   }
   return 0;
 }
@@ -1095,6 +1230,7 @@ int pst_prod_time_forces(pst_sim* s, int steps, double* out6) {
   if (!s->prod_ready) FAIL("pst_prod_time_forces: call pst_prod_init first");
   if (steps <= 0 || !out6) FAIL("pst_prod_time_forces: bad arguments");
   CK(set_dev(ctx));
+  if (sim_ensure_bins(s)) return 2;
   if (!s->census) CK(hipMalloc(&s->census, 4 * sizeof(unsigned long long)));
   CK(hipMemsetAsync(s->census, 0, 4 * sizeof(unsigned long long), ctx->stream));
   CK(hipEventCreate(&s->tev0));
@@ -1134,14 +1270,16 @@ int pst_prod_time_forces(pst_sim* s, int steps, double* out6) {
 }
 
 // Force-launch configuration of the production step, set by hand.
-int pst_prod_set_launch(pst_sim* s, int G, int blocks_cap, int xcd) {
+int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
+  if (mode < -1 || mode > 1) FAIL("pst_prod_set_launch: mode must be -1, 0 or 1");
   if (!(G == 0 || G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64))
     FAIL("pst_prod_set_launch: G must be 0 or a power of two up to 64");
   if (blocks_cap < 1) FAIL("pst_prod_set_launch: blocks_cap must be positive");
-  s->tune_G = G; s->tune_cap = blocks_cap; s->tune_xcd = xcd ? 1 : 0;
-  return 0;
+  CK(set_dev(ctx));
+  s->tune_mode = mode; s->tune_G = G; s->tune_cap = blocks_cap; s->tune_xcd = xcd ? 1 : 0;
+  return s->prod_ready ? sim_ensure_bins(s) : 0;
 }
 
 int pst_prod_outcomes(pst_sim* s, signed char* status, int* steps) {

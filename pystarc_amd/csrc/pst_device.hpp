@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "pst_types.hpp"
 
 // ---------------------------------------------------------------------------
@@ -32,6 +33,18 @@ __host__ __device__ inline PhiloxOut philox4x32_10(uint32_t c0, uint32_t c1, uin
 }
 
 enum { RNG_INIT = 1, RNG_ESC = 2, RNG_RETQ = 3, RNG_NOISE = 4, RNG_BRIDGE = 5 };
+
+// Force-kernel geometry shared by the direct and the binned gathers.
+constexpr int kForceBlock = 256;
+constexpr int kLdsAtomsMax = 1536;  // ligand atoms staged in LDS (<= 48 KB in FP64)
+
+// Sum over the G lanes of a trajectory group (butterfly, FP64).
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
 
 // Two uniforms in [0,1) with 53 random bits each.
 __device__ inline void philox_uniform2(uint64_t seed, uint64_t traj, uint32_t step,
@@ -128,8 +141,26 @@ template <> __device__ inline Pair<double> gld<Pair<double>, float2>(const float
   return {(double)v.x, (double)v.y};
 }
 
-template <typename TG, typename TC, typename V>
-__device__ inline V grid_interp(const GridDev& g, double x, double y, double z) {
+// Node addressing.  The interpolation arithmetic always works in the grid's
+// own index space (cell i0 = floor((p - o) * inv)); an accessor turns node
+// (i, j, k) into an offset into the array it reads: the whole grid in HBM, or
+// a tile of it staged in LDS whose first node is (ti, tj, tk).
+struct GlobalAcc {
+  long sx, sy;
+  __device__ long at(int i, int j, int k) const { return (long)i * sx + (long)j * sy + k; }
+};
+struct TileAcc {
+  long sx, sy;
+  int ti, tj, tk;
+  __device__ long at(int i, int j, int k) const {
+    return (long)(i - ti) * sx + (long)(j - tj) * sy + (k - tk);
+  }
+};
+__device__ inline GlobalAcc global_acc(const GridDev& g) { return {(long)g.ny * g.nz, (long)g.nz}; }
+
+template <typename TG, typename TC, typename V, typename A>
+__device__ inline V grid_interp_a(const GridDev& g, const TG* __restrict__ d, const A& a, double x,
+                                  double y, double z) {
   const double ix = (x - g.o[0]) * g.inv[0];
   const double iy = (y - g.o[1]) * g.inv[1];
   const double iz = (z - g.o[2]) * g.inv[2];
@@ -138,16 +169,20 @@ __device__ inline V grid_interp(const GridDev& g, double x, double y, double z) 
   if (i0 < 0 || i0 >= g.nx - 1 || j0 < 0 || j0 >= g.ny - 1 || k0 < 0 || k0 >= g.nz - 1)
     return V{};
   const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
-  const TG* __restrict__ d = (const TG*)g.data;
-  const long sxy = (long)g.ny * g.nz;
-  const long base = (long)i0 * sxy + (long)j0 * g.nz + k0;
+  const long sxy = a.sx, sz = a.sy;
+  const long base = a.at(i0, j0, k0);
   const V c000 = gld<V>(d, base), c001 = gld<V>(d, base + 1);
-  const V c010 = gld<V>(d, base + g.nz), c011 = gld<V>(d, base + g.nz + 1);
+  const V c010 = gld<V>(d, base + sz), c011 = gld<V>(d, base + sz + 1);
   const V c100 = gld<V>(d, base + sxy), c101 = gld<V>(d, base + sxy + 1);
-  const V c110 = gld<V>(d, base + sxy + g.nz), c111 = gld<V>(d, base + sxy + g.nz + 1);
+  const V c110 = gld<V>(d, base + sxy + sz), c111 = gld<V>(d, base + sxy + sz + 1);
   const TC gx = (TC)1 - fx, gy = (TC)1 - fy, gz = (TC)1 - fz;
   return c000 * gx * gy * gz + c100 * fx * gy * gz + c010 * gx * fy * gz + c001 * gx * gy * fz +
          c110 * fx * fy * gz + c101 * fx * gy * fz + c011 * gx * fy * fz + c111 * fx * fy * fz;
+}
+
+template <typename TG, typename TC, typename V>
+__device__ inline V grid_interp(const GridDev& g, double x, double y, double z) {
+  return grid_interp_a<TG, TC, V>(g, (const TG*)g.data, global_acc(g), x, y, z);
 }
 
 // Strictly inside the valid box (gpu_batch_engine.py:317-319).
@@ -157,17 +192,27 @@ __device__ inline bool grid_inside(const GridDev& g, double x, double y, double 
 
 // phi and grad phi by central differences at +-sp/2 (gpu_batch_engine.py:111-115),
 // seven interpolations exactly as the reference evaluates them.
+template <typename TG, typename TC, typename V, typename A>
+__device__ inline void grid_phi_grad_a(const GridDev& g, const TG* __restrict__ d, const A& a,
+                                       double px, double py, double pz, V& phi, V& gx, V& gy,
+                                       V& gz) {
+  const double h0 = g.sp[0] * 0.5, h1 = g.sp[1] * 0.5, h2 = g.sp[2] * 0.5;
+  gx = (grid_interp_a<TG, TC, V>(g, d, a, px + h0, py, pz) -
+        grid_interp_a<TG, TC, V>(g, d, a, px - h0, py, pz)) /
+       (TC)(2 * h0);
+  gy = (grid_interp_a<TG, TC, V>(g, d, a, px, py + h1, pz) -
+        grid_interp_a<TG, TC, V>(g, d, a, px, py - h1, pz)) /
+       (TC)(2 * h1);
+  gz = (grid_interp_a<TG, TC, V>(g, d, a, px, py, pz + h2) -
+        grid_interp_a<TG, TC, V>(g, d, a, px, py, pz - h2)) /
+       (TC)(2 * h2);
+  phi = grid_interp_a<TG, TC, V>(g, d, a, px, py, pz);
+}
+
 template <typename TG, typename TC, typename V>
 __device__ inline void grid_phi_grad(const GridDev& g, double px, double py, double pz, V& phi,
                                      V& gx, V& gy, V& gz) {
-  const double h0 = g.sp[0] * 0.5, h1 = g.sp[1] * 0.5, h2 = g.sp[2] * 0.5;
-  gx = (grid_interp<TG, TC, V>(g, px + h0, py, pz) - grid_interp<TG, TC, V>(g, px - h0, py, pz)) /
-       (TC)(2 * h0);
-  gy = (grid_interp<TG, TC, V>(g, px, py + h1, pz) - grid_interp<TG, TC, V>(g, px, py - h1, pz)) /
-       (TC)(2 * h1);
-  gz = (grid_interp<TG, TC, V>(g, px, py, pz + h2) - grid_interp<TG, TC, V>(g, px, py, pz - h2)) /
-       (TC)(2 * h2);
-  phi = grid_interp<TG, TC, V>(g, px, py, pz);
+  grid_phi_grad_a<TG, TC, V>(g, (const TG*)g.data, global_acc(g), px, py, pz, phi, gx, gy, gz);
 }
 
 // Same phi and central-difference gradient, evaluated from the 20 distinct
@@ -178,9 +223,10 @@ __device__ inline void grid_phi_grad(const GridDev& g, double px, double py, dou
 // are weighted, which keeps FP32 free of the cancellation of two nearly
 // equal interpolated values, and every node is loaded once.  Callers
 // guarantee the whole stencil is inside the grid (the 3-spacing valid box).
-template <typename TG, typename TC, typename V>
-__device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double py, double pz,
-                                             V& phi, V& gx, V& gy, V& gz) {
+template <typename TG, typename TC, typename V, typename A>
+__device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __restrict__ d,
+                                               const A& a, double px, double py, double pz,
+                                               V& phi, V& gx, V& gy, V& gz) {
   const double ix = (px - g.o[0]) * g.inv[0];
   const double iy = (py - g.o[1]) * g.inv[1];
   const double iz = (pz - g.o[2]) * g.inv[2];
@@ -190,9 +236,8 @@ __device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double
   const double mi = floor(ix - 0.5), mj = floor(iy - 0.5), mk = floor(iz - 0.5);
   const TC ex = (TC)(ix - 0.5 - mi), ey = (TC)(iy - 0.5 - mj), ez = (TC)(iz - 0.5 - mk);
   const int ai = (int)mi - i0, aj = (int)mj - j0, ak = (int)mk - k0;  // -1 or 0
-  const TG* __restrict__ d = (const TG*)g.data;
-  const long sy = g.nz, sx = (long)g.ny * g.nz;
-  const long b = (long)i0 * sx + (long)j0 * sy + k0;
+  const long sy = a.sy, sx = a.sx;
+  const long b = a.at(i0, j0, k0);
   // centre cell
   const V c000 = gld<V>(d, b), c001 = gld<V>(d, b + 1), c010 = gld<V>(d, b + sy),
           c011 = gld<V>(d, b + sy + 1);
@@ -252,6 +297,13 @@ __device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double
   }
 }
 
+template <typename TG, typename TC, typename V>
+__device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double py, double pz,
+                                             V& phi, V& gx, V& gy, V& gz) {
+  grid_phi_grad_stencil_a<TG, TC, V>(g, (const TG*)g.data, global_acc(g), px, py, pz, phi, gx, gy,
+                                     gz);
+}
+
 // FP64 follows the reference's seven interpolations literally (parity with
 // gpu_batch_engine.py to rounding); FP32 uses the node-difference stencil.
 template <typename TC>
@@ -289,6 +341,27 @@ __device__ inline void grid_phi_grad_pair<float>(const GridDev& g, double px, do
                                                  Pair<float>& phi, Pair<float>& gx,
                                                  Pair<float>& gy, Pair<float>& gz) {
   grid_phi_grad_stencil<float2, float, Pair<float>>(g, px, py, pz, phi, gx, gy, gz);
+}
+
+// The same two gathers through an accessor (a grid tile staged in LDS):
+// the float2 potential + Born pair, and a single FP32 field.
+template <typename TC, typename A>
+__device__ inline void grid_phi_grad_pair_a(const GridDev& g, const float2* d, const A& a,
+                                            double px, double py, double pz, Pair<TC>& phi,
+                                            Pair<TC>& gx, Pair<TC>& gy, Pair<TC>& gz) {
+  if constexpr (std::is_same<TC, double>::value)
+    grid_phi_grad_a<float2, double, Pair<double>>(g, d, a, px, py, pz, phi, gx, gy, gz);
+  else
+    grid_phi_grad_stencil_a<float2, float, Pair<float>>(g, d, a, px, py, pz, phi, gx, gy, gz);
+}
+template <typename TC, typename A>
+__device__ inline void grid_phi_grad_f32_a(const GridDev& g, const float* d, const A& a,
+                                           double px, double py, double pz, TC& phi, TC& gx,
+                                           TC& gy, TC& gz) {
+  if constexpr (std::is_same<TC, double>::value)
+    grid_phi_grad_a<float, double, double>(g, d, a, px, py, pz, phi, gx, gy, gz);
+  else
+    grid_phi_grad_stencil_a<float, float, float>(g, d, a, px, py, pz, phi, gx, gy, gz);
 }
 
 template <typename TC> __device__ inline TC pst_exp(TC x);

@@ -17,8 +17,6 @@
 
 namespace pst {
 
-constexpr int kForceBlock = 256;
-constexpr int kLdsAtomsMax = 1536;  // ligand atoms staged in LDS (<= 48 KB in FP64)
 
 template <typename TC> struct Vec4 { typedef double4 type; };
 template <> struct Vec4<float> { typedef float4 type; };
@@ -90,12 +88,6 @@ __device__ inline bool atom_reverse(const ForceSys& S, double x, double y, doubl
   return true;
 }
 
-template <int G>
-__device__ inline double group_sum(double v) {
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
-  return v;
-}
 
 template <typename TC, int G, bool POSE>
 __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W) {

@@ -17,6 +17,29 @@ hipError_t launch_census(const ForceSys& S, const ForceWork& W, unsigned long lo
 // Empty marker kernel bracketing a timing window in a kernel trace.
 hipError_t launch_window_mark(hipStream_t st);
 
+// Brick-binned force evaluation: request counts / order per brick of the
+// receptor float2 pair grid and of the ligand Born grid, and per-request results.
+struct BinBufs {
+  int* hist;            // [nb_rec + nb_lig] request counts (reset by the scan)
+  int* start;           // [nb_rec + nb_lig + 1] first request of each brick
+  int* cursor;          // [nb_rec + nb_lig] scatter cursors
+  int2* items;          // [max_items] (brick, offset) work items of the gather kernel
+  int* n_items;         // [1]
+  int* sorted;          // request ids in brick order: w*M + i, rev_base + w*n_rec + j
+  void* out;            // [max_traj * max_m] per-atom {F, E} (double4 / float4)
+  void* out_rev;        // [max_traj * max_rec] per-receptor-atom reverse force
+  int nb_rec, nb_lig;   // bricks of each grid
+  int rb[3], lb[3];     // bricks per axis
+  int rev_base;         // max_traj * max_m
+  int max_items, max_traj, max_m, max_rec;
+  int gather_blocks;    // multiple of 8
+};
+bool binned_eligible(const ForceSys& S, const ForceWork& W, const BinBufs& B, int max_n);
+hipError_t launch_forces_binned(const ForceSys& S, const ForceWork& W, const BinBufs& B,
+                                int precision, int G, int max_n, int blocks_cap, hipStream_t st);
+int bricks_per_axis(int n_nodes);
+int bin_items_bound(long max_req, int n_bins);
+
 // Ordered (stable) compaction of in_list (or 0..n-1) by flag[slot].
 hipError_t launch_compact_ordered(const int* in_list, int n_in, const int* n_in_dev,
                                   const unsigned char* flag, int* out, int* n_out, hipStream_t st);

@@ -23,6 +23,7 @@ Extra keyword-only options (defaults give the reference behaviour):
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
@@ -124,6 +125,27 @@ class GPUBatchForceEngine:
         lo, hi = _bounds(data, origin, sp)
         return {"nx": nx, "ny": ny, "nz": nz, "origin": origin, "sp": sp, "lo": lo, "hi": hi,
                 "dtype": data.dtype}
+
+    def set_gather(self, mode="direct", max_traj=0):
+        """Force evaluation of forces_pose (and the default of new simulators):
+        "direct" - one lane group per trajectory gathers from the grids;
+        "binned" - gather requests sorted by 8^3-cell grid brick, each brick
+        staged once in LDS; identical results; buffers for max_traj
+        trajectories.  Without the float2 receptor pair grid the direct
+        kernel runs."""
+        if mode not in ("direct", "binned"):
+            raise ValueError("mode must be 'direct' or 'binned'")
+        m = 1 if mode == "binned" else 0
+        self.ctx.check(self.ctx.lib.pst_set_gather_mode(self.ctx.h, m, int(max_traj) if m else 0),
+                       "pst_set_gather_mode")
+        self.gather = mode
+
+    def last_eval(self):
+        """(mode, G) of the last force evaluation: "direct" / "binned", lanes."""
+        m, g = C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.lib.pst_last_force_eval(self.ctx.h, C.byref(m), C.byref(g)),
+                       "pst_last_force_eval")
+        return {-1: None, 0: "direct", 1: "binned"}[m.value], g.value
 
     def set_ligand(self, body, charges):
         """Body-frame ligand coordinates (about the centroid) for the fused path."""

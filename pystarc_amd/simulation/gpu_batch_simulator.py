@@ -26,8 +26,11 @@ Keyword-only extensions (defaults keep the reference's physics):
   audit_tol    relative RMS force error above which the run falls back to
                the grid path for every trajectory.
   autotune     False (default) or the number of warm-up steps (True: 64)
-               after which the production force launch (lanes per trajectory,
-               workgroup cap, XCD mapping) is tuned on live steps.
+               after which the production force launch (direct or
+               brick-binned evaluation, lanes per trajectory, workgroup cap,
+               XCD mapping) is tuned on live steps.
+  gather       None (the engine's), "direct" or "binned": force evaluation
+               of the production step (identical results either way).
   hydro        "scalar" (default): the reference's scalar relative diffusion
                (D_t, or D_parallel(r) with hydrodynamic_interactions).
                "rpy12": both bodies translate and turn with the two-sphere
@@ -141,15 +144,16 @@ class GPUBatchResult:
 def autotune_sim(lib, ctx, h, rounds=2, steps_per_cand=1):
     """Runs pst_prod_autotune on a production sim handle; returns the table of
     candidates {G, cap, xcd, ms} and the chosen launch."""
-    table = np.zeros((64, 4))
+    table = np.zeros((96, 5))
     n = C.c_int()
-    best = (C.c_int * 4)()
-    ctx.check(lib.pst_prod_autotune(h, int(rounds), int(steps_per_cand), ptr(table), 64,
+    best = (C.c_int * 5)()
+    ctx.check(lib.pst_prod_autotune(h, int(rounds), int(steps_per_cand), ptr(table), 96,
                                     C.byref(n), best), "pst_prod_autotune")
-    rows = [{"G": int(r[0]), "cap": int(r[1]), "xcd": int(r[2]), "ms": float(r[3])}
-            for r in table[:n.value]]
-    return {"best": {"G": best[0], "cap": best[1], "xcd": best[2]}, "table": rows,
-            "steps": int(best[3])}
+    rows = [{"mode": "binned" if r[0] else "direct", "G": int(r[1]), "cap": int(r[2]),
+             "xcd": int(r[3]), "ms": float(r[4])} for r in table[:n.value]]
+    return {"best": {"mode": "binned" if best[0] else "direct", "G": best[1], "cap": best[2],
+                     "xcd": best[3]},
+            "table": rows, "steps": int(best[4])}
 
 
 def _shoemake_host(u):
@@ -162,7 +166,7 @@ def _shoemake_host(u):
 class GPUBatchSimulator:
     def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
                  n_slots=None, precision=None, graph_steps=16, record=False, surrogate=None,
-                 audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False):
+                 audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False, gather=None):
         if rng not in ("philox", "numpy"):
             raise ValueError("rng must be 'philox' or 'numpy'")
         if hydro not in ("scalar", "rpy12"):
@@ -173,6 +177,9 @@ class GPUBatchSimulator:
             raise ValueError("rpy12 needs both hydrodynamic radii (mob.radius1, mob.radius2)")
         self.hydro = hydro
         self.autotune = int(autotune) if autotune is not True else 64
+        if gather not in (None, "direct", "binned"):
+            raise ValueError("gather must be None, 'direct' or 'binned'")
+        self.gather = gather
         self.mol1, self.mol2, self.mob = mol1, mol2, mob
         self.pathway_set, self.params, self.engine = pathway_set, params, batch_engine
         self.rng_mode = rng
@@ -325,6 +332,9 @@ class GPUBatchSimulator:
         prec = 0 if self.precision == "fp64" else 1
         eng.ctx.check(eng.ctx.lib.pst_sim_create(eng.ctx.h, C.byref(P), prec, C.byref(h)),
                       "pst_sim_create")
+        if self.gather is not None:
+            eng.ctx.check(eng.ctx.lib.pst_prod_set_launch(
+                h, 1 if self.gather == "binned" else 0, 0, 2048, 1), "pst_prod_set_launch")
         return h
 
     # -- run -----------------------------------------------------------------------

@@ -79,7 +79,8 @@ def test_autotune_measures_every_candidate_and_keeps_outcomes():
     ms = [r["ms"] for r in tune["table"]]
     assert len(ms) >= 10 and min(ms) > 0
     best = tune["best"]
-    row = [r for r in tune["table"] if (r["G"], r["cap"], r["xcd"]) == (best["G"], best["cap"], best["xcd"])]
+    key = ("mode", "G", "cap", "xcd")
+    row = [r for r in tune["table"] if all(r[k] == best[k] for k in key)]
     assert row and row[0]["ms"] == min(ms)
     o1, o2 = a.sim_data["outcome"], b.sim_data["outcome"]
     both = (o1 > 0) & (o2 > 0)
