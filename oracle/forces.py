@@ -208,7 +208,10 @@ def born_reverse(R, cen, rec_pos, rec_q, lig_grid, alpha, chunk=None, per_atom=F
     fa_all = np.zeros((B, rec_pos.shape[0], 3))
     radius = float(max(abs(lig_grid["lo"][0]), abs(lig_grid["hi"][0])))
     no_skip = chunk == 0            # chunk=0: every trajectory evaluated, no skip
-    cs = reverse_chunk_size(rec_pos.shape[0]) if chunk is None else max(chunk, 1)
+    if no_skip:
+        cs = max(B, 1)
+    else:
+        cs = reverse_chunk_size(rec_pos.shape[0]) if chunk is None else max(chunk, 1)
     cs = min(cs, B) if B > 0 else cs
     rec_max = float(np.linalg.norm(rec_pos, axis=1).max())
     for c0 in range(0, B, cs):

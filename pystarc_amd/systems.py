@@ -257,3 +257,49 @@ def make_engine(system: BDSystem, grids: Dict[str, list], **kw):
     args.update(kw)
     return GPUBatchForceEngine(grids["elec"], grids["born"], lig_born_grids=grids["lig_born"] or None,
                                **args)
+
+
+def from_pqr(receptor_pqr, ligand_pqr, rxns_xml, *, b=None, n_traj=100_000, seed=1,
+             grid=(193, 0.5), lig_grid=None, a_rec=None, a_lig=None, debye=DEBYE_015M,
+             max_steps=1_000_000, hi=True, overlap=True, dtype="float32", name=None):
+    """A batch system from the reference's input files: receptor and ligand
+    PQR (pystarc/structures/pqr_io.py) and rxns.xml with 1-based atom indices
+    (pystarc/pipeline/geometry.py:321-389).  Both molecules are centred on
+    their centroids, as the pipeline centres them before the BD stage
+    (prepare_bd_surface.py:260-295).  APBS is not part of this build: the
+    receptor potential / Born grids and the ligand Born grid are the
+    Debye-Hueckel and Born sums of the PQR charges and radii on cubic grids
+    (``grid`` / ``lig_grid`` = (nodes, spacing)).  Hydrodynamic radii default
+    to each molecule's bounding radius (farthest atom centre + its radius);
+    b defaults to 1.1 x (a_rec + a_lig + 15 A); r_escape = 2 b as the
+    pipeline passes it (geometry.py:291-292)."""
+    from .pipeline.geometry import pathway_set_from_rxns
+    from .structures.pqr_io import parse_pqr
+    mol1, mol2 = parse_pqr(receptor_pqr), parse_pqr(ligand_pqr)
+    for mol in (mol1, mol2):
+        c = mol.centroid()
+        for a in mol.atoms:
+            a.x, a.y, a.z = a.x - c[0], a.y - c[1], a.z - c[2]
+    rp, rq, rr = mol1.positions_array(), mol1.charges_array(), mol1.radii_array()
+    lp, lq, lr = mol2.positions_array(), mol2.charges_array(), mol2.radii_array()
+    if a_rec is None:
+        a_rec = float((np.linalg.norm(rp, axis=1) + rr).max())
+    if a_lig is None:
+        a_lig = float((np.linalg.norm(lp, axis=1) + lr).max())
+    if b is None:
+        b = 1.1 * (a_rec + a_lig + 15.0)
+    ps = pathway_set_from_rxns(rxns_xml)
+    mob = MobilityTensor.from_radii(a_rec, a_lig)
+    params = _params(n_traj, float(b), seed, max_steps, overlap=overlap, hi=hi)
+    gn, gh = grid
+    ln, lh = lig_grid if lig_grid is not None else grid
+    specs = [GridSpec("elec", 0, (gn,) * 3, _cube_origin(gn, gh), gh, dtype, _atoms5(rp, rq, rr), debye),
+             GridSpec("born", 1, (gn,) * 3, _cube_origin(gn, gh), gh, dtype, _atoms5(rp, rq, rr), debye),
+             GridSpec("lig_born", 1, (ln,) * 3, _cube_origin(ln, lh), lh, dtype,
+                      _atoms5(lp, lq, lr), debye)]
+    mp = MultipoleExpansion(rp, rq, debye, 78.0)
+    kw = dict(alpha=ALPHA_DEFAULT, receptor_charge=float(rq.sum()), debye_length=debye, sdie=78.0,
+              rec_positions=rp, rec_charges=rq, multipole_expansion=mp)
+    return BDSystem(name or f"{mol1.name}-{mol2.name}", mol1, mol2, mob, ps, params, kw, specs,
+                    {"a_rec": a_rec, "a_lig": a_lig, "files": [str(receptor_pqr), str(ligand_pqr),
+                                                               str(rxns_xml)]})

@@ -1,0 +1,63 @@
+"""A system built from the reference's input files (PQR + rxns.xml of the
+two-charged-spheres example) runs through the production step, matches the
+production oracle per trajectory, and writes results.json."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle.production import ProductionOracle  # noqa: E402
+from pystarc_amd import systems as S  # noqa: E402
+from pystarc_amd.pipeline.output_writer import write_results_json  # noqa: E402
+from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchSimulator  # noqa: E402
+from tests._systems import dx_grids, numpy_grids, oracle_params, oracle_system  # noqa: E402
+
+F = Path(__file__).parent / "golden" / "formats"
+
+
+def _system(n):
+    return S.from_pqr(F / "two_spheres_receptor.pqr", F / "two_spheres_ligand.pqr",
+                      F / "two_spheres_rxns.xml", n_traj=n, seed=11111113, grid=(81, 0.5))
+
+
+def test_files_production_matches_oracle():
+    sysm = _system(400)
+    g = numpy_grids(sysm)
+    eng = S.make_engine(sysm, dx_grids(g))
+    sim = GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
+                            n_slots=400, precision="fp64")
+    drv = sim.production_driver(record=True)
+    try:
+        k = drv.advance(300)
+        pos, q, st, ns = drv.state()
+        out, steps = drv.outcomes()
+    finally:
+        drv.close()
+    orc = ProductionOracle(oracle_system(sysm, g, chunk=0), oracle_params(sysm))
+    for _ in range(k):
+        orc.step()
+    fin = (out != 0) & (orc.status != 0)
+    assert fin.sum() > 50
+    assert np.array_equal(out[fin], orc.status[fin])
+    assert np.array_equal(steps[fin], orc.out_steps[fin])
+    run = (out == 0) & (orc.status == 0)
+    assert np.array_equal(out == 0, orc.status == 0)
+    np.testing.assert_allclose(pos[run], orc.pos[run], rtol=0, atol=1e-8)
+
+
+def test_files_run_writes_results_json(tmp_path):
+    sysm = _system(20000)
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+    res = GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params,
+                            eng).run()
+    D_rel = sysm.mob.relative_translational_diffusion()
+    d = json.loads(write_results_json(res, tmp_path, D_rel, res.k_b).read_text())
+    assert d["n_trajectories"] == 20000
+    assert d["n_reacted"] + d["n_escaped"] >= int(0.995 * 20000)
+    assert 0.0 < d["P_rxn_low"] <= d["P_rxn"] <= d["P_rxn_high"] < 1.0
+    assert d["k_on"] > 0 and d["k_b"] == res.k_b and d["steps_per_sec"] > 0
