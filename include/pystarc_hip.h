@@ -155,7 +155,8 @@ int pst_replay_forces(pst_sim* sim, double* F_host, int n_sr);
 
 /* Production mode: Philox noise on the device, a fixed pool of slots refilled
  * from the trajectory counter, no host synchronisation inside a launch.
- * record != 0 keeps per-trajectory outcome / step count (n_total each). */
+ * record != 0 keeps per-trajectory outcome / step count (n_total each);
+ * record = 2 also collects the observables (pst_prod_observables). */
 int pst_prod_init(pst_sim* sim, int record);
 /* Runs up to max_steps device steps, `graph_steps` per captured hipGraph
  * launch (0 = plain launches); stops early at the reference's 99.5 % rule
@@ -165,6 +166,21 @@ int pst_prod_init(pst_sim* sim, int record);
  * 0, 0, 0, 0} as doubles. */
 int pst_prod_run(pst_sim* sim, long long max_steps, int graph_steps, double* counters16);
 int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
+/* Observables of a run initialised with record = 2, collected on the device
+ * instead of the reference's per-step host loops (sim_data of
+ * gpu_batch_simulator.py:576-620, :1255-1340, :1524-1557).  traj: host
+ * (n_total, traj_doubles) = start pos 0-2, start quaternion 3-6, closest
+ * centroid distance 7, the trajectory's own step at it 8, pose there 9-15,
+ * simulated time (ps) 16, returns 17, reacted in a Brownian bridge 18, last
+ * pose 19-25, pairs within cutoff at the last pose 26 (slots still running
+ * report their state so far).  hist: host (hist_len) counts = radial
+ * histogram (200 bins over [0, 1.2 r_esc)), angular occupancy of the
+ * centroid direction (36 theta x 72 phi), milestone crossings out / in at
+ * linspace(b, r_esc, 11), radial transition matrix (50 x 50 bins over
+ * [0, 1.2 r_esc)), per-pair contact counts at the pose a step starts from
+ * (16 slots), trajectory-steps counted.  Either pointer may be NULL. */
+int pst_prod_obs_sizes(int* traj_doubles, int* hist_len);
+int pst_prod_observables(pst_sim* sim, double* traj, unsigned long long* hist);
 /* Mean device time per step (ms) of the last pst_prod_run, HIP events. */
 int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);
 /* Autotuner of the production force launch (no reference counterpart: the

@@ -51,8 +51,9 @@ class ProductionOracle(OracleSim):
     """``system`` / ``params`` as OracleSim; hydro "scalar" or "rpy12"."""
 
     def __init__(self, system, params, torque_mode="reference", hydro="scalar",
-                 kT_over_eta=1.0 / 0.243):
+                 kT_over_eta=1.0 / 0.243, observe=False):
         super().__init__(system, params, torque_mode)
+        self.observe = bool(observe)
         self.hydro = hydro
         self.Df = float(kT_over_eta)
         self.a0, self.a1 = float(system["a_rec"]), float(system["a_lig"])
@@ -73,10 +74,54 @@ class ProductionOracle(OracleSim):
         self.total_steps = 0
         self.n_returns = 0
         self.n_overlap_rejected = 0
+        if self.observe:
+            # the reference's sim_data (gpu_batch_simulator.py:576-620,
+            # :1255-1340), per trajectory and run-wide; step_at_min is the
+            # trajectory's own step
+            P = len(self.cut)
+            self.obs_traj = np.zeros((N, 27))
+            self.obs_traj[:, 0:3] = self.pos
+            self.obs_traj[:, 3:7] = self.q
+            self.obs_traj[:, 7] = 1e30
+            self.obs_traj[:, 9:12] = self.pos
+            self.obs_traj[:, 12:16] = self.q
+            # on the b-sphere by construction: exactly b, so the milestone at
+            # b is not crossed by the rounding of |pos|
+            self.prev_r = np.full(N, self.r_b)
+            self.h_rad = np.zeros(200, dtype=np.int64)
+            self.h_ang = np.zeros((36, 72), dtype=np.int64)
+            self.h_ms_out = np.zeros(11, dtype=np.int64)
+            self.h_ms_in = np.zeros(11, dtype=np.int64)
+            self.h_trans = np.zeros((50, 50), dtype=np.int64)
+            self.h_contact = np.zeros(max(P, 1), dtype=np.int64)
+            self.h_contact_steps = 0
 
-    def _finish(self, idx, code, steps):
+    def _finish(self, idx, code, steps, bridge=0):
         self.status[idx] = code
         self.out_steps[idx] = steps
+        if self.observe and len(idx):
+            self._record_end(idx, bridge)
+
+    def _record_end(self, idx, bridge=0):
+        """Trajectory records of idx as they stand (end pose, pairs within)."""
+        t = self.obs_traj
+        t[idx, 18] = bridge
+        t[idx, 19:22] = self.pos[idx]
+        t[idx, 22:26] = self.q[idx]
+        if len(self.cut):
+            d = self.pair_dists(quats_to_rotmats(self.q[idx]), self.pos[idx])
+            t[idx, 26] = (d < self.cut[None, :]).sum(axis=1)
+
+    @staticmethod
+    def _bin(r, rmax, nb):
+        return np.clip(np.floor(r / (rmax / nb)).astype(np.int64), 0, nb - 1)
+
+    def observables(self):
+        """Records of every trajectory; running ones as they stand."""
+        run = np.where(self.status == 0)[0]
+        if len(run):
+            self._record_end(run)
+        return self.obs_traj
 
     def step(self):
         st = self.status
@@ -110,6 +155,8 @@ class ProductionOracle(OracleSim):
                 a2, _ = PH.uniform2(self.seed, ir, nr_, PH.RNG_RETQ, 1)
                 self.q[ir] = _shoemake(a0, a1, a2)
                 self.n_returns += len(ir)
+                if self.observe:
+                    self.obs_traj[ir, 17] += 1
             self._finish(ia[~ret], 2, na[~ret])
             keep = np.ones(len(run), dtype=bool)
             keep[np.where(at)[0][~ret]] = False
@@ -154,6 +201,8 @@ class ProductionOracle(OracleSim):
         self.pos[run] = newp
         self.q[run] = newq
         self.total_steps += n
+        if self.observe:
+            self._observe_move(run, ns, newp, newq, dt, old_d)
         if P:
             new_d = self.pair_dists(quats_to_rotmats(newq), newp)
             x0 = old_d - self.cut[None, :]
@@ -171,9 +220,38 @@ class ProductionOracle(OracleSim):
                 pc = np.where(both, np.exp(-x0 * x1 / Ddt), 0.0)
             fired = (u < pc) | (x1 < 0.0)
             bb = fired.sum(axis=1) >= self.n_needed
-            self._finish(run[bb], 1, ns[bb] + 1)
+            self._finish(run[bb], 1, ns[bb] + 1, bridge=1)
         self.n_steps[run] = ns + 1
         return True
+
+    def _observe_move(self, run, ns, newp, newq, dt, old_d):
+        t = self.obs_traj
+        rn = np.linalg.norm(newp, axis=1)
+        t[run, 16] += dt
+        closer = rn < t[run, 7]
+        ic = run[closer]
+        t[ic, 7] = rn[closer]
+        t[ic, 8] = ns[closer]
+        t[ic, 9:12] = newp[closer]
+        t[ic, 12:16] = newq[closer]
+        rmax = 1.2 * self.r_esc
+        np.add.at(self.h_rad, self._bin(rn, rmax, 200), 1)
+        rs = np.maximum(rn, 1e-30)
+        th = np.arccos(np.clip(newp[:, 2] / rs, -1.0, 1.0))
+        ph = np.arctan2(newp[:, 1], newp[:, 0]) + np.pi
+        ti = np.clip((th / np.pi * 36).astype(np.int64), 0, 35)
+        pj = np.clip((ph / (2.0 * np.pi) * 72).astype(np.int64), 0, 71)
+        np.add.at(self.h_ang, (ti, pj), 1)
+        rp = self.prev_r[run]
+        for m in range(11):
+            mr = self.r_b + (self.r_esc - self.r_b) * m / 10
+            self.h_ms_out[m] += int(((rp < mr) & (rn >= mr)).sum())
+            self.h_ms_in[m] += int(((rp >= mr) & (rn < mr)).sum())
+        np.add.at(self.h_trans, (self._bin(rp, rmax, 50), self._bin(rn, rmax, 50)), 1)
+        if len(self.cut):
+            self.h_contact += (old_d < self.cut[None, :]).sum(axis=0)
+        self.h_contact_steps += len(run)
+        self.prev_r[run] = rn
 
     def run(self, max_loop=None):
         k = 0

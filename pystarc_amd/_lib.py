@@ -83,6 +83,8 @@ _SIGS = {
     "pst_prod_init": ([_vp, C.c_int], C.c_int),
     "pst_prod_run": ([_vp, C.c_longlong, C.c_int, _vp], C.c_int),
     "pst_prod_outcomes": ([_vp, _vp, _vp], C.c_int),
+    "pst_prod_obs_sizes": ([_ip, _ip], C.c_int),
+    "pst_prod_observables": ([_vp, _vp, _vp], C.c_int),
     "pst_prod_autotune": ([_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp], C.c_int),
     "pst_prod_set_launch": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "pst_set_gather_mode": ([_vp, C.c_int, C.c_int], C.c_int),

@@ -93,6 +93,8 @@ struct pst_sim {
   RunCounters* ctr = nullptr;
   signed char* out_status = nullptr;
   int* out_steps = nullptr;
+  ObsBufs obs{};                     // observables (record level 2)
+  std::vector<void*> obs_allocs;
   double last_ms_per_step = 0;
   bool prod_ready = false;
   // surrogate branch
@@ -749,6 +751,7 @@ int pst_sim_destroy(pst_sim* s) {
                   s->audit_F, s->key, s->bins, s->census};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  for (void* p : s->obs_allocs) hipFree(p);
   delete s;
   return 0;
 }
@@ -967,6 +970,7 @@ static ProdBufs prod_bufs(pst_sim* s) {
   B.key = s->key; B.bins = s->bins;
   B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
   B.ctr = s->ctr; B.out_status = s->out_status; B.out_steps = s->out_steps;
+  B.obs = s->obs;
   B.n_slots = s->S;
   return B;
 }
@@ -994,6 +998,36 @@ int pst_prod_init(pst_sim* s, int record) {
   CK(hipMemsetAsync(s->bins, 0, 512 * 4, ctx->stream));
   if (s->out_status) { hipFree(s->out_status); s->out_status = nullptr; }
   if (s->out_steps) { hipFree(s->out_steps); s->out_steps = nullptr; }
+  for (void* p : s->obs_allocs) hipFree(p);
+  s->obs_allocs.clear();
+  memset(&s->obs, 0, sizeof(ObsBufs));
+  if (record >= 2) {
+    const size_t N = (size_t)s->P.n_total;
+    auto get = [&](size_t bytes) -> void* {
+      void* p = nullptr;
+      if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+      s->obs_allocs.push_back(p);
+      return p;
+    };
+    ObsBufs& O = s->obs;
+    O.s_min = (double*)get(S * 8);
+    O.s_step = (int*)get(S * 4);
+    O.s_time = (double*)get(S * 8);
+    O.s_ret = (int*)get(S * 4);
+    O.s_prev_r = (double*)get(S * 8);
+    O.s_nm = (double*)get(S * 7 * 8);
+    O.traj = (double*)get(N * kObsTraj * 8);
+    O.hist = (unsigned long long*)get(kObsHist * 8);
+    if (!O.s_min || !O.s_step || !O.s_time || !O.s_ret || !O.s_prev_r || !O.s_nm || !O.traj ||
+        !O.hist) {
+      for (void* p : s->obs_allocs) hipFree(p);
+      s->obs_allocs.clear();
+      memset(&s->obs, 0, sizeof(ObsBufs));
+      FAIL("pst_prod_init: out of device memory for observables");
+    }
+    CK(hipMemsetAsync(O.traj, 0, N * kObsTraj * 8, ctx->stream));
+    CK(hipMemsetAsync(O.hist, 0, kObsHist * 8, ctx->stream));
+  }
   if (record) {
     const size_t N = (size_t)s->P.n_total;
     CK(hipMalloc(&s->out_status, N));
@@ -1280,6 +1314,25 @@ int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   CK(set_dev(ctx));
   s->tune_mode = mode; s->tune_G = G; s->tune_cap = blocks_cap; s->tune_xcd = xcd ? 1 : 0;
   return s->prod_ready ? sim_ensure_bins(s) : 0;
+}
+
+int pst_prod_obs_sizes(int* traj_doubles, int* hist_len) {
+  if (traj_doubles) *traj_doubles = kObsTraj;
+  if (hist_len) *hist_len = kObsHist;
+  return 0;
+}
+
+int pst_prod_observables(pst_sim* s, double* traj, unsigned long long* hist) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->obs.traj) FAIL("pst_prod_observables: run was not recorded at level 2");
+  CK(set_dev(ctx));
+  CK(prod_flush_observables(s->P, prod_bufs(s), ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  if (traj)
+    CK(hipMemcpy(traj, s->obs.traj, (size_t)s->P.n_total * kObsTraj * 8, hipMemcpyDeviceToHost));
+  if (hist) CK(hipMemcpy(hist, s->obs.hist, kObsHist * 8, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int pst_prod_outcomes(pst_sim* s, signed char* status, int* steps) {

@@ -65,6 +65,27 @@ hipError_t replay_move(const StepParams& P, const ForceSys& S, const ReplayBufs&
 hipError_t replay_bridge(const StepParams& P, const ReplayBufs& B, const double* u_bb, int n_sr,
                          hipStream_t st);
 
+// Per-trajectory and run-wide observables of the production step (the
+// reference's sim_data, gpu_batch_simulator.py:576-620, :1255-1340).
+constexpr int kObsRad = 200;     // radial histogram bins over [0, 1.2 r_esc)
+constexpr int kObsTheta = 36;    // angular occupancy (theta, phi)
+constexpr int kObsPhi = 72;
+constexpr int kObsMs = 11;       // milestone radii linspace(b, r_esc, 11)
+constexpr int kObsTrans = 50;    // radial transition matrix bins over [0, 1.2 r_esc)
+constexpr int kObsTraj = 27;     // doubles per trajectory record
+// histogram block: rad | ang | ms_out | ms_in | trans | contact[16] | contact_steps
+constexpr int kObsHist = kObsRad + kObsTheta * kObsPhi + 2 * kObsMs + kObsTrans * kObsTrans +
+                         PST_MAX_PAIRS + 1;
+struct ObsBufs {
+  // per slot (accumulated while the slot holds a trajectory)
+  double* s_min; int* s_step; double* s_time; int* s_ret; double* s_prev_r; double* s_nm;  // s_nm (S,7)
+  // per trajectory id: (N, kObsTraj) = start pos 0-2, start q 3-6, min dist 7, step at min 8,
+  // near-miss pos 9-11, near-miss q 12-15, total time 16, returns 17, bridge 18,
+  // end pos 19-21, end q 22-25, pairs within cutoff at the end 26
+  double* traj;
+  unsigned long long* hist;   // [kObsHist]
+};
+
 // Production step (device Philox, refilling slots).
 struct ProdBufs {
   double* pos; double* quat; int* status; unsigned int* n_steps; long long* traj;
@@ -75,8 +96,10 @@ struct ProdBufs {
   RunCounters* ctr;
   signed char* out_status;     // per trajectory id (nullable)
   int* out_steps;              // per trajectory id (nullable)
+  ObsBufs obs;                 // observables (obs.traj null: off)
   int n_slots;
 };
+hipError_t prod_flush_observables(const StepParams& P, const ProdBufs& B, hipStream_t st);
 hipError_t prod_init(const StepParams& P, const ProdBufs& B, hipStream_t st);
 hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
 hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st);

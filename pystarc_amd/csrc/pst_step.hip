@@ -394,10 +394,48 @@ __device__ inline void start_traj(const StepParams& P, const ProdBufs& B, int s,
   B.traj[s] = t;
   B.n_steps[s] = 0;
   B.status[s] = PST_RUNNING;
+  if (B.obs.traj) {
+    const ObsBufs& O = B.obs;
+    double* rec = O.traj + (size_t)t * kObsTraj;
+    const double* q = B.quat + 4 * (size_t)s;
+    for (int k = 0; k < 3; ++k) { rec[k] = pos[k]; O.s_nm[7 * (size_t)s + k] = pos[k]; }
+    for (int k = 0; k < 4; ++k) { rec[3 + k] = q[k]; O.s_nm[7 * (size_t)s + 3 + k] = q[k]; }
+    O.s_min[s] = 1e30;
+    O.s_step[s] = 0;
+    O.s_time[s] = 0.0;
+    O.s_ret[s] = 0;
+    O.s_prev_r[s] = P.r_b;   // on the b-sphere by construction (no rounding of |pos|)
+  }
 }
 
-__device__ inline void finish_traj(const StepParams& P, const ProdBufs& B, int s, int outcome) {
+// Copy a slot's accumulated observables into its trajectory record.
+__device__ inline void obs_flush(const StepParams& P, const ProdBufs& B, int s, int bridge) {
+  const ObsBufs& O = B.obs;
+  double* rec = O.traj + (size_t)B.traj[s] * kObsTraj;
+  const double* pos = B.pos + 3 * (size_t)s;
+  const double* q = B.quat + 4 * (size_t)s;
+  rec[7] = O.s_min[s];
+  rec[8] = O.s_step[s];
+  for (int k = 0; k < 7; ++k) rec[9 + k] = O.s_nm[7 * (size_t)s + k];
+  rec[16] = O.s_time[s];
+  rec[17] = O.s_ret[s];
+  rec[18] = bridge;
+  for (int k = 0; k < 3; ++k) rec[19 + k] = pos[k];
+  for (int k = 0; k < 4; ++k) rec[22 + k] = q[k];
+  int within = 0;
+  if (P.n_pairs) {
+    double R[9], d[PST_MAX_PAIRS];
+    quat_to_rot(q, R);
+    pair_dists(P, R, pos[0], pos[1], pos[2], d);
+    for (int p = 0; p < P.n_pairs; ++p) within += d[p] < P.cutoff[p] ? 1 : 0;
+  }
+  rec[26] = within;
+}
+
+__device__ inline void finish_traj(const StepParams& P, const ProdBufs& B, int s, int outcome,
+                                   int bridge = 0) {
   const long long t = B.traj[s];
+  if (B.obs.traj) obs_flush(P, B, s, bridge);
   if (outcome == PST_REACTED) atomicAdd(&B.ctr->n_reacted, 1ull);
   if (outcome == PST_ESCAPED) atomicAdd(&B.ctr->n_escaped, 1ull);
   if (outcome == PST_MAXSTEPS) atomicAdd(&B.ctr->n_maxsteps, 1ull);
@@ -480,6 +518,7 @@ __global__ void k_prod_pre(StepParams P, ProdBufs B, int par) {
           philox_uniform2(P.seed, (uint64_t)t, ns, RNG_RETQ, 1, u2, u3);
           shoemake(u0, u1, u2, q);
           atomicAdd(&B.ctr->n_returns, 1ull);
+          if (B.obs.traj) B.obs.s_ret[s] += 1;
         } else {
           finish_traj(P, B, s, PST_ESCAPED);
           if (B.status[s] != PST_RUNNING) break;
@@ -561,41 +600,117 @@ hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Move every still-running slot (both lists).
-__global__ void k_prod_post(StepParams P, ForceSys S, ProdBufs B, int par) {
+// Histogram bins of the observables (the reference's linspace edges).
+__device__ inline int obs_bin(double r, double rmax, int nb) {
+  const int b = (int)floor(r / (rmax / nb));
+  return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+
+// Move every still-running slot (both lists).  With OBS the step also
+// accumulates the observables: per slot closest approach, step at it and the
+// pose there, simulated time; per workgroup in LDS the radial histogram, the
+// angular occupancy of the centroid direction, milestone crossings, the
+// radial transition matrix and pair contact counts (flushed with one global
+// atomic per non-zero bin).
+template <bool OBS>
+__global__ void __launch_bounds__(kStepBlock) k_prod_post(StepParams P, ForceSys S, ProdBufs B,
+                                                          int par) {
+  __shared__ int h[OBS ? kObsHist : 1];
+  if (OBS) {
+    for (int i = threadIdx.x; i < kObsHist; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+  }
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   const int n_plain = B.ctr->n_sr, n_mlp = B.ctr->n_mlp;
-  if (w >= n_plain + n_mlp) return;
-  const int s = w < n_plain ? B.plain_list[w] : B.mlp_list[w - n_plain];
-  double* pos = B.pos + 3 * (size_t)s;
-  double* q = B.quat + 4 * (size_t)s;
-  const long long t = B.traj[s];
-  const unsigned int ns = B.n_steps[s];
-  const double r = norm3(pos[0], pos[1], pos[2]);
-  const double dt = adaptive_dt(P, r);
-  const double D = diffusion(P, r);
-  B.dt[s] = dt;
-  bool rej, any;
-  double* x0 = B.x0 + (size_t)s * PST_MAX_PAIRS;
-  double* x1 = B.x1 + (size_t)s * PST_MAX_PAIRS;
-  if (P.hydro_mode == 1) {
-    double xi[12];
+  const bool active = w < n_plain + n_mlp;
+  bool any = false;
+  if (active) {
+    const int s = w < n_plain ? B.plain_list[w] : B.mlp_list[w - n_plain];
+    double* pos = B.pos + 3 * (size_t)s;
+    double* q = B.quat + 4 * (size_t)s;
+    const long long t = B.traj[s];
+    const unsigned int ns = B.n_steps[s];
+    const double r = norm3(pos[0], pos[1], pos[2]);
+    const double dt = adaptive_dt(P, r);
+    const double D = diffusion(P, r);
+    B.dt[s] = dt;
+    bool rej;
+    double* x0 = B.x0 + (size_t)s * PST_MAX_PAIRS;
+    double* x1 = B.x1 + (size_t)s * PST_MAX_PAIRS;
+    if (P.hydro_mode == 1) {
+      double xi[12];
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
-      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, (uint32_t)k, xi[2 * k], xi[2 * k + 1]);
-    any = move_one_rpy(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, xi, dt, x0, x1,
-                       &rej);
-  } else {
-    double nt[3], nr[3];
-    philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 0, nt[0], nt[1]);
-    philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 1, nt[2], nr[0]);
-    philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 2, nr[1], nr[2]);
-    any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt, D, x0, x1,
-                   &rej);
+      for (int k = 0; k < 6; ++k)
+        philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, (uint32_t)k, xi[2 * k], xi[2 * k + 1]);
+      any = move_one_rpy(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, xi, dt, x0, x1,
+                         &rej);
+    } else {
+      double nt[3], nr[3];
+      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 0, nt[0], nt[1]);
+      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 1, nt[2], nr[0]);
+      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 2, nr[1], nr[2]);
+      any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt, D, x0, x1,
+                     &rej);
+    }
+    if (rej) atomicAdd(&B.ctr->n_overlap, 1ull);
+    if (OBS) {
+      const ObsBufs& O = B.obs;
+      const double rn = norm3(pos[0], pos[1], pos[2]);
+      O.s_time[s] += dt;
+      if (rn < O.s_min[s]) {
+        O.s_min[s] = rn;
+        O.s_step[s] = (int)ns;
+        for (int k = 0; k < 3; ++k) O.s_nm[7 * (size_t)s + k] = pos[k];
+        for (int k = 0; k < 4; ++k) O.s_nm[7 * (size_t)s + 3 + k] = q[k];
+      }
+      const double rmax = 1.2 * P.r_esc;
+      atomicAdd(&h[obs_bin(rn, rmax, kObsRad)], 1);
+      const double rs = fmax(rn, 1e-30);
+      const double th = acos(fmin(fmax(pos[2] / rs, -1.0), 1.0));
+      const double ph = atan2(pos[1], pos[0]) + M_PI;
+      int ti = (int)(th / M_PI * kObsTheta), pj = (int)(ph / (2.0 * M_PI) * kObsPhi);
+      ti = ti < 0 ? 0 : (ti >= kObsTheta ? kObsTheta - 1 : ti);
+      pj = pj < 0 ? 0 : (pj >= kObsPhi ? kObsPhi - 1 : pj);
+      atomicAdd(&h[kObsRad + ti * kObsPhi + pj], 1);
+      const double rp = O.s_prev_r[s];
+      const int ms0 = kObsRad + kObsTheta * kObsPhi;
+      for (int m = 0; m < kObsMs; ++m) {
+        const double mr = P.r_b + (P.r_esc - P.r_b) * m / (kObsMs - 1);
+        if (rp < mr && rn >= mr) atomicAdd(&h[ms0 + m], 1);
+        if (rp >= mr && rn < mr) atomicAdd(&h[ms0 + kObsMs + m], 1);
+      }
+      const int tr0 = ms0 + 2 * kObsMs;
+      atomicAdd(&h[tr0 + obs_bin(rp, rmax, kObsTrans) * kObsTrans + obs_bin(rn, rmax, kObsTrans)], 1);
+      const int c0 = tr0 + kObsTrans * kObsTrans;
+      // contacts at the pose the step started from (gpu_batch_simulator.py:1300-1311)
+      for (int p = 0; p < P.n_pairs; ++p)
+        if (x0[p] < 0.0) atomicAdd(&h[c0 + p], 1);
+      atomicAdd(&h[c0 + PST_MAX_PAIRS], 1);
+      O.s_prev_r[s] = rn;
+    }
   }
-  if (rej) atomicAdd(&B.ctr->n_overlap, 1ull);
   const unsigned long long m = __ballot(any);
   if (m && __lane_id() == __ffsll((long long)m) - 1) atomicOr(&B.ctr->any_above[par], 1);
+  if (OBS) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kObsHist; i += blockDim.x)
+      if (h[i]) atomicAdd(&B.obs.hist[i], (unsigned long long)h[i]);
+  }
+}
+
+// Observables of the slots still running (after an early stop): their
+// accumulators go into their trajectory records as they stand.
+__global__ void k_prod_flush_obs(StepParams P, ProdBufs B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.n_slots || B.status[s] != PST_RUNNING) return;
+  obs_flush(P, B, s, 0);
+}
+
+hipError_t prod_flush_observables(const StepParams& P, const ProdBufs& B, hipStream_t st) {
+  if (!B.obs.traj) return hipSuccess;
+  hipLaunchKernelGGL(k_prod_flush_obs, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                     dim3(kStepBlock), 0, st, P, B);
+  return hipGetLastError();
 }
 
 // Bridge decision, n_steps += 1, finish + refill of bridged trajectories.
@@ -624,7 +739,7 @@ __global__ void k_prod_bridge(StepParams P, ProdBufs B, int par) {
   B.n_steps[s] = ns + 1;
   if (react) {
     atomicAdd(&C->n_bridge, 1ull);
-    finish_traj(P, B, s, PST_REACTED);
+    finish_traj(P, B, s, PST_REACTED, 1);
   }
 }
 
@@ -649,8 +764,12 @@ hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStr
 
 hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs& B, int par,
                           hipStream_t st) {
-  hipLaunchKernelGGL(k_prod_post, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
-                     dim3(kStepBlock), 0, st, P, S, B, par);
+  if (B.obs.traj)
+    hipLaunchKernelGGL(k_prod_post<true>, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                       dim3(kStepBlock), 0, st, P, S, B, par);
+  else
+    hipLaunchKernelGGL(k_prod_post<false>, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                       dim3(kStepBlock), 0, st, P, S, B, par);
   return hipGetLastError();
 }
 

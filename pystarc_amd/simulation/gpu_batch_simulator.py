@@ -19,7 +19,12 @@ Keyword-only extensions (defaults keep the reference's physics):
   n_slots      concurrent trajectories in philox mode (default min(N, 2^20)).
   precision    force precision "fp64" / "fp32" (default: the engine's).
   graph_steps  steps per captured hipGraph launch in philox mode.
-  record       keep per-trajectory outcome and step counts (sim_data).
+  record       True: keep per-trajectory outcome and step counts (sim_data);
+               "observables": also the reference's per-trajectory and run-wide
+               sim_data (start pose, closest approach, simulated time,
+               returns, bridge reactions, encounter / near-miss poses,
+               radial / angular / milestone / transition / contact counts),
+               collected on the device (philox mode).
   surrogate    a trained pystarc_amd.surrogate.Surrogate: trajectories beyond
                its contact shell take the MLP force (philox mode only).
   audit_n      surrogate trajectories re-evaluated on the grid path per step.
@@ -141,6 +146,50 @@ class GPUBatchResult:
         return "\n".join(lines)
 
 
+def observables(lib, ctx, h, P, outcome):
+    """sim_data entries of the reference (gpu_batch_simulator.py:1524-1557)
+    from the device observables of a record-level-2 run."""
+    td, hl = C.c_int(), C.c_int()
+    lib.pst_prod_obs_sizes(C.byref(td), C.byref(hl))
+    N = int(P.n_total)
+    tr = np.zeros((N, td.value))
+    hist = np.zeros(hl.value, dtype=np.uint64)
+    ctx.check(lib.pst_prod_observables(h, ptr(tr), ptr(hist)), "pst_prod_observables")
+    r_b, r_esc = float(P.r_b), float(P.r_esc)
+    nrad, nth, nph, nms, ntr = 200, 36, 72, 11, 50
+    o = 0
+    rad = hist[o:o + nrad].astype(np.int64); o += nrad
+    ang = hist[o:o + nth * nph].astype(np.int64).reshape(nth, nph); o += nth * nph
+    ms_out = hist[o:o + nms].astype(np.int64); o += nms
+    ms_in = hist[o:o + nms].astype(np.int64); o += nms
+    trans = hist[o:o + ntr * ntr].astype(np.int64).reshape(ntr, ntr); o += ntr * ntr
+    contact = hist[o:o + 16].astype(np.int64); o += 16
+    contact_steps = int(hist[o])
+    reacted = outcome == 1
+    escaped = outcome == 2
+    nm_pos, nm_q = tr[:, 9:12], tr[:, 12:16]
+    return {
+        "start_pos": tr[:, 0:3].copy(), "start_q": tr[:, 3:7].copy(),
+        "min_dist": tr[:, 7].copy(), "step_at_min": tr[:, 8].astype(np.int64),
+        "total_time_ps": tr[:, 16].copy(), "n_returns": tr[:, 17].astype(np.int64),
+        "bb_triggered": tr[:, 18].astype(np.int64),
+        "encounter_pos": tr[reacted, 19:22].copy(), "encounter_q": tr[reacted, 22:26].copy(),
+        "encounter_traj": np.where(reacted)[0].astype(np.int64),
+        "encounter_n_pairs": tr[reacted, 26].astype(np.int64),
+        "near_miss_pos": nm_pos[escaped].copy(), "near_miss_q": nm_q[escaped].copy(),
+        "near_miss_traj": np.where(escaped)[0].astype(np.int64),
+        "near_miss_dist": tr[escaped, 7].copy(),
+        "radial_bins": np.linspace(0, 1.2 * r_esc, nrad + 1), "radial_counts": rad,
+        "angular_theta": np.linspace(0, np.pi, nth), "angular_phi": np.linspace(0, 2 * np.pi, nph),
+        "angular_counts": ang,
+        "milestone_radii": np.linspace(r_b, r_esc, nms), "milestone_flux_out": ms_out,
+        "milestone_flux_in": ms_in,
+        "contact_pair_counts": contact[:int(P.n_pairs)] if P.n_pairs else contact[:1],
+        "contact_total_steps": contact_steps,
+        "trans_bins": np.linspace(0, 1.2 * r_esc, ntr + 1), "trans_matrix": trans,
+    }
+
+
 def autotune_sim(lib, ctx, h, rounds=2, steps_per_cand=1):
     """Runs pst_prod_autotune on a production sim handle; returns the table of
     candidates {G, cap, xcd, ms} and the chosen launch."""
@@ -189,7 +238,9 @@ class GPUBatchSimulator:
         self.audit_n = int(audit_n)
         self.audit_tol = float(audit_tol)
         self.graph_steps = int(graph_steps)
-        self.record = bool(record)
+        if record not in (False, True, "observables"):
+            raise ValueError("record must be False, True or 'observables'")
+        self.record = 2 if record == "observables" else int(bool(record))
         prec = precision if precision is not None else getattr(batch_engine, "precision", "fp64")
         if prec not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
@@ -399,7 +450,7 @@ class GPUBatchSimulator:
             N = int(P.n_total)
             n_r, n_e = int(c16[0]), int(c16[1])
             n_m = N - n_r - n_e
-            sim_data = {"n_returns": int(c16[5]), "n_overlap_rejected": int(c16[6]),
+            sim_data = {"n_returns_total": int(c16[5]), "n_overlap_rejected": int(c16[6]),
                         "n_bridge": int(c16[7]), "bd_steps": steps}
             if tuned is not None:
                 sim_data["autotune"] = tuned
@@ -412,6 +463,8 @@ class GPUBatchSimulator:
                 ctx.check(lib.pst_prod_outcomes(h, ptr(st), ptr(ns)), "pst_prod_outcomes")
                 sim_data["outcome"] = st
                 sim_data["n_steps"] = ns
+            if self.record == 2:
+                sim_data.update(observables(lib, ctx, h, P, sim_data["outcome"]))
             return self._result(N, n_r, n_e, n_m, c16[3], elapsed, sim_data)
         finally:
             lib.pst_sim_destroy(h)
@@ -554,8 +607,8 @@ class ProductionDriver:
         self.sim, self.P = sim, P
         self.lib, self.ctx = sim.engine.ctx.lib, sim.engine.ctx
         self.h = sim._make_sim(P)
-        self.record = bool(record)
-        self.ctx.check(self.lib.pst_prod_init(self.h, int(self.record)), "pst_prod_init")
+        self.record = 2 if record == "observables" else int(bool(record))
+        self.ctx.check(self.lib.pst_prod_init(self.h, self.record), "pst_prod_init")
         self.c16 = np.zeros(16)
         self.steps = 0
 
@@ -589,6 +642,11 @@ class ProductionDriver:
         self.ctx.check(self.lib.pst_sim_get_state(self.h, ptr(pos), ptr(q), ptr(st), ptr(ns)),
                        "pst_sim_get_state")
         return pos, q, st, ns
+
+    def observables(self):
+        """The reference's sim_data entries collected so far (record='observables')."""
+        st, _ = self.outcomes()
+        return observables(self.lib, self.ctx, self.h, self.P, st)
 
     def outcomes(self):
         """Per-trajectory (outcome, steps): 0 running, 1 reacted, 2 escaped, 3 max steps."""
