@@ -180,6 +180,11 @@ int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
  * [0, 1.2 r_esc)), per-pair contact counts at the pose a step starts from
  * (16 slots), trajectory-steps counted.  Either pointer may be NULL. */
 int pst_prod_obs_sizes(int* traj_doubles, int* hist_len);
+/* State-machine reaction counts (StepParams.sm_active, reactions r = pair r;
+ * gpu_batch_simulator.py:216-318, :818-854, :1185-1234): out48 = fires at the
+ * reaction check [16], fires in the Brownian bridge [16], distinct
+ * trajectories that fired each reaction [16]. */
+int pst_prod_sm_counts(pst_sim* sim, unsigned long long* out48);
 int pst_prod_observables(pst_sim* sim, double* traj, unsigned long long* hist);
 /* Mean device time per step (ms) of the last pst_prod_run, HIP events. */
 int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);

@@ -47,8 +47,31 @@ from .simulator import OracleSim
 INIT_STEP = 0xFFFFFFFF
 
 
+def sm_tables(reactions, first_state=None):
+    """State indices of the reference (gpu_batch_simulator.py:255-306) for
+    ``reactions`` = [(state_before, state_after)]: first_state (else the first
+    state_before) is 0, labels numbered in order of appearance, a label that
+    is no reaction's state_before is terminal."""
+    idx = {}
+
+    def get(label):
+        if label is None:
+            return -1
+        return idx.setdefault(label, len(idx))
+    first = first_state if first_state is not None else reactions[0][0]
+    get(first)
+    before = np.array([get(b) for b, _ in reactions])
+    after = np.array([get(a) for _, a in reactions])
+    terminal = np.array([i not in set(before.tolist()) for i in range(len(idx))])
+    return before, after, terminal
+
+
 class ProductionOracle(OracleSim):
-    """``system`` / ``params`` as OracleSim; hydro "scalar" or "rpy12"."""
+    """``system`` / ``params`` as OracleSim; hydro "scalar" or "rpy12".
+    ``system["sm_reactions"]`` = [(state_before, state_after)] (pair r is
+    reaction r's pair0) switches to state-machine reactions
+    (gpu_batch_simulator.py:818-854 check, :1014-1060 dt refinement,
+    :1185-1234 bridge)."""
 
     def __init__(self, system, params, torque_mode="reference", hydro="scalar",
                  kT_over_eta=1.0 / 0.243, observe=False):
@@ -74,6 +97,16 @@ class ProductionOracle(OracleSim):
         self.total_steps = 0
         self.n_returns = 0
         self.n_overlap_rejected = 0
+        self.sm = None
+        if system.get("sm_reactions"):
+            b, a, term = sm_tables(system["sm_reactions"], system.get("first_state"))
+            self.sm = {"before": b, "after": a, "terminal": term,
+                       "min_core_rxn_dt": float(params.get("minimum_core_reaction_dt", 0.0))}
+            self.sm_state = np.zeros(N, dtype=np.int64)
+            nr = len(b)
+            self.sm_fired = np.zeros(nr, dtype=np.int64)
+            self.sm_fired_bridge = np.zeros(nr, dtype=np.int64)
+            self.sm_sets = [set() for _ in range(nr)]
         if self.observe:
             # the reference's sim_data (gpu_batch_simulator.py:576-620,
             # :1255-1340), per trajectory and run-wide; step_at_min is the
@@ -134,15 +167,28 @@ class ProductionOracle(OracleSim):
         self._finish(run[mx], 3, ns[mx])
         run, ns = run[~mx], ns[~mx]
         P = len(self.cut)
-        if P and len(run):
+        fired = np.zeros(len(run), dtype=bool)
+        if P and len(run) and self.sm is not None:
+            R = quats_to_rotmats(self.q[run])
+            d = self.pair_dists(R, self.pos[run])
+            rx = self._sm_lowest(run, d < self.cut[None, :])
+            fired = rx >= 0
+            term = self._sm_fire(run[fired], rx[fired], bridge=False)
+            done = np.zeros(len(run), dtype=bool)
+            done[np.where(fired)[0][term]] = True
+            self._finish(run[done], 1, ns[done])
+            run, ns, fired = run[~done], ns[~done], fired[~done]
+        elif P and len(run):
             R = quats_to_rotmats(self.q[run])
             d = self.pair_dists(R, self.pos[run])
             re = (d < self.cut[None, :]).sum(axis=1) >= self.n_needed
             self._finish(run[re], 1, ns[re])
             run, ns = run[~re], ns[~re]
-        # escape / return
+            fired = np.zeros(len(run), dtype=bool)
+        # escape / return (a trajectory that fired a reaction this step is not
+        # tested, :879-884)
         r = np.linalg.norm(self.pos[run], axis=1)
-        at = r >= self.r_esc
+        at = (r >= self.r_esc) & ~fired
         if at.any():
             ia, na = run[at], ns[at]
             u, _ = PH.uniform2(self.seed, ia, na, PH.RNG_ESC, 0)
@@ -170,7 +216,10 @@ class ProductionOracle(OracleSim):
         Fo, To = self.forces(R, pos0)
         old_d = self.pair_dists(R, pos0) if P else None
         r = np.linalg.norm(pos0, axis=1)
-        dt = self.adaptive_dt(r)
+        if self.sm is not None:
+            dt = self._sm_dt(r, run, old_d)
+        else:
+            dt = self.adaptive_dt(r)
         if self.hydro == "rpy12":
             xi = np.empty((n, 12))
             for k in range(6):
@@ -218,11 +267,52 @@ class ProductionOracle(OracleSim):
             both = (x0 > 0) & (x1 > 0)
             with np.errstate(over="ignore"):
                 pc = np.where(both, np.exp(-x0 * x1 / Ddt), 0.0)
-            fired = (u < pc) | (x1 < 0.0)
-            bb = fired.sum(axis=1) >= self.n_needed
+            pf = (u < pc) | (x1 < 0.0)
+            if self.sm is not None:
+                rx = self._sm_lowest(run, pf)
+                f = rx >= 0
+                term = self._sm_fire(run[f], rx[f], bridge=True)
+                bb = np.zeros(n, dtype=bool)
+                bb[np.where(f)[0][term]] = True
+            else:
+                bb = pf.sum(axis=1) >= self.n_needed
             self._finish(run[bb], 1, ns[bb] + 1, bridge=1)
         self.n_steps[run] = ns + 1
         return True
+
+    # --- state-machine reactions ------------------------------------------
+    def _sm_lowest(self, run, mask):
+        """Lowest reaction r with mask[:, r] and state == before[r], else -1."""
+        ok = mask & (self.sm_state[run][:, None] == self.sm["before"][None, :])
+        return np.where(ok.any(axis=1), ok.argmax(axis=1), -1)
+
+    def _sm_fire(self, idx, rx, bridge):
+        """New states; counts; returns which moved to a terminal state."""
+        after = self.sm["after"][rx]
+        self.sm_state[idx] = after
+        for t, r in zip(idx.tolist(), rx.tolist()):
+            (self.sm_fired_bridge if bridge else self.sm_fired)[r] += 1
+            self.sm_sets[r].add(t)
+        return np.array([a >= 0 and bool(self.sm["terminal"][a]) for a in after], dtype=bool)
+
+    def _sm_dt(self, r, run, old_d):
+        """dt refined near a reachable reaction surface, before cap and floor."""
+        saved = (self.max_dt, self.min_core_dt)
+        self.max_dt = self.min_core_dt = 0.0
+        dt = self.adaptive_dt(r)
+        self.max_dt, self.min_core_dt = saved
+        reach = self.sm_state[run][:, None] == self.sm["before"][None, :]
+        coord = np.where(reach, np.maximum(old_d - self.cut[None, :], 0.0), 1e30).min(axis=1)
+        need = 12.0 * np.sqrt(2.0 * self.D_t * dt) > coord
+        crit = coord * coord / (288.0 * self.D_t)
+        if self.sm["min_core_rxn_dt"] > 0:
+            crit = np.maximum(crit, self.sm["min_core_rxn_dt"])
+        dt = np.where(need, np.minimum(dt, crit), dt)
+        if self.max_dt > 0:
+            dt = np.minimum(dt, self.max_dt)
+        if self.min_core_dt > 0:
+            dt = np.maximum(dt, self.min_core_dt)
+        return dt
 
     def _observe_move(self, run, ns, newp, newq, dt, old_d):
         t = self.obs_traj

@@ -42,6 +42,11 @@ class StepParams(C.Structure):
         ("a_rec", C.c_double), ("a_lig", C.c_double), ("kT_over_eta", C.c_double),
         ("surrogate_r", C.c_double),
         ("stop_done", C.c_int64),
+        ("sm_active", C.c_int),
+        ("sm_before", C.c_int * MAX_PAIRS),
+        ("sm_after", C.c_int * MAX_PAIRS),
+        ("sm_terminal", C.c_int),
+        ("min_core_rxn_dt", C.c_double),
     ]
 
 
@@ -84,6 +89,7 @@ _SIGS = {
     "pst_prod_run": ([_vp, C.c_longlong, C.c_int, _vp], C.c_int),
     "pst_prod_outcomes": ([_vp, _vp, _vp], C.c_int),
     "pst_prod_obs_sizes": ([_ip, _ip], C.c_int),
+    "pst_prod_sm_counts": ([_vp, _vp], C.c_int),
     "pst_prod_observables": ([_vp, _vp, _vp], C.c_int),
     "pst_prod_autotune": ([_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp], C.c_int),
     "pst_prod_set_launch": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),

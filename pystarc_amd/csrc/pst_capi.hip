@@ -94,6 +94,8 @@ struct pst_sim {
   signed char* out_status = nullptr;
   int* out_steps = nullptr;
   ObsBufs obs{};                     // observables (record level 2)
+  int* sm_state = nullptr;           // state-machine reactions
+  unsigned int* sm_mask = nullptr;
   std::vector<void*> obs_allocs;
   double last_ms_per_step = 0;
   bool prod_ready = false;
@@ -719,6 +721,11 @@ int pst_sim_create(pst_ctx* ctx, const void* params, int precision, pst_sim** ou
   if (P->hydro_mode != 0 && P->hydro_mode != 1) FAIL("pst_sim_create: hydro_mode must be 0 or 1");
   if (P->hydro_mode == 1 && !(P->a_rec > 0 && P->a_lig > 0 && P->kT_over_eta > 0))
     FAIL("pst_sim_create: hydro_mode 1 needs a_rec, a_lig and kT_over_eta > 0");
+  if (P->sm_active) {
+    for (int r = 0; r < P->n_pairs; ++r)
+      if (P->sm_before[r] < 0 || P->sm_before[r] >= 32 || P->sm_after[r] < -1 || P->sm_after[r] >= 32)
+        FAIL("pst_sim_create: state-machine state indices must be in [0, 32)");
+  }
   CK(set_dev(ctx));
   pst_sim* s = new pst_sim();
   s->ctx = ctx;
@@ -748,7 +755,7 @@ int pst_sim_destroy(pst_sim* s) {
   void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
                   s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
-                  s->audit_F, s->key, s->bins, s->census};
+                  s->audit_F, s->key, s->bins, s->census, s->sm_state, s->sm_mask};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (void* p : s->obs_allocs) hipFree(p);
@@ -837,6 +844,7 @@ int pst_replay_begin(pst_sim* s, int* n_run, int* n_esc, int* n_react) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
   if (s->P.hydro_mode != 0) FAIL("pst_replay_begin: replay follows the scalar diffusion (hydro_mode 0)");
+  if (s->P.sm_active) FAIL("pst_replay_begin: state-machine reactions run in production mode");
   CK(set_dev(ctx));
   if (ensure_replay(s)) return 2;
   ReplayBufs B = replay_bufs(s);
@@ -971,6 +979,8 @@ static ProdBufs prod_bufs(pst_sim* s) {
   B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
   B.ctr = s->ctr; B.out_status = s->out_status; B.out_steps = s->out_steps;
   B.obs = s->obs;
+  B.sm_state = s->sm_state;
+  B.sm_mask = s->sm_mask;
   B.n_slots = s->S;
   return B;
 }
@@ -994,6 +1004,8 @@ int pst_prod_init(pst_sim* s, int record) {
     CK(hipMalloc(&s->ctr, sizeof(RunCounters)));
     CK(hipMalloc(&s->key, S * 4));
     CK(hipMalloc(&s->bins, 512 * 4));
+    CK(hipMalloc(&s->sm_state, S * 4));
+    CK(hipMalloc(&s->sm_mask, S * 4));
   }
   CK(hipMemsetAsync(s->bins, 0, 512 * 4, ctx->stream));
   if (s->out_status) { hipFree(s->out_status); s->out_status = nullptr; }
@@ -1314,6 +1326,22 @@ int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   CK(set_dev(ctx));
   s->tune_mode = mode; s->tune_G = G; s->tune_cap = blocks_cap; s->tune_xcd = xcd ? 1 : 0;
   return s->prod_ready ? sim_ensure_bins(s) : 0;
+}
+
+int pst_prod_sm_counts(pst_sim* s, unsigned long long* out48) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready || !out48) FAIL("pst_prod_sm_counts: call pst_prod_init first");
+  CK(set_dev(ctx));
+  RunCounters h;
+  CK(hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  for (int r = 0; r < PST_MAX_PAIRS; ++r) {
+    out48[r] = h.sm_fired[r];
+    out48[PST_MAX_PAIRS + r] = h.sm_fired_bridge[r];
+    out48[2 * PST_MAX_PAIRS + r] = h.sm_unique[r];
+  }
+  return 0;
 }
 
 int pst_prod_obs_sizes(int* traj_doubles, int* hist_len) {

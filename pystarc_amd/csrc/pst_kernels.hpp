@@ -97,6 +97,8 @@ struct ProdBufs {
   signed char* out_status;     // per trajectory id (nullable)
   int* out_steps;              // per trajectory id (nullable)
   ObsBufs obs;                 // observables (obs.traj null: off)
+  int* sm_state;               // per-slot reaction state (state-machine mode)
+  unsigned int* sm_mask;       // per-slot reactions fired so far (bit r)
   int n_slots;
 };
 hipError_t prod_flush_observables(const StepParams& P, const ProdBufs& B, hipStream_t st);

@@ -72,6 +72,44 @@ __device__ inline double adaptive_dt(const StepParams& P, double r) {
   return dt;
 }
 
+// State-machine mode: the same dt, refined near a reachable reaction surface
+// before the cap and floor (:1014-1067): rc = min over reactions reachable
+// from the current state of max(d_r - cutoff_r, 0); when 12 sqrt(2 D_t dt)
+// exceeds rc, dt <- min(dt, max(rc^2 / (288 D_t), min_core_rxn_dt)).
+__device__ inline double adaptive_dt_sm(const StepParams& P, double r, int state,
+                                        const double* R, const double* pos) {
+  StepParams Q = P;
+  Q.max_dt = 0;
+  Q.min_core_dt = 0;
+  double dt = adaptive_dt(Q, r);
+  double d[PST_MAX_PAIRS];
+  pair_dists(P, R, pos[0], pos[1], pos[2], d);
+  double rc = 1.0e30;
+  for (int p = 0; p < P.n_pairs; ++p)
+    if (P.sm_before[p] == state) rc = fmin(rc, fmax(d[p] - P.cutoff[p], 0.0));
+  const double sig = sqrt(2.0 * P.D_t * dt);
+  if (12.0 * sig > rc) {
+    double crit = rc * rc / (288.0 * P.D_t);
+    if (P.min_core_rxn_dt > 0) crit = fmax(crit, P.min_core_rxn_dt);
+    dt = fmin(dt, crit);
+  }
+  if (P.max_dt > 0) dt = fmin(dt, P.max_dt);
+  if (P.min_core_dt > 0) dt = fmax(dt, P.min_core_dt);
+  return dt;
+}
+
+// State-machine reaction check (:1683-1735): the lowest-index reaction whose
+// source state is the trajectory's and whose pair is within its cutoff.
+__device__ inline int sm_check(const StepParams& P, const double* pos, const double* q,
+                               int state) {
+  double R[9], d[PST_MAX_PAIRS];
+  quat_to_rot(q, R);
+  pair_dists(P, R, pos[0], pos[1], pos[2], d);
+  for (int p = 0; p < P.n_pairs; ++p)
+    if (P.sm_before[p] == state && d[p] < P.cutoff[p]) return p;
+  return -1;
+}
+
 // Scalar relative diffusion, isotropic D_parallel with HI (:1074-1084).
 __device__ inline double diffusion(const StepParams& P, double r) {
   if (!P.use_hi) return P.D_t;
@@ -394,6 +432,10 @@ __device__ inline void start_traj(const StepParams& P, const ProdBufs& B, int s,
   B.traj[s] = t;
   B.n_steps[s] = 0;
   B.status[s] = PST_RUNNING;
+  if (P.sm_active) {
+    B.sm_state[s] = 0;
+    B.sm_mask[s] = 0u;
+  }
   if (B.obs.traj) {
     const ObsBufs& O = B.obs;
     double* rec = O.traj + (size_t)t * kObsTraj;
@@ -449,6 +491,22 @@ __device__ inline void finish_traj(const StepParams& P, const ProdBufs& B, int s
   }
 }
 
+// Fire reaction r on slot s: new state, counts; true when the new state is terminal.
+__device__ inline bool sm_fire(const StepParams& P, const ProdBufs& B, int s, int r, bool bridge) {
+  const int after = P.sm_after[r];
+  B.sm_state[s] = after;
+  if (bridge)
+    atomicAdd(&B.ctr->sm_fired_bridge[r], 1ull);
+  else
+    atomicAdd(&B.ctr->sm_fired[r], 1ull);
+  const unsigned int bit = 1u << r;
+  if (!(B.sm_mask[s] & bit)) {
+    B.sm_mask[s] |= bit;
+    atomicAdd(&B.ctr->sm_unique[r], 1ull);
+  }
+  return after >= 0 && ((P.sm_terminal >> after) & 1);
+}
+
 __global__ void k_prod_init(StepParams P, ProdBufs B) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B.n_slots) return;
@@ -500,13 +558,24 @@ __global__ void k_prod_pre(StepParams P, ProdBufs B, int par) {
         if (B.status[s] != PST_RUNNING) break;
         continue;
       }
-      if (reacted_at(P, pos, q)) {
+      bool fired = false;
+      if (P.sm_active) {
+        const int rx = sm_check(P, pos, q, B.sm_state[s]);
+        if (rx >= 0) {
+          fired = true;
+          if (sm_fire(P, B, s, rx, false)) {
+            finish_traj(P, B, s, PST_REACTED);
+            if (B.status[s] != PST_RUNNING) break;
+            continue;
+          }
+        }
+      } else if (reacted_at(P, pos, q)) {
         finish_traj(P, B, s, PST_REACTED);
         if (B.status[s] != PST_RUNNING) break;
         continue;
       }
       const double r = norm3(pos[0], pos[1], pos[2]);
-      if (r >= P.r_esc) {
+      if (r >= P.r_esc && !fired) {
         double u, unused;
         philox_uniform2(P.seed, (uint64_t)t, ns, RNG_ESC, 0, u, unused);
         if (u < P.return_prob) {
@@ -631,7 +700,14 @@ __global__ void __launch_bounds__(kStepBlock) k_prod_post(StepParams P, ForceSys
     const long long t = B.traj[s];
     const unsigned int ns = B.n_steps[s];
     const double r = norm3(pos[0], pos[1], pos[2]);
-    const double dt = adaptive_dt(P, r);
+    double dt;
+    if (P.sm_active) {
+      double R[9];
+      quat_to_rot(q, R);
+      dt = adaptive_dt_sm(P, r, B.sm_state[s], R, pos);
+    } else {
+      dt = adaptive_dt(P, r);
+    }
     const double D = diffusion(P, r);
     B.dt[s] = dt;
     bool rej;
@@ -732,9 +808,27 @@ __global__ void k_prod_bridge(StepParams P, ProdBufs B, int par) {
       u[p] = a;
       if (p + 1 < PST_MAX_PAIRS) u[p + 1] = b;
     }
-    const int n = bridge_count(P, B.x0 + (size_t)s * PST_MAX_PAIRS,
-                               B.x1 + (size_t)s * PST_MAX_PAIRS, B.dt[s], u);
-    react = n >= P.n_needed;
+    const double* x0 = B.x0 + (size_t)s * PST_MAX_PAIRS;
+    const double* x1 = B.x1 + (size_t)s * PST_MAX_PAIRS;
+    if (P.sm_active) {
+      // fired pairs gated by the current state, lowest index (:1185-1234)
+      const int state = B.sm_state[s];
+      for (int p = 0; p < P.n_pairs; ++p) {
+        if (P.sm_before[p] != state) continue;
+        double pc = 0.0;
+        if (x0[p] > 0 && x1[p] > 0) {
+          const double Ddt = fmax((P.has_bb_deff ? P.bb_deff[p] : P.D_t) * B.dt[s], 1e-30);
+          pc = exp(-x0[p] * x1[p] / Ddt);
+        }
+        if (u[p] < pc || x1[p] < 0.0) {
+          react = sm_fire(P, B, s, p, true);
+          break;
+        }
+      }
+    } else {
+      const int n = bridge_count(P, x0, x1, B.dt[s], u);
+      react = n >= P.n_needed;
+    }
   }
   B.n_steps[s] = ns + 1;
   if (react) {

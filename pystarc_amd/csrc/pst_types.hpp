@@ -114,6 +114,14 @@ struct StepParams {
   double a_rec, a_lig, kT_over_eta;
   double surrogate_r;    // centre distance above which the surrogate is used (<=0: off)
   int64_t stop_done;     // early stop when done >= this (int(0.995 N))
+  // state-machine reactions: reaction r is pair r (its pair0), fires only from
+  // state sm_before[r], moves the trajectory to sm_after[r]; a state whose bit
+  // is set in sm_terminal ends the trajectory as reacted
+  int sm_active;
+  int sm_before[PST_MAX_PAIRS];
+  int sm_after[PST_MAX_PAIRS];
+  int sm_terminal;       // bit mask over state indices (< 32 states)
+  double min_core_rxn_dt;
 };
 
 // Device-side run counters (production mode).
@@ -134,4 +142,9 @@ struct RunCounters {
   int pad;
   double audit_err2;
   double audit_ref2;
+  // state-machine fire counts per reaction: at the reaction check, in the
+  // Brownian bridge, and distinct trajectories that fired it
+  unsigned long long sm_fired[PST_MAX_PAIRS];
+  unsigned long long sm_fired_bridge[PST_MAX_PAIRS];
+  unsigned long long sm_unique[PST_MAX_PAIRS];
 };

@@ -99,9 +99,24 @@ def parse_rxns_xml_reaction_groups(path) -> List[ReactionGroup]:
     return groups
 
 
-def pathway_set_from_rxns(path) -> PathwaySet:
-    """One flattened reaction the batch step reads (reactions[0].criteria)."""
-    pairs, n_needed = parse_rxns_xml_criteria(path)
-    crit = ReactionCriteria(name="rxns", pairs=[ContactPair(p.rec_index, p.lig_index, p.cutoff)
-                                                for p in pairs], n_needed=n_needed)
-    return PathwaySet([ReactionInterface("rxns", crit)])
+def pathway_set_from_rxns(path, states=False) -> PathwaySet:
+    """states=False: one flattened reaction (all pairs, n_needed), the batch
+    step's default criterion.  states=True: one reaction per <reaction> with
+    its state_before / state_after labels and first_state, which switches
+    the batch step to state-machine reactions."""
+    if not states:
+        pairs, n_needed = parse_rxns_xml_criteria(path)
+        crit = ReactionCriteria(name="rxns", pairs=[ContactPair(p.rec_index, p.lig_index,
+                                                                p.cutoff) for p in pairs],
+                                n_needed=n_needed)
+        return PathwaySet([ReactionInterface("rxns", crit)])
+    root = ET.parse(str(Path(path))).getroot()
+    fs = root.find("first_state")
+    first = fs.text.strip() if fs is not None and fs.text else None
+    rxns = []
+    for g in parse_rxns_xml_reaction_groups(path):
+        crit = ReactionCriteria(name=g.name, pairs=[ContactPair(p.rec_index, p.lig_index, p.cutoff)
+                                                    for p in g.pairs], n_needed=g.n_needed)
+        rxns.append(ReactionInterface(g.name, crit, state_before=g.state_before,
+                                      state_after=g.state_after))
+    return PathwaySet(rxns, first_state=first)

@@ -11,6 +11,7 @@ from pathlib import Path
 
 
 def results_dict(result, D_rel: float, k_b: float, confidence: float = 0.95) -> dict:
+    """The results.json mapping; state-machine runs add completed_reactions."""
     k_on = result.rate_constant(D_rel, k_b)
     k_lo, k_hi = result.rate_constant_ci(D_rel, k_b, confidence)
     p_lo, p_hi = result.reaction_probability_ci(confidence)
@@ -24,6 +25,9 @@ def results_dict(result, D_rel: float, k_b: float, confidence: float = 0.95) -> 
          "confidence_level": confidence}
     if k_on > 0:
         d["log10_k_on"] = math.log10(k_on)
+    sd = getattr(result, "sim_data", None) or {}
+    if "completed_reactions" in sd:
+        d["completed_reactions"] = sd["completed_reactions"]
     return d
 
 

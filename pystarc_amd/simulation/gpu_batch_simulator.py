@@ -146,6 +146,55 @@ class GPUBatchResult:
         return "\n".join(lines)
 
 
+def state_machine(pathway_set):
+    """State-machine tables of the reference (gpu_batch_simulator.py:216-306):
+    reactions with a pair, their state labels; first_state (or the first
+    reaction's state_before) is state 0, other labels are numbered in order
+    of appearance; a state that is no reaction's state_before is terminal.
+    None when no reaction carries a state_before label."""
+    rxns = [r for r in pathway_set.reactions if r.criteria.pairs]
+    if not any(getattr(r, "state_before", None) is not None for r in pathway_set.reactions):
+        return None
+    if not rxns:
+        return None
+    if len(rxns) > MAX_PAIRS:
+        raise ValueError(f"at most {MAX_PAIRS} reactions are supported")
+    first = getattr(pathway_set, "first_state", None) or rxns[0].state_before
+    idx, labels = {}, []
+
+    def get(label):
+        if label is None:
+            return -1
+        if label not in idx:
+            idx[label] = len(labels)
+            labels.append(label)
+        return idx[label]
+    if first is not None:
+        get(first)
+    before = [get(r.state_before) for r in rxns]
+    after = [get(r.state_after) for r in rxns]
+    if len(labels) > 32:
+        raise ValueError("at most 32 reaction states are supported")
+    sources = set(before)
+    return {"names": [r.name for r in rxns], "before": before, "after": after,
+            "labels": labels, "terminal": [i not in sources for i in range(len(labels))],
+            "before_labels": [r.state_before for r in rxns],
+            "after_labels": [r.state_after for r in rxns]}
+
+
+def sm_report(lib, ctx, h, sm):
+    """Per-reaction fire counts as the reference reports them (:1558-1581)."""
+    c = np.zeros(3 * MAX_PAIRS, dtype=np.uint64)
+    ctx.check(lib.pst_prod_sm_counts(h, ptr(c)), "pst_prod_sm_counts")
+    rows = []
+    for r, name in enumerate(sm["names"]):
+        d, b, u = int(c[r]), int(c[MAX_PAIRS + r]), int(c[2 * MAX_PAIRS + r])
+        rows.append({"name": name, "n_fired": d + b, "state_before": sm["before_labels"][r],
+                     "state_after": sm["after_labels"][r], "n_fired_discrete": d,
+                     "n_fired_bridge": b, "unique_trajectories": u})
+    return {"completed_reactions": rows}
+
+
 def observables(lib, ctx, h, P, outcome):
     """sim_data entries of the reference (gpu_batch_simulator.py:1524-1557)
     from the device observables of a record-level-2 run."""
@@ -251,10 +300,18 @@ class GPUBatchSimulator:
         self._N_lig = len(self._charges)
         self._use_hi = bool(getattr(params, "hydrodynamic_interactions", False))
         self._qb_factor = QB_FACTOR
-        # flattened reaction pairs (:186-205)
+        # state-machine reactions (:216-318): one pair (pair0) per reaction
+        self._sm = state_machine(pathway_set)
+        if self._sm is not None and rng != "philox":
+            raise ValueError("state-machine reactions run in philox mode")
+        # flattened reaction pairs (:186-205); in state-machine mode pair r is
+        # reaction r's pair0
         rec_idx, lig_idx, cut = [], [], []
         for rxn in pathway_set.reactions:
-            for pair in rxn.criteria.pairs:
+            pairs = rxn.criteria.pairs
+            if self._sm is not None:
+                pairs = pairs[:1]
+            for pair in pairs:
                 rec_idx.append(pair.mol1_atom_index)
                 lig_idx.append(pair.mol2_atom_index)
                 cut.append(pair.distance_cutoff)
@@ -267,8 +324,8 @@ class GPUBatchSimulator:
             self._rxn_n_needed = len(cut) if (nn < 0 or nn > len(cut)) else nn
         else:
             self._rxn_n_needed = 1
-        if any(getattr(r, "state_before", None) is not None for r in pathway_set.reactions):
-            raise NotImplementedError("state-machine reactions are not part of this build")
+        if self._sm is not None:
+            self._rxn_n_needed = 1
         rec_pos = np.asarray(mol1.positions_array(), dtype=np.float64)
         self._rec_gho_pos = rec_pos[rec_idx] if rec_idx else None
         N = int(params.n_trajectories)
@@ -374,6 +431,14 @@ class GPUBatchSimulator:
         P.kT_over_eta = 1.0 / MU_REF
         P.surrogate_r = -1.0
         P.stop_done = int(N * 0.995)
+        if self._sm is not None:
+            sm = self._sm
+            P.sm_active = 1
+            for r, (b, a) in enumerate(zip(sm["before"], sm["after"])):
+                P.sm_before[r] = b
+                P.sm_after[r] = a
+            P.sm_terminal = sum(1 << i for i, t in enumerate(sm["terminal"]) if t)
+            P.min_core_rxn_dt = float(getattr(p, "minimum_core_reaction_dt", 0.0) or 0.0)
         return P
 
     def _make_sim(self, P):
@@ -465,6 +530,8 @@ class GPUBatchSimulator:
                 sim_data["n_steps"] = ns
             if self.record == 2:
                 sim_data.update(observables(lib, ctx, h, P, sim_data["outcome"]))
+            if self._sm is not None:
+                sim_data.update(sm_report(lib, ctx, h, self._sm))
             return self._result(N, n_r, n_e, n_m, c16[3], elapsed, sim_data)
         finally:
             lib.pst_sim_destroy(h)
@@ -642,6 +709,12 @@ class ProductionDriver:
         self.ctx.check(self.lib.pst_sim_get_state(self.h, ptr(pos), ptr(q), ptr(st), ptr(ns)),
                        "pst_sim_get_state")
         return pos, q, st, ns
+
+    def completed_reactions(self):
+        """State-machine fire counts so far (state-machine pathway sets)."""
+        if self.sim._sm is None:
+            return None
+        return sm_report(self.lib, self.ctx, self.h, self.sim._sm)["completed_reactions"]
 
     def observables(self):
         """The reference's sim_data entries collected so far (record='observables')."""

@@ -18,7 +18,8 @@ from oracle.production import ProductionOracle  # noqa: E402
 from pystarc_amd import systems as S  # noqa: E402
 from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchSimulator  # noqa: E402
 from tests._systems import (dx_grids, numpy_grids, oracle_params, oracle_system,  # noqa: E402
-                            small_protein_ligand, small_protein_protein, two_spheres_small)
+                            small_protein_ligand, small_protein_protein,
+                            state_machine_protein_protein, two_spheres_small)
 
 
 def _case(name):
@@ -32,11 +33,13 @@ def _case(name):
         return small_protein_protein(n_traj=128), "rpy12", "physical", 150
     if name == "protein_protein_rpy12_zero_torque":
         return small_protein_protein(n_traj=128, seed=10), "rpy12", "reference", 150
+    if name == "state_machine":
+        return state_machine_protein_protein(n_traj=128), "scalar", "reference", 300
     raise KeyError(name)
 
 
 @pytest.mark.parametrize("name", ["two_spheres", "protein_ligand_hi", "protein_protein_rpy12",
-                                  "protein_protein_rpy12_zero_torque"])
+                                  "protein_protein_rpy12_zero_torque", "state_machine"])
 def test_production_matches_oracle_per_trajectory(name):
     sysm, hydro, torque, K = _case(name)
     N = int(sysm.params.n_trajectories)
@@ -50,6 +53,7 @@ def test_production_matches_oracle_per_trajectory(name):
         pos, q, st, ns = drv.state()
         out, steps = drv.outcomes()
         ctr = drv.counters()
+        sm = drv.completed_reactions()
     finally:
         drv.close()
     # chunk=0: reverse Born evaluated for every trajectory.  The production
@@ -79,3 +83,9 @@ def test_production_matches_oracle_per_trajectory(name):
     assert ctr["total_steps"] == orc.total_steps
     assert ctr["n_reacted"] == int((orc.status == 1).sum())
     assert ctr["n_escaped"] == int((orc.status == 2).sum())
+    if orc.sm is not None:
+        print("state machine:", sm)
+        assert [r["n_fired_discrete"] for r in sm] == orc.sm_fired.tolist()
+        assert [r["n_fired_bridge"] for r in sm] == orc.sm_fired_bridge.tolist()
+        assert [r["unique_trajectories"] for r in sm] == [len(x) for x in orc.sm_sets]
+        assert sum(r["n_fired"] for r in sm) > 0
