@@ -53,6 +53,11 @@ int pst_struct_size(int which);
 int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int nx, int ny, int nz,
                  const double* origin3, const double* spacing3);
 int pst_clear_grids(pst_ctx* ctx);
+/* dst uses src's receptor potential / Born grids (and their float2 pair)
+ * without a copy: one resident set for many ligand contexts.  src must
+ * outlive dst.  No reference counterpart (the reference builds one engine per
+ * run, gpu_batch_engine.py:139-246). */
+int pst_share_receptor_grids(pst_ctx* dst, const pst_ctx* src);
 /* Yukawa far-field (gpu_batch_engine.py:161-175): dipole3 / quad9 may be NULL
  * (no multipole expansion, monopole only). */
 int pst_set_farfield(pst_ctx* ctx, double receptor_charge, double debye_length, double sdie,
@@ -180,6 +185,15 @@ int pst_prod_outcomes(pst_sim* sim, signed char* status, int* steps);
  * [0, 1.2 r_esc)), per-pair contact counts at the pose a step starts from
  * (16 slots), trajectory-steps counted.  Either pointer may be NULL. */
 int pst_prod_obs_sizes(int* traj_doubles, int* hist_len);
+/* Several simulators (one ligand each; receptor grids shared with
+ * pst_share_receptor_grids; same precision) stepped in lock-step: per step
+ * each simulator's pre phases, ONE force launch over all grid-path lists
+ * (workgroups partitioned by simulator), each simulator's post phases;
+ * graph_steps per captured hipGraph.  c16 = (L,16) counters as
+ * pst_prod_run; ms_per_step = device time per group step.  No reference
+ * counterpart (the reference runs one ligand per process, multi_GPU_runs.py). */
+int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, double* c16,
+                  double* ms_per_step);
 /* State-machine reaction counts (StepParams.sm_active, reactions r = pair r;
  * gpu_batch_simulator.py:216-318, :818-854, :1185-1234): out48 = fires at the
  * reaction check [16], fires in the Brownian bridge [16], distinct

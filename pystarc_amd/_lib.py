@@ -62,6 +62,8 @@ _SIGS = {
     "pst_last_error": ([_vp], C.c_char_p),
     "pst_add_grid": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "pst_clear_grids": ([_vp], C.c_int),
+    "pst_share_receptor_grids": ([_vp, _vp], C.c_int),
+    "pst_group_run": ([_vp, C.c_int, C.c_longlong, C.c_int, _vp, _dp], C.c_int),
     "pst_set_farfield": ([_vp, C.c_double, C.c_double, C.c_double, _vp, _vp], C.c_int),
     "pst_set_alpha": ([_vp, C.c_double], C.c_int),
     "pst_set_receptor": ([_vp, C.c_int, _vp, _vp], C.c_int),

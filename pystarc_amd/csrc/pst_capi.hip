@@ -7,6 +7,7 @@
 
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/pystarc_hip.h"
 #include "pst_kernels.hpp"
@@ -20,6 +21,7 @@ namespace {
 struct DevGrid {
   GridDev g;
   void* mem = nullptr;
+  bool owned = true;   // false: device memory of another context's grid
 };
 
 }  // namespace
@@ -234,7 +236,7 @@ int pst_ctx_create(pst_ctx** out, int device, void* stream, int use_stream) {
 
 static void free_grids(std::vector<DevGrid>& v) {
   for (auto& g : v)
-    if (g.mem) hipFree(g.mem);
+    if (g.mem && g.owned) hipFree(g.mem);
   v.clear();
 }
 
@@ -245,7 +247,7 @@ int pst_ctx_destroy(pst_ctx* ctx) {
   free_grids(ctx->elec);
   free_grids(ctx->born);
   free_grids(ctx->lig_born);
-  if (ctx->pair.mem) hipFree(ctx->pair.mem);
+  if (ctx->pair.mem && ctx->pair.owned) hipFree(ctx->pair.mem);
   if (ctx->d_body) hipFree(ctx->d_body);
   if (ctx->d_q) hipFree(ctx->d_q);
   if (ctx->d_rec) hipFree(ctx->d_rec);
@@ -263,8 +265,9 @@ const char* pst_last_error(const pst_ctx* ctx) { return ctx ? ctx->err.c_str() :
 // when there is exactly one of each, both FP32, with identical geometry.
 static int update_pair(pst_ctx* ctx) {
   if (ctx->pair.mem) {
-    hipFree(ctx->pair.mem);
+    if (ctx->pair.owned) hipFree(ctx->pair.mem);
     ctx->pair.mem = nullptr;
+    ctx->pair.owned = true;
   }
   if (ctx->elec.size() != 1 || ctx->born.size() != 1) return 0;
   const GridDev& a = ctx->elec[0].g;
@@ -317,7 +320,24 @@ int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int n
     d.g.hi[i] = origin3[i] + (n[i] - 1 - 3.0) * spacing3[i];
   }
   dst.push_back(d);
-  return update_pair(ctx);
+  // the float2 pair depends on the receptor grids only
+  return kind == 2 ? 0 : update_pair(ctx);
+}
+
+int pst_share_receptor_grids(pst_ctx* dst, const pst_ctx* src) {
+  if (!dst) return 1;
+  pst_ctx* ctx = dst;
+  if (!src) FAIL("pst_share_receptor_grids: null source context");
+  if (src->device != dst->device) FAIL("pst_share_receptor_grids: contexts on different devices");
+  if (!dst->elec.empty() || !dst->born.empty())
+    FAIL("pst_share_receptor_grids: the destination already has receptor grids");
+  CK(set_dev(ctx));
+  for (const auto& g : src->elec) { DevGrid d = g; d.owned = false; dst->elec.push_back(d); }
+  for (const auto& g : src->born) { DevGrid d = g; d.owned = false; dst->born.push_back(d); }
+  if (dst->pair.mem && dst->pair.owned) hipFree(dst->pair.mem);
+  dst->pair = src->pair;
+  dst->pair.owned = false;
+  return 0;
 }
 
 int pst_clear_grids(pst_ctx* ctx) {
@@ -1053,11 +1073,8 @@ int pst_prod_init(pst_sim* s, int record) {
   return sim_ensure_bins(s);
 }
 
-static int enqueue_step(pst_sim* s, int par) {
-  pst_ctx* ctx = s->ctx;
-  ProdBufs B = prod_bufs(s);
-  CK(prod_step_pre(s->P, B, par, ctx->stream));
-  CK(prod_step_sort(B, ctx->stream));
+// The main force list of a production step: still-running grid-path slots.
+static ForceWork main_work(pst_sim* s) {
   ForceWork W;
   memset(&W, 0, sizeof(W));
   W.list = s->plain_list;
@@ -1067,14 +1084,24 @@ static int enqueue_step(pst_sim* s, int par) {
   W.quat = s->quat;
   W.F = s->F; W.T = s->T; W.E = nullptr;
   W.rev_skip_mode = 0;
-  ForceSys S;
-  fill_sys(ctx, S);
-  const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
-  if (s->census_on) CK(launch_census(S, W, s->census, ctx->stream));
-  if (s->tev0) CK(hipEventRecord(s->tev0, ctx->stream));
-  const int mode = s->tune_mode >= 0 ? s->tune_mode : ctx->gather_mode;
-  CK(launch_force_eval(ctx, S, W, s->precision, G, s->S, s->tune_cap, mode));
-  if (s->tev1) CK(hipEventRecord(s->tev1, ctx->stream));
+  return W;
+}
+
+// Step phases before the force evaluation: reaction / escape / refill and
+// the grid-path list sorted by direction.
+static int enqueue_pre(pst_sim* s, int par) {
+  pst_ctx* ctx = s->ctx;
+  ProdBufs B = prod_bufs(s);
+  CK(prod_step_pre(s->P, B, par, ctx->stream));
+  CK(prod_step_sort(B, ctx->stream));
+  return 0;
+}
+
+// Step phases after the grid-path forces: surrogate forces and their audit,
+// the move, the Brownian bridge.
+static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par) {
+  pst_ctx* ctx = s->ctx;
+  ProdBufs B = prod_bufs(s);
   if (s->mlp && s->P.surrogate_r > 0) {
     CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
                  ctx->stream));
@@ -1096,6 +1123,22 @@ static int enqueue_step(pst_sim* s, int par) {
   CK(prod_step_post(s->P, S, B, par, ctx->stream));
   CK(prod_step_bridge(s->P, B, par, ctx->stream));
   return 0;
+}
+
+static int enqueue_step(pst_sim* s, int par) {
+  pst_ctx* ctx = s->ctx;
+  int rc = enqueue_pre(s, par);
+  if (rc) return rc;
+  ForceWork W = main_work(s);
+  ForceSys S;
+  fill_sys(ctx, S);
+  const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
+  if (s->census_on) CK(launch_census(S, W, s->census, ctx->stream));
+  if (s->tev0) CK(hipEventRecord(s->tev0, ctx->stream));
+  const int mode = s->tune_mode >= 0 ? s->tune_mode : ctx->gather_mode;
+  CK(launch_force_eval(ctx, S, W, s->precision, G, s->S, s->tune_cap, mode));
+  if (s->tev1) CK(hipEventRecord(s->tev1, ctx->stream));
+  return enqueue_post(s, S, W, par);
 }
 
 int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) {
@@ -1326,6 +1369,170 @@ int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   CK(set_dev(ctx));
   s->tune_mode = mode; s->tune_G = G; s->tune_cap = blocks_cap; s->tune_xcd = xcd ? 1 : 0;
   return s->prod_ready ? sim_ensure_bins(s) : 0;
+}
+
+// Several simulators (one ligand each, receptor grids shared) stepped in
+// lock-step: per step every simulator's pre phases, ONE force launch over
+// all of their grid-path lists, then every simulator's post phases; captured
+// graph_steps at a time in a hipGraph.  Stops when every simulator has
+// stopped or has no live trajectory.  c16 = (L,16) counters as pst_prod_run.
+int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, double* c16,
+                  double* ms_per_step) {
+  if (!sims || L <= 0 || !sims[0]) return 1;
+  pst_ctx* ctx = sims[0]->ctx;
+  if (L > 64) FAIL("pst_group_run: at most 64 simulators");
+  if (max_steps < 0 || graph_steps < 0) FAIL("pst_group_run: bad arguments");
+  for (int l = 0; l < L; ++l) {
+    if (!sims[l] || !sims[l]->prod_ready) FAIL("pst_group_run: every simulator needs pst_prod_init");
+    if (sims[l]->precision != sims[0]->precision) FAIL("pst_group_run: mixed precisions");
+    if (sims[l]->ctx->device != ctx->device) FAIL("pst_group_run: simulators on different devices");
+  }
+  CK(set_dev(ctx));
+  const hipStream_t st = ctx->stream;
+  std::vector<hipStream_t> saved(L);
+  for (int l = 0; l < L; ++l) {
+    saved[l] = sims[l]->ctx->stream;
+    CK(hipStreamSynchronize(saved[l]));
+  }
+  // per-simulator force descriptors and the workgroup partition
+  std::vector<ForceSys> S(L);
+  std::vector<ForceWork> W(L);
+  std::vector<int> start(L + 1, 0);
+  int G = 16;
+  size_t lds = 0;
+  for (int l = 0; l < L; ++l) {
+    fill_sys(sims[l]->ctx, S[l]);
+    W[l] = main_work(sims[l]);
+    W[l].xcd_chunks = 1;
+    G = std::max(G, group_for(sims[l]->ctx, S[l]));
+    if (S[l].M <= 1536)
+      lds = std::max(lds, (size_t)S[l].M * (sims[0]->precision == 0 ? 32 : 16));
+  }
+  G = G >= 64 ? 64 : (G >= 32 ? 32 : 16);
+  const int gpb = 256 / G;
+  for (int l = 0; l < L; ++l) {
+    int b = (sims[l]->S + gpb - 1) / gpb;
+    if (b > 1024) b = 1024;
+    b = (b + 7) / 8 * 8;
+    start[l + 1] = start[l] + b;
+  }
+  ForceSys* dS = nullptr;
+  ForceWork* dW = nullptr;
+  int* dstart = nullptr;
+  auto cleanup = [&]() {
+    for (int l = 0; l < L; ++l) sims[l]->ctx->stream = saved[l];
+    if (dS) hipFree(dS);
+    if (dW) hipFree(dW);
+    if (dstart) hipFree(dstart);
+  };
+  if (hipMalloc(&dS, sizeof(ForceSys) * L) != hipSuccess ||
+      hipMalloc(&dW, sizeof(ForceWork) * L) != hipSuccess ||
+      hipMalloc(&dstart, sizeof(int) * (L + 1)) != hipSuccess) {
+    cleanup();
+    FAIL("pst_group_run: out of device memory");
+  }
+  if (hipMemcpy(dS, S.data(), sizeof(ForceSys) * L, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dW, W.data(), sizeof(ForceWork) * L, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dstart, start.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) != hipSuccess) {
+    cleanup();
+    FAIL("pst_group_run: upload failed");
+  }
+  auto step = [&](int par) -> int {
+    for (int l = 0; l < L; ++l) {
+      const int r = enqueue_pre(sims[l], par);
+      if (r) return r;
+    }
+    hipError_t e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G,
+                                       sims[0]->ctx->stream);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
+    for (int l = 0; l < L; ++l) {
+      const int r = enqueue_post(sims[l], S[l], W[l], par);
+      if (r) return r;
+    }
+    return 0;
+  };
+  for (int l = 0; l < L; ++l) sims[l]->ctx->stream = st;
+  int rc = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  const int gs = graph_steps;
+  if (gs > 0) {
+    hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { cleanup(); ctx->err = hipGetErrorString(e); return 2; }
+    for (int l = 0; l < L; ++l) sims[l]->ctx->stream = cap;
+    for (int i = 0; i < gs && !rc; ++i) rc = step(i & 1);
+    for (int l = 0; l < L; ++l) sims[l]->ctx->stream = st;
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cap, &g);
+    if (rc || e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      hipStreamDestroy(cap);
+      cleanup();
+      if (!rc) { ctx->err = hipGetErrorString(e); rc = 2; }
+      return rc;
+    }
+    graph = g;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      hipGraphDestroy(graph);
+      hipStreamDestroy(cap);
+      cleanup();
+      ctx->err = hipGetErrorString(e);
+      return 2;
+    }
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, st);
+  long long steps = 0;
+  std::vector<RunCounters> h(L);
+  bool stop = false;
+  while (steps < max_steps && !stop && !rc) {
+    if (gs > 0 && steps + gs <= max_steps) {
+      hipError_t e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = 2; break; }
+      steps += gs;
+    } else {
+      rc = step((int)(steps & 1));
+      if (rc) break;
+      steps += 1;
+    }
+    bool all_done = true;
+    for (int l = 0; l < L; ++l)
+      hipMemcpyAsync(&h[l], sims[l]->ctr, sizeof(RunCounters), hipMemcpyDeviceToHost, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = 2; break; }
+    for (int l = 0; l < L; ++l)
+      if (!(h[l].stop || (h[l].n_sr + h[l].n_mlp) == 0)) all_done = false;
+    stop = all_done;
+  }
+  hipEventRecord(e1, st);
+  hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  if (cap) hipStreamDestroy(cap);
+  cleanup();
+  if (rc) return rc;
+  if (ms_per_step) *ms_per_step = steps ? ms / steps : 0.0;
+  if (c16) {
+    for (int l = 0; l < L; ++l) {
+      const RunCounters& x = h[l];
+      double v[16] = {(double)x.n_reacted, (double)x.n_escaped, (double)x.n_maxsteps,
+                      (double)x.total_steps, (double)x.next_traj, (double)x.n_returns,
+                      (double)x.n_overlap, (double)x.n_bridge, (double)steps, (double)x.stop,
+                      (double)x.n_sr, (double)x.n_mlp, x.audit_err2, x.audit_ref2,
+                      (double)x.audit_n, 0};
+      memcpy(c16 + 16 * l, v, sizeof(v));
+    }
+  }
+  return 0;
 }
 
 int pst_prod_sm_counts(pst_sim* s, unsigned long long* out48) {

@@ -89,8 +89,10 @@ __device__ inline bool atom_reverse(const ForceSys& S, double x, double y, doubl
 }
 
 
+// One system's force evaluation by workgroup `blk` of `nblk` (the workgroups
+// of k_forces, or the slice of a grouped launch that belongs to the system).
 template <typename TC, int G, bool POSE>
-__global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W) {
+__device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int blk, int nblk) {
   typedef typename Vec4<TC>::type V4;
   extern __shared__ char smem_raw[];
   V4* s_lig = reinterpret_cast<V4*>(smem_raw);
@@ -116,12 +118,12 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
   // direction, is cut into 8 contiguous chunks and XCD x walks chunk x, so
   // the trajectories in flight on one XCD gather from one region of the grids
   // and share its L2.
-  int lo = 0, hi = n, first = blockIdx.x * gpb, stride = gridDim.x * gpb;
+  int lo = 0, hi = n, first = blk * gpb, stride = nblk * gpb;
   if (W.xcd_chunks) {
-    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+    const int xcd = blk & 7, per = nblk >> 3;
     lo = (int)(((long)n * xcd) >> 3);
     hi = (int)(((long)n * (xcd + 1)) >> 3);
-    first = lo + (blockIdx.x >> 3) * gpb;
+    first = lo + (blk >> 3) * gpb;
     stride = per * gpb;
   }
   for (int base = first; base < hi; base += stride) {
@@ -246,6 +248,49 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
       if (W.T) { W.T[3 * o] = Tx; W.T[3 * o + 1] = Ty; W.T[3 * o + 2] = Tz; }
       if (W.E) W.E[o] = Ee;
     }
+  }
+}
+
+template <typename TC, int G, bool POSE>
+__global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W) {
+  forces_body<TC, G, POSE>(S, W, blockIdx.x, gridDim.x);
+}
+
+// Several systems (one ligand each, sharing the receptor grids) in one
+// launch: workgroup b belongs to system l with start[l] <= b < start[l + 1]
+// (start[] multiples of 8, so the XCD of a workgroup is that of its slice).
+template <typename TC, int G>
+__global__ void __launch_bounds__(kForceBlock) k_forces_group(const ForceSys* Sg,
+                                                              const ForceWork* Wg,
+                                                              const int* start, int L) {
+  const int b = blockIdx.x;
+  int l = 0;
+  while (l + 1 < L && b >= start[l + 1]) ++l;
+  forces_body<TC, G, true>(Sg[l], Wg[l], b - start[l], start[l + 1] - start[l]);
+}
+
+template <typename TC, int G>
+static hipError_t launch_group_g(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
+                                 int blocks, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((k_forces_group<TC, G>), dim3(blocks), dim3(kForceBlock), lds, st, Sg, Wg,
+                     start, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_forces_group(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
+                               int blocks, size_t lds, int precision, int G, hipStream_t st) {
+  if (blocks <= 0) return hipSuccess;
+  if (precision == 0) {
+    switch (G) {
+      case 16: return launch_group_g<double, 16>(Sg, Wg, start, L, blocks, lds, st);
+      case 32: return launch_group_g<double, 32>(Sg, Wg, start, L, blocks, lds, st);
+      default: return launch_group_g<double, 64>(Sg, Wg, start, L, blocks, lds, st);
+    }
+  }
+  switch (G) {
+    case 16: return launch_group_g<float, 16>(Sg, Wg, start, L, blocks, lds, st);
+    case 32: return launch_group_g<float, 32>(Sg, Wg, start, L, blocks, lds, st);
+    default: return launch_group_g<float, 64>(Sg, Wg, start, L, blocks, lds, st);
   }
 }
 

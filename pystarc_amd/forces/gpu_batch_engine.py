@@ -20,6 +20,9 @@ Extra keyword-only options (defaults give the reference behaviour):
   torque_mode "reference" (zero torque, :597-599) or "physical" (lever arm
               about the centroid of charged atoms, pystarc/forces/engine.py:144-183).
   grid_dtype  None keeps each grid's dtype, or "float32"/"float64".
+  receptor_from  another engine on the same device whose receptor grids this
+              engine uses without a copy (several ligands against one
+              receptor; the other engine must stay alive).
 """
 from __future__ import annotations
 
@@ -52,7 +55,7 @@ class GPUBatchForceEngine:
                  lig_born_grids=None, rec_positions=None, rec_charges=None,
                  multipole_expansion=None, rec_radii=None, lig_radii=None, use_lj=False, *,
                  precision: str = "fp64", torque_mode: str = "reference", grid_dtype=None,
-                 device: int = 0, force_group: int = 0):
+                 device: int = 0, force_group: int = 0, receptor_from=None):
         if use_lj:
             raise NotImplementedError("WCA repulsion (use_lj=True) is not part of this build")
         if precision not in ("fp64", "fp32"):
@@ -86,8 +89,16 @@ class GPUBatchForceEngine:
         if lig_born_grids and rec_positions is not None and rec_charges is not None:
             lig_born = sorted(list(lig_born_grids),
                               key=lambda g: float(np.asarray(g.delta)[0, 0]))[:1]
-        self._elec_grids_gpu = [self._upload("elec", g) for g in elec]
-        self._born_grids_gpu = [self._upload("born", g) for g in born]
+        self._receptor_src = receptor_from
+        if receptor_from is not None:
+            # one resident copy of the receptor grids for several ligand engines
+            self.ctx.check(lib.pst_share_receptor_grids(h, receptor_from.ctx.h),
+                           "pst_share_receptor_grids")
+            self._elec_grids_gpu = receptor_from._elec_grids_gpu
+            self._born_grids_gpu = receptor_from._born_grids_gpu
+        else:
+            self._elec_grids_gpu = [self._upload("elec", g) for g in elec]
+            self._born_grids_gpu = [self._upload("born", g) for g in born]
         self._lig_born_grids_gpu = [self._upload("lig_born", g) for g in lig_born]
 
         dip = quad = None

@@ -739,3 +739,83 @@ class ProductionDriver:
             self.close()
         except Exception:
             pass
+
+
+class GPUBatchGroup:
+    """Several ligands against one receptor in one run: each ligand is a
+    GPUBatchSimulator (its own engine, built with ``receptor_from`` so the
+    receptor grids are resident once), and every step evaluates the
+    grid-path forces of all ligands in ONE force launch (pst_group_run).
+    ``run()`` returns one GPUBatchResult per ligand.  Philox mode; every
+    simulator must use the same force precision; surrogates per ligand are
+    allowed (their audits are collected but a fallback is not applied inside
+    a group run)."""
+
+    def __init__(self, simulators, graph_steps=16):
+        if not simulators:
+            raise ValueError("no simulators")
+        for s in simulators:
+            if s.rng_mode != "philox":
+                raise ValueError("group runs need rng='philox'")
+        if len({s.precision for s in simulators}) != 1:
+            raise ValueError("all simulators of a group need the same precision")
+        self.sims = list(simulators)
+        self.graph_steps = int(graph_steps)
+        self.last_ms_per_step = 0.0
+
+    def run(self):
+        sims = self.sims
+        lib, ctx = sims[0].engine.ctx.lib, sims[0].engine.ctx
+        handles, Ps = [], []
+        try:
+            for s in sims:
+                s._return_prob = s._compute_return_prob()
+                P = s._step_params()
+                h = s._make_sim(P)
+                handles.append(h)
+                Ps.append(P)
+                s.engine.ctx.check(s.engine.ctx.lib.pst_prod_init(h, int(s.record)),
+                                   "pst_prod_init")
+                if s.surrogate is not None:
+                    p9 = np.ascontiguousarray(s.surrogate.sim_params(s.audit_n), dtype=np.float64)
+                    s.engine.ctx.check(s.engine.ctx.lib.pst_sim_set_surrogate(h, s.surrogate.h,
+                                                                              ptr(p9)),
+                                       "pst_sim_set_surrogate")
+            L = len(sims)
+            arr = (C.c_void_p * L)(*[h.value for h in handles])
+            c16 = np.zeros((L, 16))
+            ms = C.c_double()
+            max_steps = max(int(P.max_steps) for P in Ps)
+            t0 = time.time()
+            steps = 0
+            while steps < max_steps:
+                todo = int(min(max(self.graph_steps, 1) * 64, max_steps - steps))
+                ctx.check(lib.pst_group_run(arr, L, todo, self.graph_steps, ptr(c16), C.byref(ms)),
+                          "pst_group_run")
+                steps += int(c16[0, 8])
+                done = all(c16[l, 9] or (c16[l, 10] + c16[l, 11]) == 0 for l in range(L))
+                if done or int(c16[0, 8]) < todo:
+                    break
+            elapsed = time.time() - t0
+            self.last_ms_per_step = ms.value
+            out = []
+            for l, (s, h, P) in enumerate(zip(sims, handles, Ps)):
+                c = c16[l]
+                N = int(P.n_total)
+                n_r, n_e = int(c[0]), int(c[1])
+                sd = {"n_returns_total": int(c[5]), "n_overlap_rejected": int(c[6]),
+                      "n_bridge": int(c[7]), "bd_steps": steps, "group_size": L}
+                if s.surrogate is not None and c[13] > 0:
+                    sd["surrogate_audit"] = {"err2": c[12], "ref2": c[13], "n": int(c[14]),
+                                             "rel_rms": math.sqrt(c[12] / c[13])}
+                if s.record:
+                    st = np.zeros(N, dtype=np.int8)
+                    ns = np.zeros(N, dtype=np.int32)
+                    ctx.check(lib.pst_prod_outcomes(h, ptr(st), ptr(ns)), "pst_prod_outcomes")
+                    sd["outcome"] = st
+                    sd["n_steps"] = ns
+                out.append(s._result(N, n_r, n_e, N - n_r - n_e, c[3], elapsed, sd))
+            return out
+        finally:
+            for s, h in zip(sims, handles):
+                s.engine.ctx.lib.pst_sim_destroy(h)

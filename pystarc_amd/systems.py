@@ -303,3 +303,51 @@ def from_pqr(receptor_pqr, ligand_pqr, rxns_xml, *, b=None, n_traj=100_000, seed
     return BDSystem(name or f"{mol1.name}-{mol2.name}", mol1, mol2, mob, ps, params, kw, specs,
                     {"a_rec": a_rec, "a_lig": a_lig, "files": [str(receptor_pqr), str(ligand_pqr),
                                                                str(rxns_xml)]})
+
+
+def multi_ligand(n_traj=1_000_000, seed=5015, rec_seed=5005, lig_seed=5006, n_rec=4700,
+                 n_lig=16, lig_atoms=(12, 64), rec_grid=(257, 0.5), lig_grid=(65, 0.4),
+                 b=40.0, max_steps=1_000_000, rec_net=4.0, density=0.0826):
+    """Config 5: a 4,700-atom receptor on 257^3 FP32 potential + Born grids
+    (0.5 A) and n_lig ligands of U{12..64} atoms (charges N(0, 0.3) e, integer
+    nets in [-2, 2]), each with its own 65^3 Born grid at 0.4 A and its own
+    two-of-three contact criterion (4.0 A, docked pose); b = 40 A.  Atom
+    clouds are jittered lattices at the atom density of the 1,700-atom
+    globule (0.0826 / A^3).  Returns one BDSystem per ligand: the first
+    carries the receptor grids, the others only their ligand grid (their
+    engines share the receptor grids, ``make_engine(..., receptor_from=)``)."""
+    rr_ = np.random.default_rng(rec_seed)
+    r_rec = (n_rec / density * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
+    rp, s1 = lattice_cloud(n_rec, (r_rec,) * 3, rr_)
+    rq = charges_with_net(n_rec, 0.35, rec_net, rr_)
+    rrad = rr_.uniform(1.4, 2.0, n_rec)
+    mol1 = Molecule.from_arrays(rp, rq, rrad, name="receptor")
+    ra = _atoms5(rp, rq, rrad)
+    gn, gh = rec_grid
+    ln, lh = lig_grid
+    mp = MultipoleExpansion(rp, rq, DEBYE_015M, 78.0)
+    kw = dict(alpha=ALPHA_DEFAULT, receptor_charge=float(rq.sum()), debye_length=DEBYE_015M,
+              sdie=78.0, rec_positions=rp, rec_charges=rq, multipole_expansion=mp)
+    lr_ = np.random.default_rng(lig_seed)
+    systems = []
+    for k in range(n_lig):
+        m = int(lr_.integers(lig_atoms[0], lig_atoms[1] + 1))
+        net = float(lr_.integers(-2, 3))
+        r_l = max((m / density * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0), 2.5)
+        lp, _ = lattice_cloud(m, (r_l,) * 3, lr_)
+        lq = charges_with_net(m, 0.3, net, lr_)
+        lrad = lr_.uniform(1.4, 2.0, m)
+        mol2 = Molecule.from_arrays(lp, lq, lrad, name=f"ligand{k}")
+        pairs = _contact_pairs(rp, lp, 3, 4.0)
+        ps = PathwaySet([ReactionInterface("rxn", ReactionCriteria("contact", pairs, n_needed=2))])
+        mob = MobilityTensor.from_radii(r_rec, r_l)
+        params = _params(n_traj, b, seed + k, max_steps, overlap=True)
+        specs = [GridSpec("lig_born", 1, (ln,) * 3, _cube_origin(ln, lh), lh, "float32",
+                          _atoms5(lp, lq, lrad))]
+        if k == 0:
+            specs = [GridSpec("elec", 0, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", ra),
+                     GridSpec("born", 1, (gn,) * 3, _cube_origin(gn, gh), gh, "float32", ra)] + specs
+        systems.append(BDSystem(f"multi_ligand_{k}", mol1, mol2, mob, ps, params, dict(kw), specs,
+                                {"ligand": k, "atoms": m, "net": net, "r_rec": r_rec,
+                                 "r_lig": r_l, "lattice_spacing": s1}))
+    return systems

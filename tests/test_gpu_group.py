@@ -1,0 +1,64 @@
+"""Several ligands against one receptor in one run (pst_group_run): the
+receptor grids are resident once, every step evaluates all ligands' forces in
+one launch, and each ligand's trajectories are exactly those of its own
+separate run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from pystarc_amd import systems as S  # noqa: E402
+from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchGroup, GPUBatchSimulator  # noqa: E402
+from tests._systems import dx_grids, numpy_grids  # noqa: E402
+
+
+def _small(n_traj=600, precision="fp64"):
+    """Four ligands of 12-30 atoms against a 300-atom receptor; every engine
+    uses 64 lanes per trajectory, the group launch's width, so the FP64
+    per-trajectory sums (and with them the trajectories) are identical."""
+    ls = S.multi_ligand(n_traj=n_traj, n_rec=300, n_lig=4, lig_atoms=(12, 30), rec_grid=(41, 0.5),
+                        lig_grid=(25, 0.4), b=16.0, max_steps=20000)
+    engines = []
+    for k, s in enumerate(ls):
+        g = dx_grids(numpy_grids(s))
+        kw = {"receptor_from": engines[0]} if k else {}
+        engines.append(S.make_engine(s, g, precision=precision, force_group=64, **kw))
+    return ls, engines
+
+
+def _sim(s, eng, **kw):
+    return GPUBatchSimulator(s.mol1, s.mol2, s.mob, s.pathway_set, s.params, eng, record=True,
+                             n_slots=int(s.params.n_trajectories), **kw)
+
+
+def test_shared_receptor_grids_give_the_same_forces():
+    ls, engines = _small()
+    own = S.make_engine(ls[2], dx_grids({**numpy_grids(ls[0]), "lig_born": numpy_grids(ls[2])["lig_born"]}),
+                        force_group=64)
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(200, 3))
+    c = d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(8, 24, (200, 1))
+    from oracle.physics import shoemake
+    q = shoemake(rng.random((200, 3)))
+    body = ls[2].mol2.positions_array() - ls[2].mol2.centroid()
+    for e in (engines[2], own):
+        e.set_ligand(body, ls[2].mol2.charges_array())
+    a = engines[2].forces_pose(c, q)
+    b = own.forces_pose(c, q)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("graph_steps", [0, 8])
+def test_group_run_equals_separate_runs(graph_steps):
+    ls, engines = _small()
+    sep = [_sim(s, e).run() for s, e in zip(ls, engines)]
+    grp = GPUBatchGroup([_sim(s, e) for s, e in zip(ls, engines)], graph_steps=graph_steps).run()
+    assert len(grp) == len(ls)
+    for a, b in zip(sep, grp):
+        assert (a.n_reacted, a.n_escaped, a.total_steps) == (b.n_reacted, b.n_escaped, b.total_steps)
+        assert np.array_equal(a.sim_data["outcome"], b.sim_data["outcome"])
+        assert np.array_equal(a.sim_data["n_steps"], b.sim_data["n_steps"])
+    assert sum(r.n_reacted for r in grp) > 0
