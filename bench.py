@@ -233,6 +233,127 @@ def pmc_mode(args):
     log(f"pmc mode done: {len(r['pos'])} live poses, copy of {src.numel() * 4} bytes")
 
 
+ML_WORKLOAD = ("multi-ligand: 4,700-atom receptor on 257^3 FP32 potential+Born grids (0.5 A), "
+               "16 ligands of U{12..64} atoms each with its own 65^3 Born grid (0.4 A), one force "
+               "launch per step for all 16, reverse Born over all receptor atoms, b=40 A, "
+               "2-of-3 contacts < 4 A")
+
+
+def multi_ligand_bench(args):
+    """The multi-ligand workload: 16 ligands against one receptor whose grids
+    are resident once, every step one grouped force launch.  Paths fp64 (plain)
+    and surrogate (FP32 grid path + one MLP per ligand, audited, with the grid
+    path taking over a ligand whose audit fails).  Prints one JSON line with
+    the group's trajectory-steps/s and per-ligand k_on of both paths."""
+    import torch
+    from pystarc_amd import systems as S
+    from pystarc_amd._lib import Context
+    from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchGroup, GPUBatchSimulator
+    from pystarc_amd.surrogate import build_surrogate
+    ls = S.multi_ligand(n_traj=args.kon_trajectories)
+    t0 = time.time()
+    ctx = Context(0)
+    grids = [S.build_grids(s, ctx) for s in ls]
+    ctx.close()
+    log(f"multi-ligand: {len(ls)} ligands, atoms {[s.meta['atoms'] for s in ls]}, "
+        f"grids built in {time.time() - t0:.1f}s")
+    engines = {}
+    for prec in ("fp64", "fp32"):
+        e0 = S.make_engine(ls[0], grids[0], precision=prec)
+        engines[prec] = [e0] + [S.make_engine(s, g, precision=prec, receptor_from=e0)
+                                for s, g in zip(ls[1:], grids[1:])]
+    paths = [p for p in args.paths.split(",") if p in ("fp64", "fp32", "surrogate")]
+    surr = []
+    t0 = time.time()
+    for k, (s, e) in enumerate(zip(ls, engines["fp64"]) if "surrogate" in paths else ()):
+        surr.append(build_surrogate(e, s, n_train=args.ml_train_poses, epochs=args.ml_train_epochs,
+                                    batch=args.train_batch, max_rows=max(args.ligand_slots, args.train_batch),
+                                    seed=k))
+        log(f"ligand {k}: surrogate validation force error {surr[-1].val_error:.3f}")
+    t_train = time.time() - t0
+    if surr:
+        log(f"{len(surr)} surrogates trained in {t_train:.1f}s")
+
+    def group(path, n_slots):
+        prec = "fp64" if path == "fp64" else "fp32"
+        sims = [GPUBatchSimulator(s.mol1, s.mol2, s.mob, s.pathway_set, s.params, e,
+                                  n_slots=n_slots, graph_steps=args.graph_steps,
+                                  surrogate=surr[k] if path == "surrogate" else None,
+                                  audit_n=args.audit_n, audit_tol=args.audit_tol,
+                                  hydro=args.hydro)
+                for k, (s, e) in enumerate(zip(ls, engines[prec]))]
+        return GPUBatchGroup(sims, graph_steps=args.graph_steps)
+
+    results, kon = {}, {}
+    for p in paths:
+        for s in ls:
+            s.params.n_trajectories = 10_000_000
+        r = group(p, args.ligand_slots).timed_window(args.warmup, args.steps)
+        log(f"path {p}: {r['value']:.4g} traj-steps/s, {r['device_ms_per_step']:.3f} ms/step device")
+        results[p] = r
+    if not args.no_kon:
+        for p in ("fp64", "surrogate"):
+            if p not in paths:
+                continue
+            for s in ls:
+                s.params.n_trajectories = args.kon_trajectories
+            t0 = time.time()
+            last = [t0]
+
+            def progress(steps, c16, p=p, t0=t0, last=last):
+                if time.time() - last[0] > 20.0:
+                    last[0] = time.time()
+                    done = int((c16[:, 0] + c16[:, 1]).sum())
+                    log(f"k_on {p}: step {steps}, {done} of {len(ls) * args.kon_trajectories} "
+                        f"trajectories ended, {time.time() - t0:.0f}s")
+
+            res = group(p, min(args.kon_trajectories, args.ligand_slots)).run(progress)
+            wall = time.time() - t0
+            per = []
+            for k, x in enumerate(res):
+                lo, hi = x.reaction_probability_ci(0.95)
+                d = {"ligand": k, "atoms": ls[k].meta["atoms"], "net": ls[k].meta["net"],
+                     "n": x.n_trajectories, "reacted": x.n_reacted, "escaped": x.n_escaped,
+                     "P": x.reaction_probability, "P_ci95": [lo, hi],
+                     "k_on": x.rate_constant(0.0, x.k_b)}
+                if "surrogate_audit" in x.sim_data:
+                    a = x.sim_data["surrogate_audit"]
+                    d["audit_rel_rms"] = a["rel_rms"]
+                    d["fallback_step"] = a["fallback_step"]
+                per.append(d)
+            tot = sum(x.total_steps for x in res)
+            kon[p] = {"wall_s": wall, "traj_steps": tot, "traj_steps_per_s": tot / wall,
+                      "ligands": per}
+            log(f"k_on {p}: {len(res)} ligands x {args.kon_trajectories} trajectories in "
+                f"{wall:.1f}s ({tot / wall:.4g} traj-steps/s)")
+        if "fp64" in kon and "surrogate" in kon:
+            kon["surrogate_in_plain_ci95"] = [
+                bool(a["P_ci95"][0] <= b["P"] <= a["P_ci95"][1])
+                for a, b in zip(kon["fp64"]["ligands"], kon["surrogate"]["ligands"])]
+    head = "surrogate" if "surrogate" in results else paths[0]
+    best = results[head]
+    plain = results.get("fp64")
+    out = {
+        "metric": METRIC, "value": best["value"], "unit": "trajectory-steps/s", "n_gpus": 1,
+        "steps": best["steps"], "warmup": args.warmup,
+        "ms_per_step": best["wall_s"] * 1e3 / max(best["steps"], 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"surrogate": "f64 state, f32 grid forces, bf16 MLP (f32 accumulate)",
+                  "fp32": "f64 state, f32 forces", "fp64": "f64"}[head],
+        "data": "synthetic",
+        "config": {"workload": ML_WORKLOAD, "path": head, "slots_per_ligand": args.ligand_slots,
+                   "graph_steps": args.graph_steps, "hydro": args.hydro},
+        "speedup_vs_plain_fp64": (best["value"] / plain["value"]) if plain else None,
+        "roofline": None, "cpu_baseline": None,
+        "surrogates": {"train_poses": args.ml_train_poses, "epochs": args.ml_train_epochs,
+                       "train_s": t_train, "validation_force_rel_rms": [x.val_error for x in surr]},
+        "k_on": kon,
+        "paths": {k: {kk: v[kk] for kk in ("value", "device_ms_per_step", "traj_steps_per_ligand",
+                                           "audit_rel_rms")} for k, v in results.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +376,11 @@ def main():
                     help="two-body RPY 12x12 mobility (config default) or scalar D_parallel")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the heuristic force launch instead of tuning it in the warm-up")
+    ap.add_argument("--workload", default="protein_protein", choices=("protein_protein", "multi_ligand"),
+                    help="protein_protein (the headline) or multi_ligand (16 ligands, one launch)")
+    ap.add_argument("--ligand-slots", type=int, default=8192, help="multi_ligand: slots per ligand")
+    ap.add_argument("--ml-train-poses", type=int, default=1_000_000)
+    ap.add_argument("--ml-train-epochs", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU in this build (asked for {args.gpus}); "
@@ -267,6 +393,9 @@ def main():
         sys.exit(2)
     if args.pmc:
         pmc_mode(args)
+        return
+    if args.workload == "multi_ligand":
+        multi_ligand_bench(args)
         return
     from pystarc_amd import systems as S
     sysm, grids = build_system(args)

@@ -130,15 +130,23 @@ __global__ void __launch_bounds__(kForceBlock) k_bin_pass(ForceSys S, ForceWork 
     }
     if (do_rev) {
       const bool live = valid && !(rc - S.rec_max_r > S.rev_cull);
-      for (int j0 = 0; j0 < S.n_rec; j0 += G) {
-        const int j = j0 + lane;
-        int key = -1;
-        if (live && j < S.n_rec && fabs(S.rec_q[j]) >= 1e-9) {
-          double x, y, z;
-          rec_in_lig(S, R, cx, cy, cz, j, x, y, z);
-          if (grid_inside(S.lig_born, x, y, z)) key = B.nb_rec + brick_of(S.lig_born, B.lb, x, y, z);
+      // runs of receptor atoms out of the ligand grid's reach emit nothing;
+      // a run is skipped when no lane of the wave needs it (bin_emit is a
+      // wave-wide operation)
+      for (int k = 0; k < S.n_rec_chunk; ++k) {
+        const bool near = live && rec_chunk_near(S, k, cx, cy, cz, S.rev_cull);
+        if (__ballot(near) == 0ull) continue;
+        const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
+        for (int j0 = k * PST_REC_CHUNK; j0 < j1; j0 += G) {
+          const int j = j0 + lane;
+          int key = -1;
+          if (near && j < j1 && fabs(S.rec_q[j]) >= 1e-9) {
+            double x, y, z;
+            rec_in_lig(S, R, cx, cy, cz, j, x, y, z);
+            if (grid_inside(S.lig_born, x, y, z)) key = B.nb_rec + brick_of(S.lig_born, B.lb, x, y, z);
+          }
+          bin_emit<PASS>(B, key, B.rev_base + w * S.n_rec + j);
         }
-        bin_emit<PASS>(B, key, B.rev_base + w * S.n_rec + j);
       }
     }
   }
@@ -304,17 +312,21 @@ __global__ void __launch_bounds__(kForceBlock) k_bin_reduce(ForceSys S, ForceWor
     }
     TC rx = 0, ry = 0, rz = 0, rtx = 0, rty = 0, rtz = 0;
     if (S.has_rev && valid && S.alpha > 1e-12 && !(rc - S.rec_max_r > S.rev_cull)) {
-      for (int j = lane; j < S.n_rec; j += G) {
-        double x, y, z;
-        rec_in_lig(S, R, cx, cy, cz, j, x, y, z);
-        if (grid_inside(S.lig_born, x, y, z) && fabs(S.rec_q[j]) >= 1e-9) {
-          const O4 v = out_rev[(size_t)w * S.n_rec + j];
-          const TC ax = v.x, ay = v.y, az = v.z;
-          rx += ax; ry += ay; rz += az;
-          if (S.torque_mode == 1) {
-            const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
-                     lz = (TC)(z - S.cq_body[2]);
-            rtx += ly * az - lz * ay; rty += lz * ax - lx * az; rtz += lx * ay - ly * ax;
+      for (int k = 0; k < S.n_rec_chunk; ++k) {
+        if (!rec_chunk_near(S, k, cx, cy, cz, S.rev_cull)) continue;
+        const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
+        for (int j = k * PST_REC_CHUNK + lane; j < j1; j += G) {
+          double x, y, z;
+          rec_in_lig(S, R, cx, cy, cz, j, x, y, z);
+          if (grid_inside(S.lig_born, x, y, z) && fabs(S.rec_q[j]) >= 1e-9) {
+            const O4 v = out_rev[(size_t)w * S.n_rec + j];
+            const TC ax = v.x, ay = v.y, az = v.z;
+            rx += ax; ry += ay; rz += az;
+            if (S.torque_mode == 1) {
+              const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
+                       lz = (TC)(z - S.cq_body[2]);
+              rtx += ly * az - lz * ay; rty += lz * ax - lx * az; rtz += lx * ay - ly * ax;
+            }
           }
         }
       }

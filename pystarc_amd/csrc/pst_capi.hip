@@ -45,6 +45,8 @@ struct pst_ctx {
   int n_rec = 0;
   double* d_rec = nullptr;
   double* d_rec_q = nullptr;
+  double* d_rec_chunk = nullptr;
+  int n_rec_chunk = 0;
   double rec_max_r = 0;
   int torque_mode = 0;
   int force_group = 0;
@@ -135,6 +137,61 @@ static hipError_t set_dev(pst_ctx* ctx) { return hipSetDevice(ctx->device); }
 
 static GridDev grid_record(const DevGrid& d) { return d.g; }
 
+// Recursive bisection of the receptor atoms into runs of PST_REC_CHUNK
+// (the last run takes the remainder): split along the longest edge of the
+// bounding box at a multiple of PST_REC_CHUNK atoms, ties broken by atom
+// index; each final run is kept in input order.  spheres gets (centre,
+// radius) per run, the radius rounded up so that |r_j - centre| <= radius
+// holds for every atom of the run in floating point.
+static void rec_chunk_order(int n, const double* pos, std::vector<int>& order,
+                            std::vector<double>& spheres) {
+  order.resize(n);
+  for (int j = 0; j < n; ++j) order[j] = j;
+  struct Span { int lo, hi; };
+  std::vector<Span> todo{{0, n}}, leaves;
+  while (!todo.empty()) {
+    const Span s = todo.back();
+    todo.pop_back();
+    const int cnt = s.hi - s.lo;
+    if (cnt <= PST_REC_CHUNK) { leaves.push_back(s); continue; }
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int j = s.lo; j < s.hi; ++j)
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = fmin(mn[d], pos[3 * order[j] + d]);
+        mx[d] = fmax(mx[d], pos[3 * order[j] + d]);
+      }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d) if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+    const int chunks = (cnt + PST_REC_CHUNK - 1) / PST_REC_CHUNK;
+    const int mid = s.lo + (chunks / 2) * PST_REC_CHUNK;
+    std::nth_element(order.begin() + s.lo, order.begin() + mid, order.begin() + s.hi,
+                     [&](int a, int b) {
+                       const double pa = pos[3 * a + ax], pb = pos[3 * b + ax];
+                       return pa < pb || (pa == pb && a < b);
+                     });
+    todo.push_back({mid, s.hi});
+    todo.push_back({s.lo, mid});
+  }
+  std::sort(leaves.begin(), leaves.end(), [](const Span& a, const Span& b) { return a.lo < b.lo; });
+  spheres.assign(4 * leaves.size(), 0.0);
+  for (size_t k = 0; k < leaves.size(); ++k) {
+    const Span s = leaves[k];
+    std::sort(order.begin() + s.lo, order.begin() + s.hi);
+    double c[3] = {0, 0, 0};
+    for (int j = s.lo; j < s.hi; ++j)
+      for (int d = 0; d < 3; ++d) c[d] += pos[3 * order[j] + d];
+    for (int d = 0; d < 3; ++d) c[d] /= (double)(s.hi - s.lo);
+    double r = 0;
+    for (int j = s.lo; j < s.hi; ++j) {
+      const double* p = pos + 3 * order[j];
+      r = fmax(r, sqrt((p[0] - c[0]) * (p[0] - c[0]) + (p[1] - c[1]) * (p[1] - c[1]) +
+                       (p[2] - c[2]) * (p[2] - c[2])));
+    }
+    spheres[4 * k] = c[0]; spheres[4 * k + 1] = c[1]; spheres[4 * k + 2] = c[2];
+    spheres[4 * k + 3] = r * (1.0 + 1e-12) + 1e-9;
+  }
+}
+
 static void fill_sys(pst_ctx* ctx, ForceSys& S) {
   memset(&S, 0, sizeof(S));
   S.n_elec = (int)ctx->elec.size();
@@ -201,6 +258,8 @@ static void fill_sys(pst_ctx* ctx, ForceSys& S) {
   S.rec_pos = ctx->d_rec;
   S.rec_q = ctx->d_rec_q;
   S.rec_max_r = ctx->rec_max_r;
+  S.n_rec_chunk = ctx->n_rec_chunk;
+  S.rec_chunk = ctx->d_rec_chunk;
   S.torque_mode = ctx->torque_mode;
 }
 
@@ -252,6 +311,7 @@ int pst_ctx_destroy(pst_ctx* ctx) {
   if (ctx->d_q) hipFree(ctx->d_q);
   if (ctx->d_rec) hipFree(ctx->d_rec);
   if (ctx->d_rec_q) hipFree(ctx->d_rec_q);
+  if (ctx->d_rec_chunk) hipFree(ctx->d_rec_chunk);
   if (ctx->d_chunk) hipFree(ctx->d_chunk);
   for (void* p : ctx->bin_allocs) hipFree(p);
   if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
@@ -390,13 +450,33 @@ int pst_set_receptor(pst_ctx* ctx, int n, const double* pos, const double* charg
   CK(set_dev(ctx));
   if (ctx->d_rec) { hipFree(ctx->d_rec); ctx->d_rec = nullptr; }
   if (ctx->d_rec_q) { hipFree(ctx->d_rec_q); ctx->d_rec_q = nullptr; }
+  if (ctx->d_rec_chunk) { hipFree(ctx->d_rec_chunk); ctx->d_rec_chunk = nullptr; }
   ctx->n_rec = n;
+  ctx->n_rec_chunk = 0;
   ctx->rec_max_r = 0;
   if (n == 0) return 0;
+  // Device copy in chunk order: spatially compact runs of PST_REC_CHUNK atoms
+  // with a bounding sphere each, so the reverse-Born and overlap loops skip a
+  // whole run that cannot reach the ligand.  Both loops only add (reverse
+  // Born: per lane, in atom order) or take a minimum over the atoms, so the
+  // order the atoms are stored in is the only thing this changes.
+  std::vector<int> order;
+  std::vector<double> spheres;
+  rec_chunk_order(n, pos, order, spheres);
+  std::vector<double> sp(3 * (size_t)n), sq(n);
+  for (int j = 0; j < n; ++j) {
+    const int a = order[j];
+    sp[3 * j] = pos[3 * a]; sp[3 * j + 1] = pos[3 * a + 1]; sp[3 * j + 2] = pos[3 * a + 2];
+    sq[j] = charges[a];
+  }
+  ctx->n_rec_chunk = (int)(spheres.size() / 4);
   CK(hipMalloc(&ctx->d_rec, sizeof(double) * 3 * n));
   CK(hipMalloc(&ctx->d_rec_q, sizeof(double) * n));
-  CK(hipMemcpy(ctx->d_rec, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->d_rec_q, charges, sizeof(double) * n, hipMemcpyHostToDevice));
+  CK(hipMalloc(&ctx->d_rec_chunk, sizeof(double) * spheres.size()));
+  CK(hipMemcpy(ctx->d_rec, sp.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->d_rec_q, sq.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->d_rec_chunk, spheres.data(), sizeof(double) * spheres.size(),
+               hipMemcpyHostToDevice));
   for (int j = 0; j < n; ++j) {
     const double r = sqrt(pos[3 * j] * pos[3 * j] + pos[3 * j + 1] * pos[3 * j + 1] +
                           pos[3 * j + 2] * pos[3 * j + 2]);

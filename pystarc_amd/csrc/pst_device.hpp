@@ -38,6 +38,18 @@ enum { RNG_INIT = 1, RNG_ESC = 2, RNG_RETQ = 3, RNG_NOISE = 4, RNG_BRIDGE = 5 };
 constexpr int kForceBlock = 256;
 constexpr int kLdsAtomsMax = 1536;  // ligand atoms staged in LDS (<= 48 KB in FP64)
 
+// Whether receptor run k (PST_REC_CHUNK consecutive atoms, bounding sphere
+// rec_chunk[k]) can hold an atom within `reach` of (cx, cy, cz).  False only
+// when every atom of the run is farther than reach (the sphere radius and the
+// reaches it is used with carry margins above rounding).
+__device__ inline bool rec_chunk_near(const ForceSys& S, int k, double cx, double cy, double cz,
+                                      double reach) {
+  const double* s = S.rec_chunk + 4 * k;
+  const double dx = s[0] - cx, dy = s[1] - cy, dz = s[2] - cz;
+  const double lim = reach + s[3];
+  return dx * dx + dy * dy + dz * dz <= lim * lim;
+}
+
 // Sum over the G lanes of a trajectory group (butterfly, FP64).
 template <int G>
 __device__ inline double group_sum(double v) {

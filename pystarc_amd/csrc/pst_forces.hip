@@ -207,20 +207,27 @@ __device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int bl
         skip = rc - S.rec_max_r > S.rev_cull;
       }
       if (!skip) {
-        for (int j = lane; j < S.n_rec; j += G) {
-          const double dx = S.rec_pos[3 * j] - cx, dy = S.rec_pos[3 * j + 1] - cy,
-                       dz = S.rec_pos[3 * j + 2] - cz;
-          // x = R^T (r - c) (:641-644)
-          const double x = R[0] * dx + R[3] * dy + R[6] * dz;
-          const double y = R[1] * dx + R[4] * dy + R[7] * dz;
-          const double z = R[2] * dx + R[5] * dy + R[8] * dz;
-          TC ax, ay, az;
-          if (atom_reverse<TC>(S, x, y, z, S.rec_q[j], ax, ay, az)) {
-            rx += ax; ry += ay; rz += az;
-            if (S.torque_mode == 1) {
-              const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
-                       lz = (TC)(z - S.cq_body[2]);
-              rtx += ly * az - lz * ay; rty += lz * ax - lx * az; rtz += lx * ay - ly * ax;
+        // runs of receptor atoms farther than rev_cull from the centre have no
+        // atom in the ligand grid box; lane `lane` still visits its atoms
+        // (j = lane mod G) in increasing order
+        for (int k = 0; k < S.n_rec_chunk; ++k) {
+          if (!rec_chunk_near(S, k, cx, cy, cz, S.rev_cull)) continue;
+          const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
+          for (int j = k * PST_REC_CHUNK + lane; j < j1; j += G) {
+            const double dx = S.rec_pos[3 * j] - cx, dy = S.rec_pos[3 * j + 1] - cy,
+                         dz = S.rec_pos[3 * j + 2] - cz;
+            // x = R^T (r - c) (:641-644)
+            const double x = R[0] * dx + R[3] * dy + R[6] * dz;
+            const double y = R[1] * dx + R[4] * dy + R[7] * dz;
+            const double z = R[2] * dx + R[5] * dy + R[8] * dz;
+            TC ax, ay, az;
+            if (atom_reverse<TC>(S, x, y, z, S.rec_q[j], ax, ay, az)) {
+              rx += ax; ry += ay; rz += az;
+              if (S.torque_mode == 1) {
+                const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
+                         lz = (TC)(z - S.cq_body[2]);
+                rtx += ly * az - lz * ay; rty += lz * ax - lx * az; rtz += lx * ay - ly * ax;
+              }
             }
           }
         }

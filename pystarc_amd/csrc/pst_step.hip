@@ -120,17 +120,24 @@ __device__ inline double diffusion(const StepParams& P, double r) {
 }
 
 // Overlap: min_j |c - r_j| < threshold (:1106-1128).  The loop is skipped
-// when |c| - max|r_j| already exceeds the threshold, which cannot change the
-// outcome.
+// when |c| - max|r_j| already exceeds the threshold, and so is every run of
+// receptor atoms whose bounding sphere stays farther than the threshold: an
+// atom closer than the threshold lies in a visited run, so the minimum
+// compared with the threshold, and the outcome, are those of the full loop.
 __device__ inline bool overlaps(const StepParams& P, const ForceSys& S, double cx, double cy,
                                 double cz) {
   const double rc = norm3(cx, cy, cz);
-  if (rc - S.rec_max_r > P.overlap_thresh + 1e-9) return false;
+  const double reach = P.overlap_thresh + 1e-9;
+  if (rc - S.rec_max_r > reach) return false;
   double m2 = 1e300;
-  for (int j = 0; j < S.n_rec; ++j) {
-    const double dx = cx - S.rec_pos[3 * j], dy = cy - S.rec_pos[3 * j + 1],
-                 dz = cz - S.rec_pos[3 * j + 2];
-    m2 = fmin(m2, dx * dx + dy * dy + dz * dz);
+  for (int k = 0; k < S.n_rec_chunk; ++k) {
+    if (!rec_chunk_near(S, k, cx, cy, cz, reach)) continue;
+    const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
+    for (int j = k * PST_REC_CHUNK; j < j1; ++j) {
+      const double dx = cx - S.rec_pos[3 * j], dy = cy - S.rec_pos[3 * j + 1],
+                   dz = cz - S.rec_pos[3 * j + 2];
+      m2 = fmin(m2, dx * dx + dy * dy + dz * dz);
+    }
   }
   return sqrt(m2) < P.overlap_thresh;
 }

@@ -5,6 +5,7 @@
 
 #define PST_MAX_GRIDS 4
 #define PST_MAX_PAIRS 16
+#define PST_REC_CHUNK 64
 
 // Trajectory status codes (pystarc/simulation/gpu_batch_simulator.py:514).
 enum { PST_RUNNING = 0, PST_REACTED = 1, PST_ESCAPED = 2, PST_MAXSTEPS = 3, PST_IDLE = 4 };
@@ -59,6 +60,11 @@ struct ForceSys {
   const double* rec_q;      // (Nr,)
   const float* rec_f4;      // (Nr,4)
   double rec_max_r;         // max |r_rec|
+  // receptor atoms are stored in spatially compact chunks of PST_REC_CHUNK
+  // consecutive atoms; chunk k covers atoms [k*PST_REC_CHUNK, (k+1)*PST_REC_CHUNK)
+  // and rec_chunk[4k..4k+3] = (centre x, y, z, radius) bounds all of them
+  int n_rec_chunk;
+  const double* rec_chunk;  // (n_rec_chunk,4)
   double rev_radius;        // max(|lo_x|,|hi_x|) of the ligand Born grid (:623)
   double rev_cull;          // exact cull: largest distance from the ligand centre to the valid box
   int torque_mode;          // 0 = reference (zero), 1 = lever arm about charged centroid

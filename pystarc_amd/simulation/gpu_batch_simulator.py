@@ -747,9 +747,11 @@ class GPUBatchGroup:
     receptor grids are resident once), and every step evaluates the
     grid-path forces of all ligands in ONE force launch (pst_group_run).
     ``run()`` returns one GPUBatchResult per ligand.  Philox mode; every
-    simulator must use the same force precision; surrogates per ligand are
-    allowed (their audits are collected but a fallback is not applied inside
-    a group run)."""
+    simulator must use the same force precision.  Each ligand may carry its
+    own surrogate: its audit is read after every chunk of steps and, when the
+    chunk's relative RMS force error exceeds that simulator's audit_tol, the
+    grid path takes over that ligand's trajectories from then on (the other
+    ligands keep theirs), as in a single run."""
 
     def __init__(self, simulators, graph_steps=16):
         if not simulators:
@@ -763,29 +765,75 @@ class GPUBatchGroup:
         self.graph_steps = int(graph_steps)
         self.last_ms_per_step = 0.0
 
-    def run(self):
+    def _open(self, handles, Ps, record=None):
+        for s in self.sims:
+            s._return_prob = s._compute_return_prob()
+            P = s._step_params()
+            h = s._make_sim(P)
+            handles.append(h)
+            Ps.append(P)
+            rec = int(s.record) if record is None else int(record)
+            s.engine.ctx.check(s.engine.ctx.lib.pst_prod_init(h, rec), "pst_prod_init")
+            if s.surrogate is not None:
+                p9 = np.ascontiguousarray(s.surrogate.sim_params(s.audit_n), dtype=np.float64)
+                s.engine.ctx.check(s.engine.ctx.lib.pst_sim_set_surrogate(h, s.surrogate.h, ptr(p9)),
+                                   "pst_sim_set_surrogate")
+
+    def timed_window(self, warmup, steps):
+        """Throughput of the group on live steps: `warmup` untimed steps, then
+        exactly `steps` steps timed on the host around device synchronisation
+        (slots refill from each ligand's pool, so every step is a full batch
+        while trajectories remain).  Returns trajectory-steps, wall time and
+        the device time per step; surrogates stay on (no audit fallback)."""
+        import torch
         sims = self.sims
         lib, ctx = sims[0].engine.ctx.lib, sims[0].engine.ctx
         handles, Ps = [], []
         try:
-            for s in sims:
-                s._return_prob = s._compute_return_prob()
-                P = s._step_params()
-                h = s._make_sim(P)
-                handles.append(h)
-                Ps.append(P)
-                s.engine.ctx.check(s.engine.ctx.lib.pst_prod_init(h, int(s.record)),
-                                   "pst_prod_init")
-                if s.surrogate is not None:
-                    p9 = np.ascontiguousarray(s.surrogate.sim_params(s.audit_n), dtype=np.float64)
-                    s.engine.ctx.check(s.engine.ctx.lib.pst_sim_set_surrogate(h, s.surrogate.h,
-                                                                              ptr(p9)),
-                                       "pst_sim_set_surrogate")
+            self._open(handles, Ps, 0)
+            L = len(sims)
+            arr = (C.c_void_p * L)(*[h.value for h in handles])
+            c16 = np.zeros((L, 16))
+            ms = C.c_double()
+            if warmup > 0:
+                ctx.check(lib.pst_group_run(arr, L, int(warmup), self.graph_steps, ptr(c16),
+                                            C.byref(ms)), "pst_group_run")
+            for h in handles:
+                if sims[0].surrogate is not None:
+                    ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
+            base = c16[:, 3].copy()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.check(lib.pst_group_run(arr, L, int(steps), self.graph_steps, ptr(c16), C.byref(ms)),
+                      "pst_group_run")
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            per = c16[:, 3] - base
+            return {"steps": int(c16[0, 8]), "wall_s": wall, "traj_steps": float(per.sum()),
+                    "traj_steps_per_ligand": per.tolist(), "device_ms_per_step": ms.value,
+                    "value": float(per.sum()) / wall if wall > 0 else 0.0,
+                    "audit_rel_rms": [math.sqrt(c[12] / c[13]) if c[13] > 0 else None for c in c16]}
+        finally:
+            for s, h in zip(sims, handles):
+                s.engine.ctx.lib.pst_sim_destroy(h)
+
+    def run(self, progress=None):
+        """Run every ligand's trajectories to completion.  ``progress``, if
+        given, is called after each chunk of steps with (steps so far, the
+        (L, 16) counter array)."""
+        sims = self.sims
+        lib, ctx = sims[0].engine.ctx.lib, sims[0].engine.ctx
+        handles, Ps = [], []
+        try:
+            self._open(handles, Ps)
             L = len(sims)
             arr = (C.c_void_p * L)(*[h.value for h in handles])
             c16 = np.zeros((L, 16))
             ms = C.c_double()
             max_steps = max(int(P.max_steps) for P in Ps)
+            surr_on = [s.surrogate is not None for s in sims]
+            audits = [{"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None}
+                      for _ in sims]
             t0 = time.time()
             steps = 0
             while steps < max_steps:
@@ -793,6 +841,22 @@ class GPUBatchGroup:
                 ctx.check(lib.pst_group_run(arr, L, todo, self.graph_steps, ptr(c16), C.byref(ms)),
                           "pst_group_run")
                 steps += int(c16[0, 8])
+                for l, s in enumerate(sims):
+                    if not surr_on[l] or c16[l, 13] <= 0:
+                        continue
+                    a, err = audits[l], math.sqrt(c16[l, 12] / c16[l, 13])
+                    a["err2"] += c16[l, 12]
+                    a["ref2"] += c16[l, 13]
+                    a["n"] += int(c16[l, 14])
+                    a["worst"] = max(a["worst"], err)
+                    ctx.check(lib.pst_sim_reset_audit(handles[l]), "pst_sim_reset_audit")
+                    if err > s.audit_tol:
+                        ctx.check(lib.pst_sim_set_surrogate(handles[l], None, None),
+                                  "pst_sim_set_surrogate")
+                        surr_on[l] = False
+                        a["fallback_step"] = steps
+                if progress is not None:
+                    progress(steps, c16)
                 done = all(c16[l, 9] or (c16[l, 10] + c16[l, 11]) == 0 for l in range(L))
                 if done or int(c16[0, 8]) < todo:
                     break
@@ -805,9 +869,10 @@ class GPUBatchGroup:
                 n_r, n_e = int(c[0]), int(c[1])
                 sd = {"n_returns_total": int(c[5]), "n_overlap_rejected": int(c[6]),
                       "n_bridge": int(c[7]), "bd_steps": steps, "group_size": L}
-                if s.surrogate is not None and c[13] > 0:
-                    sd["surrogate_audit"] = {"err2": c[12], "ref2": c[13], "n": int(c[14]),
-                                             "rel_rms": math.sqrt(c[12] / c[13])}
+                if s.surrogate is not None:
+                    a = audits[l]
+                    a["rel_rms"] = math.sqrt(a["err2"] / a["ref2"]) if a["ref2"] > 0 else 0.0
+                    sd["surrogate_audit"] = a
                 if s.record:
                     st = np.zeros(N, dtype=np.int8)
                     ns = np.zeros(N, dtype=np.int32)
