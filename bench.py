@@ -304,8 +304,10 @@ def multi_ligand_bench(args):
                 if time.time() - last[0] > 20.0:
                     last[0] = time.time()
                     done = int((c16[:, 0] + c16[:, 1]).sum())
+                    running = {k: int(c[0] + c[1]) for k, c in enumerate(c16) if not c[9]}
                     log(f"k_on {p}: step {steps}, {done} of {len(ls) * args.kon_trajectories} "
-                        f"trajectories ended, {time.time() - t0:.0f}s")
+                        f"trajectories ended, {time.time() - t0:.0f}s; ligands still running "
+                        f"(ended so far): {running}")
 
             res = group(p, min(args.kon_trajectories, args.ligand_slots)).run(progress)
             wall = time.time() - t0

@@ -47,6 +47,10 @@ struct pst_ctx {
   double* d_rec_q = nullptr;
   double* d_rec_chunk = nullptr;
   int n_rec_chunk = 0;
+  int cell_n[3] = {0, 0, 0};
+  double cell_o[3] = {0, 0, 0};
+  int* d_cell_start = nullptr;
+  double* d_cell_pos = nullptr;
   double rec_max_r = 0;
   int torque_mode = 0;
   int force_group = 0;
@@ -136,6 +140,46 @@ struct pst_sim {
 static hipError_t set_dev(pst_ctx* ctx) { return hipSetDevice(ctx->device); }
 
 static GridDev grid_record(const DevGrid& d) { return d.g; }
+
+// Cubic cells of kRecCell A over the receptor's bounding box plus one cell on
+// each side; atom a goes to cell floor((x_a - o) / kRecCell) per axis (the
+// division written as a multiplication by the reciprocal, as the kernels
+// form their query ranges).
+constexpr double kRecCell = 2.0;
+
+static void rec_cells(int n, const double* pos, int* cn, double* co, std::vector<int>& cstart,
+                      std::vector<double>& cpos) {
+  const double inv = 1.0 / kRecCell;
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int j = 0; j < n; ++j)
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = fmin(mn[d], pos[3 * j + d]);
+      mx[d] = fmax(mx[d], pos[3 * j + d]);
+    }
+  for (int d = 0; d < 3; ++d) {
+    co[d] = mn[d] - kRecCell;
+    cn[d] = (int)floor((mx[d] - co[d]) * inv) + 2;
+  }
+  const long nc = (long)cn[0] * cn[1] * cn[2];
+  std::vector<int> cell(n);
+  cstart.assign(nc + 1, 0);
+  for (int j = 0; j < n; ++j) {
+    int ijk[3];
+    for (int d = 0; d < 3; ++d) {
+      ijk[d] = (int)floor((pos[3 * j + d] - co[d]) * inv);
+      ijk[d] = ijk[d] < 0 ? 0 : (ijk[d] >= cn[d] ? cn[d] - 1 : ijk[d]);
+    }
+    cell[j] = (ijk[0] * cn[1] + ijk[1]) * cn[2] + ijk[2];
+    cstart[cell[j] + 1] += 1;
+  }
+  for (long c = 0; c < nc; ++c) cstart[c + 1] += cstart[c];
+  std::vector<int> cur(cstart.begin(), cstart.end() - 1);
+  cpos.assign(3 * (size_t)n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    const int p = cur[cell[j]]++;
+    for (int d = 0; d < 3; ++d) cpos[3 * (size_t)p + d] = pos[3 * j + d];
+  }
+}
 
 // Recursive bisection of the receptor atoms into runs of PST_REC_CHUNK
 // (the last run takes the remainder): split along the longest edge of the
@@ -260,6 +304,10 @@ static void fill_sys(pst_ctx* ctx, ForceSys& S) {
   S.rec_max_r = ctx->rec_max_r;
   S.n_rec_chunk = ctx->n_rec_chunk;
   S.rec_chunk = ctx->d_rec_chunk;
+  for (int d = 0; d < 3; ++d) { S.cell_n[d] = ctx->cell_n[d]; S.cell_o[d] = ctx->cell_o[d]; }
+  S.cell_inv = 1.0 / kRecCell;
+  S.cell_start = ctx->d_cell_start;
+  S.cell_pos = ctx->d_cell_pos;
   S.torque_mode = ctx->torque_mode;
 }
 
@@ -312,6 +360,8 @@ int pst_ctx_destroy(pst_ctx* ctx) {
   if (ctx->d_rec) hipFree(ctx->d_rec);
   if (ctx->d_rec_q) hipFree(ctx->d_rec_q);
   if (ctx->d_rec_chunk) hipFree(ctx->d_rec_chunk);
+  if (ctx->d_cell_start) hipFree(ctx->d_cell_start);
+  if (ctx->d_cell_pos) hipFree(ctx->d_cell_pos);
   if (ctx->d_chunk) hipFree(ctx->d_chunk);
   for (void* p : ctx->bin_allocs) hipFree(p);
   if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
@@ -451,8 +501,11 @@ int pst_set_receptor(pst_ctx* ctx, int n, const double* pos, const double* charg
   if (ctx->d_rec) { hipFree(ctx->d_rec); ctx->d_rec = nullptr; }
   if (ctx->d_rec_q) { hipFree(ctx->d_rec_q); ctx->d_rec_q = nullptr; }
   if (ctx->d_rec_chunk) { hipFree(ctx->d_rec_chunk); ctx->d_rec_chunk = nullptr; }
+  if (ctx->d_cell_start) { hipFree(ctx->d_cell_start); ctx->d_cell_start = nullptr; }
+  if (ctx->d_cell_pos) { hipFree(ctx->d_cell_pos); ctx->d_cell_pos = nullptr; }
   ctx->n_rec = n;
   ctx->n_rec_chunk = 0;
+  for (int d = 0; d < 3; ++d) ctx->cell_n[d] = 0;
   ctx->rec_max_r = 0;
   if (n == 0) return 0;
   // Device copy in chunk order: spatially compact runs of PST_REC_CHUNK atoms
@@ -477,6 +530,15 @@ int pst_set_receptor(pst_ctx* ctx, int n, const double* pos, const double* charg
   CK(hipMemcpy(ctx->d_rec_q, sq.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   CK(hipMemcpy(ctx->d_rec_chunk, spheres.data(), sizeof(double) * spheres.size(),
                hipMemcpyHostToDevice));
+  // uniform cell grid (counting sort by cell) for the overlap test
+  std::vector<int> cstart;
+  std::vector<double> cpos;
+  rec_cells(n, pos, ctx->cell_n, ctx->cell_o, cstart, cpos);
+  CK(hipMalloc(&ctx->d_cell_start, sizeof(int) * cstart.size()));
+  CK(hipMalloc(&ctx->d_cell_pos, sizeof(double) * cpos.size()));
+  CK(hipMemcpy(ctx->d_cell_start, cstart.data(), sizeof(int) * cstart.size(),
+               hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->d_cell_pos, cpos.data(), sizeof(double) * cpos.size(), hipMemcpyHostToDevice));
   for (int j = 0; j < n; ++j) {
     const double r = sqrt(pos[3 * j] * pos[3 * j] + pos[3 * j + 1] * pos[3 * j + 1] +
                           pos[3 * j + 2] * pos[3 * j + 2]);
@@ -1177,11 +1239,9 @@ static int enqueue_pre(pst_sim* s, int par) {
   return 0;
 }
 
-// Step phases after the grid-path forces: surrogate forces and their audit,
-// the move, the Brownian bridge.
-static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par) {
+// Surrogate forces of the step and their audit (after the grid-path forces).
+static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W) {
   pst_ctx* ctx = s->ctx;
-  ProdBufs B = prod_bufs(s);
   if (s->mlp && s->P.surrogate_r > 0) {
     CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
                  ctx->stream));
@@ -1200,6 +1260,16 @@ static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int p
                     ctx->stream));
     }
   }
+  return 0;
+}
+
+// Step phases after the grid-path forces: surrogate forces and their audit,
+// the move, the Brownian bridge.
+static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par) {
+  pst_ctx* ctx = s->ctx;
+  ProdBufs B = prod_bufs(s);
+  const int r = enqueue_surr(s, S, W);
+  if (r) return r;
   CK(prod_step_post(s->P, S, B, par, ctx->stream));
   CK(prod_step_bridge(s->P, B, par, ctx->stream));
   return 0;
@@ -1462,10 +1532,13 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
   pst_ctx* ctx = sims[0]->ctx;
   if (L > 64) FAIL("pst_group_run: at most 64 simulators");
   if (max_steps < 0 || graph_steps < 0) FAIL("pst_group_run: bad arguments");
+  const bool obs = prod_bufs(sims[0]).obs.traj != nullptr;
   for (int l = 0; l < L; ++l) {
     if (!sims[l] || !sims[l]->prod_ready) FAIL("pst_group_run: every simulator needs pst_prod_init");
     if (sims[l]->precision != sims[0]->precision) FAIL("pst_group_run: mixed precisions");
     if (sims[l]->ctx->device != ctx->device) FAIL("pst_group_run: simulators on different devices");
+    if ((prod_bufs(sims[l]).obs.traj != nullptr) != obs)
+      FAIL("pst_group_run: every simulator needs the same record mode (observables or not)");
   }
   CK(set_dev(ctx));
   const hipStream_t st = ctx->stream;
@@ -1474,14 +1547,21 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     saved[l] = sims[l]->ctx->stream;
     CK(hipStreamSynchronize(saved[l]));
   }
-  // per-simulator force descriptors and the workgroup partition
+  // per-simulator force descriptors, step parameters and buffers, and the
+  // workgroup partitions of the force launch and of the step-phase launches
   std::vector<ForceSys> S(L);
   std::vector<ForceWork> W(L);
-  std::vector<int> start(L + 1, 0);
+  std::vector<StepParams> P(L);
+  std::vector<ProdBufs> B(L);
+  std::vector<int> start(L + 1, 0), sstart(L + 1, 0);
   int G = 16;
   size_t lds = 0;
+  const int sb = prod_step_block();
   for (int l = 0; l < L; ++l) {
     fill_sys(sims[l]->ctx, S[l]);
+    P[l] = sims[l]->P;
+    B[l] = prod_bufs(sims[l]);
+    sstart[l + 1] = sstart[l] + (B[l].n_slots + sb - 1) / sb;
     W[l] = main_work(sims[l]);
     W[l].xcd_chunks = 1;
     G = std::max(G, group_for(sims[l]->ctx, S[l]));
@@ -1489,46 +1569,65 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       lds = std::max(lds, (size_t)S[l].M * (sims[0]->precision == 0 ? 32 : 16));
   }
   G = G >= 64 ? 64 : (G >= 32 ? 32 : 16);
+  // workgroups per system: the grid-stride loop covers any count, so the
+  // launch holds about 4096 resident workgroups in all (a launch of mostly
+  // idle workgroups costs dispatch time when few trajectories are left)
   const int gpb = 256 / G;
+  const int wg_cap = std::max(8, (4096 / L + 7) / 8 * 8);
   for (int l = 0; l < L; ++l) {
     int b = (sims[l]->S + gpb - 1) / gpb;
-    if (b > 1024) b = 1024;
+    if (b > wg_cap) b = wg_cap;
     b = (b + 7) / 8 * 8;
     start[l + 1] = start[l] + b;
   }
   ForceSys* dS = nullptr;
   ForceWork* dW = nullptr;
+  StepParams* dP = nullptr;
+  ProdBufs* dB = nullptr;
   int* dstart = nullptr;
+  int* dsstart = nullptr;
   auto cleanup = [&]() {
     for (int l = 0; l < L; ++l) sims[l]->ctx->stream = saved[l];
     if (dS) hipFree(dS);
     if (dW) hipFree(dW);
+    if (dP) hipFree(dP);
+    if (dB) hipFree(dB);
     if (dstart) hipFree(dstart);
+    if (dsstart) hipFree(dsstart);
   };
   if (hipMalloc(&dS, sizeof(ForceSys) * L) != hipSuccess ||
       hipMalloc(&dW, sizeof(ForceWork) * L) != hipSuccess ||
-      hipMalloc(&dstart, sizeof(int) * (L + 1)) != hipSuccess) {
+      hipMalloc(&dP, sizeof(StepParams) * L) != hipSuccess ||
+      hipMalloc(&dB, sizeof(ProdBufs) * L) != hipSuccess ||
+      hipMalloc(&dstart, sizeof(int) * (L + 1)) != hipSuccess ||
+      hipMalloc(&dsstart, sizeof(int) * (L + 1)) != hipSuccess) {
     cleanup();
     FAIL("pst_group_run: out of device memory");
   }
   if (hipMemcpy(dS, S.data(), sizeof(ForceSys) * L, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dW, W.data(), sizeof(ForceWork) * L, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dstart, start.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(dP, P.data(), sizeof(StepParams) * L, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dB, B.data(), sizeof(ProdBufs) * L, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dstart, start.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dsstart, sstart.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) !=
+          hipSuccess) {
     cleanup();
     FAIL("pst_group_run: upload failed");
   }
+  // one step of every system: each phase is one launch over all systems, the
+  // surrogate phase (own network per system) one set of launches per system
   auto step = [&](int par) -> int {
-    for (int l = 0; l < L; ++l) {
-      const int r = enqueue_pre(sims[l], par);
-      if (r) return r;
-    }
-    hipError_t e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G,
-                                       sims[0]->ctx->stream);
+    const hipStream_t cs = sims[0]->ctx->stream;
+    hipError_t e = prod_group_pre(dP, dB, dsstart, L, sstart[L], par, cs);
+    if (e == hipSuccess)
+      e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     for (int l = 0; l < L; ++l) {
-      const int r = enqueue_post(sims[l], S[l], W[l], par);
+      const int r = enqueue_surr(sims[l], S[l], W[l]);
       if (r) return r;
     }
+    e = prod_group_post(dP, dS, dB, dsstart, L, sstart[L], obs, par, cs);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     return 0;
   };
   for (int l = 0; l < L; ++l) sims[l]->ctx->stream = st;

@@ -112,6 +112,12 @@ hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st);
 hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs& B, int par,
                           hipStream_t st);
 hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
+int prod_step_block();
+hipError_t prod_group_pre(const StepParams* Pg, const ProdBufs* Bg, const int* start, int L,
+                          int blocks, int par, hipStream_t st);
+hipError_t prod_group_post(const StepParams* Pg, const ForceSys* Sg, const ProdBufs* Bg,
+                           const int* start, int L, int blocks, bool obs, int par,
+                           hipStream_t st);
 
 // Two-body RPY: 12x12 mobility / noise factor columns, and the move itself.
 hipError_t launch_rpy_eval(int n, const double* pos, double a0, double a1, double* mob,

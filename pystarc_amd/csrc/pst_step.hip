@@ -119,26 +119,35 @@ __device__ inline double diffusion(const StepParams& P, double r) {
   return fmax(P.hi_Df * P.hi_ainv + Di, 1e-6);
 }
 
-// Overlap: min_j |c - r_j| < threshold (:1106-1128).  The loop is skipped
-// when |c| - max|r_j| already exceeds the threshold, and so is every run of
-// receptor atoms whose bounding sphere stays farther than the threshold: an
-// atom closer than the threshold lies in a visited run, so the minimum
-// compared with the threshold, and the outcome, are those of the full loop.
+// Overlap: min_j |c - r_j| < threshold (:1106-1128).  The test is skipped
+// when |c| - max|r_j| already exceeds the threshold; otherwise only the
+// receptor cells that intersect the cube of half-edge threshold (+1e-9)
+// about c are visited.  Every atom closer than the threshold lies in one of
+// them, so the minimum compared with the threshold, and the outcome, are
+// those of the loop over all atoms.
 __device__ inline bool overlaps(const StepParams& P, const ForceSys& S, double cx, double cy,
                                 double cz) {
   const double rc = norm3(cx, cy, cz);
   const double reach = P.overlap_thresh + 1e-9;
   if (rc - S.rec_max_r > reach) return false;
-  double m2 = 1e300;
-  for (int k = 0; k < S.n_rec_chunk; ++k) {
-    if (!rec_chunk_near(S, k, cx, cy, cz, reach)) continue;
-    const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
-    for (int j = k * PST_REC_CHUNK; j < j1; ++j) {
-      const double dx = cx - S.rec_pos[3 * j], dy = cy - S.rec_pos[3 * j + 1],
-                   dz = cz - S.rec_pos[3 * j + 2];
-      m2 = fmin(m2, dx * dx + dy * dy + dz * dz);
-    }
+  const double c[3] = {cx, cy, cz};
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = max((int)floor((c[d] - reach - S.cell_o[d]) * S.cell_inv), 0);
+    hi[d] = min((int)floor((c[d] + reach - S.cell_o[d]) * S.cell_inv), S.cell_n[d] - 1);
+    if (lo[d] > hi[d]) return false;
   }
+  double m2 = 1e300;
+  for (int i = lo[0]; i <= hi[0]; ++i)
+    for (int j = lo[1]; j <= hi[1]; ++j) {
+      const int row = (i * S.cell_n[1] + j) * S.cell_n[2];
+      const int a0 = S.cell_start[row + lo[2]], a1 = S.cell_start[row + hi[2] + 1];
+      for (int a = a0; a < a1; ++a) {
+        const double dx = cx - S.cell_pos[3 * a], dy = cy - S.cell_pos[3 * a + 1],
+                     dz = cz - S.cell_pos[3 * a + 2];
+        m2 = fmin(m2, dx * dx + dy * dy + dz * dz);
+      }
+    }
   return sqrt(m2) < P.overlap_thresh;
 }
 
@@ -549,8 +558,9 @@ __device__ inline void wave_append(int* list, int* counter, int v, bool pred) {
 
 // Start of step: stop test, reaction check, escape / return, refill,
 // partition the still-running slots into the grid list and the surrogate list.
-__global__ void k_prod_pre(StepParams P, ProdBufs B, int par) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// Workgroup `blk` of this system's launch.
+__device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int par, int blk) {
+  const int s = blk * blockDim.x + threadIdx.x;
   const RunCounters* C = B.ctr;
   const bool stop = C->stop != 0;
   bool live = false, far = false;
@@ -639,9 +649,28 @@ __global__ void k_prod_pre(StepParams P, ProdBufs B, int par) {
   if (hist[threadIdx.x]) atomicAdd(&B.bins[threadIdx.x], hist[threadIdx.x]);
 }
 
+// Several systems in one launch: workgroup b belongs to system l with
+// start[l] <= b < start[l + 1].
+__device__ inline int slice_of(const int* start, int L, int b) {
+  int l = 0;
+  while (l + 1 < L && b >= start[l + 1]) ++l;
+  return l;
+}
+
+__global__ void __launch_bounds__(kStepBlock) k_prod_pre(StepParams P, ProdBufs B, int par) {
+  prod_pre_body(P, B, par, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kStepBlock) k_prod_pre_group(const StepParams* Pg,
+                                                               const ProdBufs* Bg,
+                                                               const int* start, int L, int par) {
+  const int l = slice_of(start, L, blockIdx.x);
+  prod_pre_body(Pg[l], Bg[l], par, blockIdx.x - start[l]);
+}
+
 // Exclusive scan of the key histogram into scatter cursors; sets the length
 // of the grid-path list and clears the histogram for the next step.
-__global__ void __launch_bounds__(256) k_bins_scan(ProdBufs B) {
+__device__ inline void bins_scan_body(const ProdBufs& B) {
   __shared__ int sc[256];
   const int t = threadIdx.x;
   const int v = B.bins[t];
@@ -658,13 +687,27 @@ __global__ void __launch_bounds__(256) k_bins_scan(ProdBufs B) {
   if (t == 255) B.ctr->n_sr = sc[255];
 }
 
-__global__ void k_bins_scatter(ProdBufs B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_bins_scan(ProdBufs B) { bins_scan_body(B); }
+
+// one workgroup per system
+__global__ void __launch_bounds__(256) k_bins_scan_group(const ProdBufs* Bg) {
+  bins_scan_body(Bg[blockIdx.x]);
+}
+
+__device__ inline void bins_scatter_body(const ProdBufs& B, int blk) {
+  const int s = blk * blockDim.x + threadIdx.x;
   if (s >= B.n_slots) return;
   const int k = B.key[s];
   if (k < 0) return;
   const int p = atomicAdd(&B.bins[256 + k], 1);
   B.plain_list[p] = s;
+}
+
+__global__ void k_bins_scatter(ProdBufs B) { bins_scatter_body(B, blockIdx.x); }
+
+__global__ void k_bins_scatter_group(const ProdBufs* Bg, const int* start, int L) {
+  const int l = slice_of(start, L, blockIdx.x);
+  bins_scatter_body(Bg[l], blockIdx.x - start[l]);
 }
 
 hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st) {
@@ -689,14 +732,14 @@ __device__ inline int obs_bin(double r, double rmax, int nb) {
 // radial transition matrix and pair contact counts (flushed with one global
 // atomic per non-zero bin).
 template <bool OBS>
-__global__ void __launch_bounds__(kStepBlock) k_prod_post(StepParams P, ForceSys S, ProdBufs B,
-                                                          int par) {
+__device__ inline void prod_post_body(const StepParams& P, const ForceSys& S, const ProdBufs& B,
+                                      int par, int blk) {
   __shared__ int h[OBS ? kObsHist : 1];
   if (OBS) {
     for (int i = threadIdx.x; i < kObsHist; i += blockDim.x) h[i] = 0;
     __syncthreads();
   }
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blk * blockDim.x + threadIdx.x;
   const int n_plain = B.ctr->n_sr, n_mlp = B.ctr->n_mlp;
   const bool active = w < n_plain + n_mlp;
   bool any = false;
@@ -781,6 +824,21 @@ __global__ void __launch_bounds__(kStepBlock) k_prod_post(StepParams P, ForceSys
   }
 }
 
+template <bool OBS>
+__global__ void __launch_bounds__(kStepBlock) k_prod_post(StepParams P, ForceSys S, ProdBufs B,
+                                                          int par) {
+  prod_post_body<OBS>(P, S, B, par, blockIdx.x);
+}
+
+template <bool OBS>
+__global__ void __launch_bounds__(kStepBlock) k_prod_post_group(const StepParams* Pg,
+                                                                const ForceSys* Sg,
+                                                                const ProdBufs* Bg,
+                                                                const int* start, int L, int par) {
+  const int l = slice_of(start, L, blockIdx.x);
+  prod_post_body<OBS>(Pg[l], Sg[l], Bg[l], par, blockIdx.x - start[l]);
+}
+
 // Observables of the slots still running (after an early stop): their
 // accumulators go into their trajectory records as they stand.
 __global__ void k_prod_flush_obs(StepParams P, ProdBufs B) {
@@ -797,8 +855,8 @@ hipError_t prod_flush_observables(const StepParams& P, const ProdBufs& B, hipStr
 }
 
 // Bridge decision, n_steps += 1, finish + refill of bridged trajectories.
-__global__ void k_prod_bridge(StepParams P, ProdBufs B, int par) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void prod_bridge_body(const StepParams& P, const ProdBufs& B, int par, int blk) {
+  const int w = blk * blockDim.x + threadIdx.x;
   RunCounters* C = B.ctr;
   const int n_plain = C->n_sr, n_mlp = C->n_mlp;
   if (w == 0) atomicAdd(&C->total_steps, (unsigned long long)(n_plain + n_mlp));
@@ -844,14 +902,32 @@ __global__ void k_prod_bridge(StepParams P, ProdBufs B, int par) {
   }
 }
 
+__global__ void k_prod_bridge(StepParams P, ProdBufs B, int par) {
+  prod_bridge_body(P, B, par, blockIdx.x);
+}
+
+__global__ void k_prod_bridge_group(const StepParams* Pg, const ProdBufs* Bg, const int* start,
+                                    int L, int par) {
+  const int l = slice_of(start, L, blockIdx.x);
+  prod_bridge_body(Pg[l], Bg[l], par, blockIdx.x - start[l]);
+}
+
 // Between steps: decide the early stop (:1347-1348) and reset the per-step
 // counters for the next step.
-__global__ void k_prod_stop(StepParams P, RunCounters* C, int par) {
+__device__ inline void prod_stop_body(const StepParams& P, RunCounters* C, int par) {
   const unsigned long long done = C->n_reacted + C->n_escaped;
   if ((long long)done >= P.stop_done) C->stop = 1;
   C->n_sr = 0;
   C->n_mlp = 0;
   C->any_above[par] = 0;
+}
+
+__global__ void k_prod_stop(StepParams P, RunCounters* C, int par) { prod_stop_body(P, C, par); }
+
+// one thread per system
+__global__ void k_prod_stop_group(const StepParams* Pg, const ProdBufs* Bg, int L, int par) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < L) prod_stop_body(Pg[l], Bg[l].ctr, par);
 }
 
 hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStream_t st) {
@@ -877,6 +953,43 @@ hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs
 hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st) {
   hipLaunchKernelGGL(k_prod_bridge, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
                      dim3(kStepBlock), 0, st, P, B, par);
+  return hipGetLastError();
+}
+
+int prod_step_block() { return kStepBlock; }
+
+// The phases of one step for L systems at once (device arrays of their
+// parameters and buffers; system l owns workgroups start[l] .. start[l+1]-1,
+// ceil(n_slots / prod_step_block()) of them).  Each system's arithmetic is
+// that of its own single-system launches.
+hipError_t prod_group_pre(const StepParams* Pg, const ProdBufs* Bg, const int* start, int L,
+                          int blocks, int par, hipStream_t st) {
+  hipLaunchKernelGGL(k_prod_stop_group, dim3((L + 63) / 64), dim3(64), 0, st, Pg, Bg, L, par);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prod_pre_group, dim3(blocks), dim3(kStepBlock), 0, st, Pg, Bg, start, L, par);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bins_scan_group, dim3(L), dim3(256), 0, st, Bg);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bins_scatter_group, dim3(blocks), dim3(kStepBlock), 0, st, Bg, start, L);
+  return hipGetLastError();
+}
+
+hipError_t prod_group_post(const StepParams* Pg, const ForceSys* Sg, const ProdBufs* Bg,
+                           const int* start, int L, int blocks, bool obs, int par,
+                           hipStream_t st) {
+  if (obs)
+    hipLaunchKernelGGL(k_prod_post_group<true>, dim3(blocks), dim3(kStepBlock), 0, st, Pg, Sg, Bg,
+                       start, L, par);
+  else
+    hipLaunchKernelGGL(k_prod_post_group<false>, dim3(blocks), dim3(kStepBlock), 0, st, Pg, Sg,
+                       Bg, start, L, par);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prod_bridge_group, dim3(blocks), dim3(kStepBlock), 0, st, Pg, Bg, start, L,
+                     par);
   return hipGetLastError();
 }
 

@@ -65,6 +65,14 @@ struct ForceSys {
   // and rec_chunk[4k..4k+3] = (centre x, y, z, radius) bounds all of them
   int n_rec_chunk;
   const double* rec_chunk;  // (n_rec_chunk,4)
+  // the same atoms binned on a uniform grid of cubic cells (edge 1/cell_inv)
+  // for short-range queries: cell (i,j,k) holds cell_pos rows
+  // cell_start[c] .. cell_start[c+1]-1, c = (i*cell_n[1] + j)*cell_n[2] + k
+  int cell_n[3];
+  double cell_o[3];
+  double cell_inv;
+  const int* cell_start;    // (n_cells+1,)
+  const double* cell_pos;   // (Nr,3)
   double rev_radius;        // max(|lo_x|,|hi_x|) of the ligand Born grid (:623)
   double rev_cull;          // exact cull: largest distance from the ligand centre to the valid box
   int torque_mode;          // 0 = reference (zero), 1 = lever arm about charged centroid
