@@ -328,6 +328,7 @@ def multi_ligand_bench(args):
                       "ligands": per}
             log(f"k_on {p}: {len(res)} ligands x {args.kon_trajectories} trajectories in "
                 f"{wall:.1f}s ({tot / wall:.4g} traj-steps/s)")
+            log(f"k_on result {p}: " + json.dumps(kon[p]))
         if "fp64" in kon and "surrogate" in kon:
             kon["surrogate_in_plain_ci95"] = [
                 bool(a["P_ci95"][0] <= b["P"] <= a["P_ci95"][1])
