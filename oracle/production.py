@@ -74,8 +74,10 @@ class ProductionOracle(OracleSim):
     :1185-1234 bridge)."""
 
     def __init__(self, system, params, torque_mode="reference", hydro="scalar",
-                 kT_over_eta=1.0 / 0.243, observe=False):
+                 kT_over_eta=1.0 / 0.243, observe=False, traj_base=0):
         super().__init__(system, params, torque_mode)
+        # random-stream id of trajectory t is traj_base + t (a shard of a run)
+        self.traj_base = int(traj_base)
         self.observe = bool(observe)
         self.hydro = hydro
         self.Df = float(kT_over_eta)
@@ -84,10 +86,10 @@ class ProductionOracle(OracleSim):
         self.max_steps = int(params.get("max_steps", 10 ** 9))
         N = self.N
         t = np.arange(N)
-        z0, z1 = PH.normal2(self.seed, t, INIT_STEP, PH.RNG_INIT, 0)
-        z2, _ = PH.normal2(self.seed, t, INIT_STEP, PH.RNG_INIT, 1)
-        u0, u1 = PH.uniform2(self.seed, t, INIT_STEP, PH.RNG_INIT, 2)
-        u2, _ = PH.uniform2(self.seed, t, INIT_STEP, PH.RNG_INIT, 3)
+        z0, z1 = PH.normal2(self.seed, t + self.traj_base, INIT_STEP, PH.RNG_INIT, 0)
+        z2, _ = PH.normal2(self.seed, t + self.traj_base, INIT_STEP, PH.RNG_INIT, 1)
+        u0, u1 = PH.uniform2(self.seed, t + self.traj_base, INIT_STEP, PH.RNG_INIT, 2)
+        u2, _ = PH.uniform2(self.seed, t + self.traj_base, INIT_STEP, PH.RNG_INIT, 3)
         v = np.stack([z0, z1, z2], axis=1)
         self.pos = v / np.linalg.norm(v, axis=1, keepdims=True) * self.r_b
         self.q = _shoemake(u0, u1, u2)
@@ -191,14 +193,14 @@ class ProductionOracle(OracleSim):
         at = (r >= self.r_esc) & ~fired
         if at.any():
             ia, na = run[at], ns[at]
-            u, _ = PH.uniform2(self.seed, ia, na, PH.RNG_ESC, 0)
+            u, _ = PH.uniform2(self.seed, ia + self.traj_base, na, PH.RNG_ESC, 0)
             ret = u < self.return_prob
             ir, nr_ = ia[ret], na[ret]
             if len(ir):
                 p = self.pos[ir]
                 self.pos[ir] = p / np.linalg.norm(p, axis=1, keepdims=True) * self.r_b
-                a0, a1 = PH.uniform2(self.seed, ir, nr_, PH.RNG_RETQ, 0)
-                a2, _ = PH.uniform2(self.seed, ir, nr_, PH.RNG_RETQ, 1)
+                a0, a1 = PH.uniform2(self.seed, ir + self.traj_base, nr_, PH.RNG_RETQ, 0)
+                a2, _ = PH.uniform2(self.seed, ir + self.traj_base, nr_, PH.RNG_RETQ, 1)
                 self.q[ir] = _shoemake(a0, a1, a2)
                 self.n_returns += len(ir)
                 if self.observe:
@@ -223,16 +225,16 @@ class ProductionOracle(OracleSim):
         if self.hydro == "rpy12":
             xi = np.empty((n, 12))
             for k in range(6):
-                xi[:, 2 * k], xi[:, 2 * k + 1] = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, k)
+                xi[:, 2 * k], xi[:, 2 * k + 1] = PH.normal2(self.seed, run + self.traj_base, ns, PH.RNG_NOISE, k)
             tm = 0 if self.torque_mode == "reference" else 1
             FR, TR = rpy12.receptor_load_v(pos0, Fo, To, tm)
             dX = rpy12.displacement_v(pos0, Fo, To, FR, TR, xi, dt, self.Df, self.a0, self.a1)
             newp, newq = rpy12.move_v(pos0, q0, dX)
         else:
             D = self.diffusion(r)
-            a, b = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, 0)
-            c, e = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, 1)
-            f, g = PH.normal2(self.seed, run, ns, PH.RNG_NOISE, 2)
+            a, b = PH.normal2(self.seed, run + self.traj_base, ns, PH.RNG_NOISE, 0)
+            c, e = PH.normal2(self.seed, run + self.traj_base, ns, PH.RNG_NOISE, 1)
+            f, g = PH.normal2(self.seed, run + self.traj_base, ns, PH.RNG_NOISE, 2)
             nt = np.stack([a, b, c], axis=1)
             nrot = np.stack([e, f, g], axis=1)
             newp = pos0 + (D * dt)[:, None] * Fo + np.sqrt(2.0 * D * dt)[:, None] * nt
@@ -258,7 +260,7 @@ class ProductionOracle(OracleSim):
             x1 = new_d - self.cut[None, :]
             u = np.empty((n, P))
             for p in range(0, P, 2):
-                ua, ub = PH.uniform2(self.seed, run, ns, PH.RNG_BRIDGE, p >> 1)
+                ua, ub = PH.uniform2(self.seed, run + self.traj_base, ns, PH.RNG_BRIDGE, p >> 1)
                 u[:, p] = ua
                 if p + 1 < P:
                     u[:, p + 1] = ub

@@ -23,6 +23,7 @@ class StepParams(C.Structure):
     """Mirror of StepParams in csrc/pst_types.hpp (field order and types)."""
     _fields_ = [
         ("n_total", C.c_int64), ("n_slots", C.c_int64), ("seed", C.c_uint64),
+        ("traj_base", C.c_int64),
         ("r_b", C.c_double), ("r_esc", C.c_double), ("return_prob", C.c_double),
         ("D_t", C.c_double), ("D_r", C.c_double),
         ("use_hi", C.c_int),

@@ -23,6 +23,13 @@ namespace pst {
 
 constexpr int kStepBlock = 256;
 
+// Random-stream id of the run's trajectory t: trajectory ids of a shard run
+// are offset by its first id, so a shard draws exactly what the same
+// trajectories draw in the whole run.
+__device__ inline uint64_t rng_key(const StepParams& P, long long t) {
+  return (uint64_t)(P.traj_base + t);
+}
+
 // ----------------------------------------------------------------------------
 // shared step physics
 // ----------------------------------------------------------------------------
@@ -435,10 +442,10 @@ hipError_t replay_bridge(const StepParams& P, const ReplayBufs& B, const double*
 // orientation (gpu_batch_simulator.py:509-513, Philox stream RNG_INIT).
 __device__ inline void start_traj(const StepParams& P, const ProdBufs& B, int s, long long t) {
   double z0, z1, z2, z3, u0, u1, u2, u3;
-  philox_normal2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 0, z0, z1);
-  philox_normal2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 1, z2, z3);
-  philox_uniform2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 2, u0, u1);
-  philox_uniform2(P.seed, (uint64_t)t, 0xFFFFFFFFu, RNG_INIT, 3, u2, u3);
+  philox_normal2(P.seed, rng_key(P, t), 0xFFFFFFFFu, RNG_INIT, 0, z0, z1);
+  philox_normal2(P.seed, rng_key(P, t), 0xFFFFFFFFu, RNG_INIT, 1, z2, z3);
+  philox_uniform2(P.seed, rng_key(P, t), 0xFFFFFFFFu, RNG_INIT, 2, u0, u1);
+  philox_uniform2(P.seed, rng_key(P, t), 0xFFFFFFFFu, RNG_INIT, 3, u2, u3);
   const double n = norm3(z0, z1, z2);
   double* pos = B.pos + 3 * (size_t)s;
   pos[0] = z0 / n * P.r_b;
@@ -594,14 +601,14 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
       const double r = norm3(pos[0], pos[1], pos[2]);
       if (r >= P.r_esc && !fired) {
         double u, unused;
-        philox_uniform2(P.seed, (uint64_t)t, ns, RNG_ESC, 0, u, unused);
+        philox_uniform2(P.seed, rng_key(P, t), ns, RNG_ESC, 0, u, unused);
         if (u < P.return_prob) {
           pos[0] = pos[0] / r * P.r_b;
           pos[1] = pos[1] / r * P.r_b;
           pos[2] = pos[2] / r * P.r_b;
           double u0, u1, u2, u3;
-          philox_uniform2(P.seed, (uint64_t)t, ns, RNG_RETQ, 0, u0, u1);
-          philox_uniform2(P.seed, (uint64_t)t, ns, RNG_RETQ, 1, u2, u3);
+          philox_uniform2(P.seed, rng_key(P, t), ns, RNG_RETQ, 0, u0, u1);
+          philox_uniform2(P.seed, rng_key(P, t), ns, RNG_RETQ, 1, u2, u3);
           shoemake(u0, u1, u2, q);
           atomicAdd(&B.ctr->n_returns, 1ull);
           if (B.obs.traj) B.obs.s_ret[s] += 1;
@@ -767,14 +774,14 @@ __device__ inline void prod_post_body(const StepParams& P, const ForceSys& S, co
       double xi[12];
 #pragma unroll
       for (int k = 0; k < 6; ++k)
-        philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, (uint32_t)k, xi[2 * k], xi[2 * k + 1]);
+        philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, (uint32_t)k, xi[2 * k], xi[2 * k + 1]);
       any = move_one_rpy(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, xi, dt, x0, x1,
                          &rej);
     } else {
       double nt[3], nr[3];
-      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 0, nt[0], nt[1]);
-      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 1, nt[2], nr[0]);
-      philox_normal2(P.seed, (uint64_t)t, ns, RNG_NOISE, 2, nr[1], nr[2]);
+      philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, 0, nt[0], nt[1]);
+      philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, 1, nt[2], nr[0]);
+      philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, 2, nr[1], nr[2]);
       any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt, D, x0, x1,
                      &rej);
     }
@@ -869,7 +876,7 @@ __device__ inline void prod_bridge_body(const StepParams& P, const ProdBufs& B, 
     double u[PST_MAX_PAIRS];
     for (int p = 0; p < P.n_pairs; p += 2) {
       double a, b;
-      philox_uniform2(P.seed, (uint64_t)t, ns, RNG_BRIDGE, (uint32_t)(p >> 1), a, b);
+      philox_uniform2(P.seed, rng_key(P, t), ns, RNG_BRIDGE, (uint32_t)(p >> 1), a, b);
       u[p] = a;
       if (p + 1 < PST_MAX_PAIRS) u[p + 1] = b;
     }

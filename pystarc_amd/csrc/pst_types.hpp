@@ -109,6 +109,7 @@ struct StepParams {
   int64_t n_total;       // trajectories in the run
   int64_t n_slots;       // concurrent slots
   uint64_t seed;
+  int64_t traj_base;     // random-stream id of the run's trajectory 0 (a shard's first id)
   double r_b, r_esc, return_prob;
   double D_t, D_r;
   int use_hi;

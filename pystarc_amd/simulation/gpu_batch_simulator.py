@@ -43,6 +43,11 @@ Keyword-only extensions (defaults keep the reference's physics):
                kT/eta = 1/0.243 A^3/ps as the reference's HI constant) and its
                Cholesky noise; the pose is then re-expressed in the turned
                receptor frame (philox mode only).
+  traj_range   (first, count): run trajectories first .. first+count-1 of
+               the params' run (philox mode).  Each draws the random numbers
+               it draws in the whole run, so shards of one run (one per GPU,
+               pystarc_amd.pipeline.multi_gpu) reproduce its trajectories;
+               sim_data records are indexed from 0 within the shard.
 """
 from __future__ import annotations
 
@@ -264,9 +269,18 @@ def _shoemake_host(u):
 class GPUBatchSimulator:
     def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
                  n_slots=None, precision=None, graph_steps=16, record=False, surrogate=None,
-                 audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False, gather=None):
+                 audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False, gather=None,
+                 traj_range=None):
         if rng not in ("philox", "numpy"):
             raise ValueError("rng must be 'philox' or 'numpy'")
+        if traj_range is not None:
+            first, count = (int(x) for x in traj_range)
+            if first < 0 or count <= 0:
+                raise ValueError("traj_range needs first >= 0 and count > 0")
+            if rng != "philox":
+                raise ValueError("traj_range needs rng='philox' (replay shards split by seed)")
+            traj_range = (first, count)
+        self.traj_range = traj_range
         if hydro not in ("scalar", "rpy12"):
             raise ValueError("hydro must be 'scalar' or 'rpy12'")
         if hydro == "rpy12" and rng != "philox":
@@ -328,7 +342,7 @@ class GPUBatchSimulator:
             self._rxn_n_needed = 1
         rec_pos = np.asarray(mol1.positions_array(), dtype=np.float64)
         self._rec_gho_pos = rec_pos[rec_idx] if rec_idx else None
-        N = int(params.n_trajectories)
+        N = int(params.n_trajectories) if traj_range is None else traj_range[1]
         self.n_slots = int(n_slots) if n_slots else min(N, 1 << 20)
         if rng == "numpy":
             self.n_slots = N
@@ -376,6 +390,8 @@ class GPUBatchSimulator:
         p = self.params
         P = StepParams()
         N = int(p.n_trajectories)
+        if self.traj_range is not None:
+            P.traj_base, N = self.traj_range
         P.n_total = N
         P.n_slots = self.n_slots
         P.seed = int(p.seed if p.seed is not None else 0) & ((1 << 64) - 1)

@@ -25,6 +25,9 @@ from tests._systems import (dx_grids, numpy_grids, oracle_params, oracle_system,
 def _case(name):
     if name == "two_spheres":
         return two_spheres_small(n_traj=600), "scalar", "reference", 400
+    if name == "two_spheres_shard":
+        # trajectories 200 .. 499 of a run, as one part of a split run
+        return two_spheres_small(n_traj=300), "scalar", "reference", 400
     if name == "protein_ligand_hi":
         s = small_protein_ligand(n_traj=128)
         s.params.hydrodynamic_interactions = True
@@ -38,15 +41,18 @@ def _case(name):
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", ["two_spheres", "protein_ligand_hi", "protein_protein_rpy12",
-                                  "protein_protein_rpy12_zero_torque", "state_machine"])
+@pytest.mark.parametrize("name", ["two_spheres", "two_spheres_shard", "protein_ligand_hi",
+                                  "protein_protein_rpy12", "protein_protein_rpy12_zero_torque",
+                                  "state_machine"])
 def test_production_matches_oracle_per_trajectory(name):
     sysm, hydro, torque, K = _case(name)
     N = int(sysm.params.n_trajectories)
+    base = 200 if name.endswith("_shard") else 0
     g = numpy_grids(sysm)
     eng = S.make_engine(sysm, dx_grids(g), torque_mode=torque)
     sim = GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
-                            n_slots=N, precision="fp64", hydro=hydro)
+                            n_slots=N, precision="fp64", hydro=hydro,
+                            traj_range=(base, N) if base else None)
     drv = sim.production_driver(record=True)
     try:
         k = drv.advance(K)
@@ -62,7 +68,7 @@ def test_production_matches_oracle_per_trajectory(name):
     # batch-wide skip (gpu_batch_engine.py:633-639) compares against the x
     # extent only and depends on which trajectories share a chunk.
     orc = ProductionOracle(oracle_system(sysm, g, chunk=0), oracle_params(sysm),
-                           torque_mode=torque, hydro=hydro)
+                           torque_mode=torque, hydro=hydro, traj_base=base)
     assert abs(orc.return_prob - sim._return_prob) < 1e-12
     for _ in range(k):
         orc.step()
@@ -89,3 +95,26 @@ def test_production_matches_oracle_per_trajectory(name):
         assert [r["n_fired_bridge"] for r in sm] == orc.sm_fired_bridge.tolist()
         assert [r["unique_trajectories"] for r in sm] == [len(x) for x in orc.sm_sets]
         assert sum(r["n_fired"] for r in sm) > 0
+
+
+def test_shards_reproduce_the_whole_run():
+    """Two parts of a run (trajectory ranges, same seed) on the device give
+    each trajectory the outcome and step count it has in the whole run."""
+    sysm = small_protein_ligand(n_traj=500)
+    sysm.params.max_steps = 20000
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+
+    def run(rng_range):
+        return GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params,
+                                 eng, record=True, n_slots=256, traj_range=rng_range).run()
+
+    whole = run(None)
+    parts = [run(r) for r in ((0, 180), (180, 320))]
+    out = np.concatenate([p.sim_data["outcome"] for p in parts])
+    ns = np.concatenate([p.sim_data["n_steps"] for p in parts])
+    wo, wn = whole.sim_data["outcome"], whole.sim_data["n_steps"]
+    done = (wo == 1) | (wo == 2)
+    both = done & ((out == 1) | (out == 2))
+    assert both.sum() >= 0.98 * len(wo)
+    assert np.array_equal(out[both], wo[both]) and np.array_equal(ns[both], wn[both])
+    assert (wo[both] == 1).sum() > 0

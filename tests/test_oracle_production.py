@@ -102,3 +102,31 @@ def test_production_oracle_start_distribution():
     np.testing.assert_allclose(np.linalg.norm(o.q, axis=1), 1.0, rtol=1e-14)
     # Shoemake: uniform rotations -> E[w^2] = 1/4
     assert abs((o.q[:, 0] ** 2).mean() - 0.25) < 0.01
+
+
+def test_production_oracle_shards_reproduce_the_whole_run():
+    """Trajectories first .. first+count-1 run as a shard (random keys offset
+    by first) are those trajectories of the whole run: same poses, outcomes
+    and step counts after the same number of steps."""
+    whole_sys = two_spheres_small(n_traj=60)
+    g = numpy_grids(whole_sys)
+    whole = ProductionOracle(oracle_system(whole_sys, g), oracle_params(whole_sys))
+    parts = []
+    for first, count in ((0, 25), (25, 35)):
+        s = two_spheres_small(n_traj=count)
+        parts.append(ProductionOracle(oracle_system(s, g), oracle_params(s), traj_base=first))
+    for _ in range(120):
+        whole.step()
+        for p in parts:
+            p.step()
+    pos = np.concatenate([p.pos for p in parts])
+    q = np.concatenate([p.q for p in parts])
+    st = np.concatenate([p.status for p in parts])
+    ns = np.concatenate([p.out_steps for p in parts])
+    assert np.array_equal(pos, whole.pos) and np.array_equal(q, whole.q)
+    assert np.array_equal(st, whole.status) and np.array_equal(ns, whole.out_steps)
+    assert (whole.status != 0).sum() > 0
+    # and a shard is not simply the whole run's first trajectories re-labelled
+    s = two_spheres_small(n_traj=35)
+    unshifted = ProductionOracle(oracle_system(s, g), oracle_params(s))
+    assert not np.array_equal(unshifted.pos, whole.pos[25:])
