@@ -74,8 +74,11 @@ class ProductionOracle(OracleSim):
     :1185-1234 bridge)."""
 
     def __init__(self, system, params, torque_mode="reference", hydro="scalar",
-                 kT_over_eta=1.0 / 0.243, observe=False, traj_base=0):
+                 kT_over_eta=1.0 / 0.243, observe=False, traj_base=0, outer=None):
         super().__init__(system, params, torque_mode)
+        # LMZ outer propagator (oracle/outer.OuterModel) replacing the return
+        # probability at r_esc: trajectories at r >= 1.1 b take an excursion
+        self.outer = outer
         # random-stream id of trajectory t is traj_base + t (a shard of a run)
         self.traj_base = int(traj_base)
         self.observe = bool(observe)
@@ -190,7 +193,26 @@ class ProductionOracle(OracleSim):
         # escape / return (a trajectory that fired a reaction this step is not
         # tested, :879-884)
         r = np.linalg.norm(self.pos[run], axis=1)
-        at = (r >= self.r_esc) & ~fired
+        if self.outer is not None:
+            from .outer import QB_FACTOR, Draws, propagate
+            at = (r >= QB_FACTOR * self.r_b) & ~fired
+            keep = np.ones(len(run), dtype=bool)
+            for k in np.where(at)[0]:
+                i = run[k]
+                dr = Draws(self.seed, int(i) + self.traj_base, int(ns[k]))
+                back, p, qq = propagate(self.outer, dr, self.pos[i], self.q[i])
+                if back:
+                    self.pos[i], self.q[i] = p, qq
+                    self.n_returns += 1
+                    if self.observe:
+                        self.obs_traj[i, 17] += 1
+                else:
+                    self._finish(np.array([i]), 2, np.array([ns[k]]))
+                    keep[k] = False
+            run, ns = run[keep], ns[keep]
+            at = np.zeros(len(run), dtype=bool)
+        else:
+            at = (r >= self.r_esc) & ~fired
         if at.any():
             ia, na = run[at], ns[at]
             u, _ = PH.uniform2(self.seed, ia + self.traj_base, na, PH.RNG_ESC, 0)

@@ -48,6 +48,12 @@ class StepParams(C.Structure):
         ("sm_after", C.c_int * MAX_PAIRS),
         ("sm_terminal", C.c_int),
         ("min_core_rxn_dt", C.c_double),
+        ("lmz_on", C.c_int), ("lmz_hi", C.c_int),
+        ("lmz_trigger", C.c_double), ("lmz_b", C.c_double), ("lmz_q", C.c_double),
+        ("lmz_b_cover", C.c_double), ("lmz_q_cover", C.c_double),
+        ("lmz_V", C.c_double), ("lmz_L", C.c_double), ("lmz_return_q", C.c_double),
+        ("lmz_Dt", C.c_double), ("lmz_dpre", C.c_double), ("lmz_ainv", C.c_double),
+        ("lmz_a2", C.c_double), ("lmz_drot0", C.c_double), ("lmz_drot1", C.c_double),
     ]
 
 

@@ -104,6 +104,7 @@ struct pst_sim {
   ObsBufs obs{};                     // observables (record level 2)
   int* sm_state = nullptr;           // state-machine reactions
   unsigned int* sm_mask = nullptr;
+  int* lmz_escaped = nullptr;        // outer-propagator escapes of the step
   std::vector<void*> obs_allocs;
   double last_ms_per_step = 0;
   bool prod_ready = false;
@@ -883,6 +884,12 @@ int pst_sim_create(pst_ctx* ctx, const void* params, int precision, pst_sim** ou
   if (P->hydro_mode != 0 && P->hydro_mode != 1) FAIL("pst_sim_create: hydro_mode must be 0 or 1");
   if (P->hydro_mode == 1 && !(P->a_rec > 0 && P->a_lig > 0 && P->kT_over_eta > 0))
     FAIL("pst_sim_create: hydro_mode 1 needs a_rec, a_lig and kT_over_eta > 0");
+  if (P->lmz_on &&
+      !(P->lmz_b > 0 && P->lmz_q > P->lmz_trigger && P->lmz_trigger >= P->lmz_b &&
+        P->lmz_b_cover > P->lmz_b && P->lmz_q_cover < P->lmz_q && P->lmz_L > 0 && P->lmz_Dt > 0 &&
+        (!P->lmz_hi || (P->lmz_dpre > 0 && P->lmz_ainv > 0))))
+    FAIL("pst_sim_create: inconsistent outer-propagator parameters (need 0 < b <= trigger < q, "
+         "covers inside (b, q), Debye length and D_t > 0)");
   if (P->sm_active) {
     for (int r = 0; r < P->n_pairs; ++r)
       if (P->sm_before[r] < 0 || P->sm_before[r] >= 32 || P->sm_after[r] < -1 || P->sm_after[r] >= 32)
@@ -917,7 +924,8 @@ int pst_sim_destroy(pst_sim* s) {
   void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
                   s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
-                  s->audit_F, s->key, s->bins, s->census, s->sm_state, s->sm_mask};
+                  s->audit_F, s->key, s->bins, s->census, s->sm_state, s->sm_mask,
+                  s->lmz_escaped};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (void* p : s->obs_allocs) hipFree(p);
@@ -1007,6 +1015,7 @@ int pst_replay_begin(pst_sim* s, int* n_run, int* n_esc, int* n_react) {
   pst_ctx* ctx = s->ctx;
   if (s->P.hydro_mode != 0) FAIL("pst_replay_begin: replay follows the scalar diffusion (hydro_mode 0)");
   if (s->P.sm_active) FAIL("pst_replay_begin: state-machine reactions run in production mode");
+  if (s->P.lmz_on) FAIL("pst_replay_begin: the outer propagator runs in production mode");
   CK(set_dev(ctx));
   if (ensure_replay(s)) return 2;
   ReplayBufs B = replay_bufs(s);
@@ -1143,6 +1152,7 @@ static ProdBufs prod_bufs(pst_sim* s) {
   B.obs = s->obs;
   B.sm_state = s->sm_state;
   B.sm_mask = s->sm_mask;
+  B.lmz_escaped = s->lmz_escaped;
   B.n_slots = s->S;
   return B;
 }
@@ -1168,8 +1178,10 @@ int pst_prod_init(pst_sim* s, int record) {
     CK(hipMalloc(&s->bins, 512 * 4));
     CK(hipMalloc(&s->sm_state, S * 4));
     CK(hipMalloc(&s->sm_mask, S * 4));
+    CK(hipMalloc(&s->lmz_escaped, S * 4));
   }
   CK(hipMemsetAsync(s->bins, 0, 512 * 4, ctx->stream));
+  CK(hipMemsetAsync(s->lmz_escaped, 0, S * 4, ctx->stream));
   if (s->out_status) { hipFree(s->out_status); s->out_status = nullptr; }
   if (s->out_steps) { hipFree(s->out_steps); s->out_steps = nullptr; }
   for (void* p : s->obs_allocs) hipFree(p);
@@ -1557,7 +1569,9 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
   int G = 16;
   size_t lds = 0;
   const int sb = prod_step_block();
+  bool any_outer = false;
   for (int l = 0; l < L; ++l) {
+    any_outer = any_outer || sims[l]->P.lmz_on;
     fill_sys(sims[l]->ctx, S[l]);
     P[l] = sims[l]->P;
     B[l] = prod_bufs(sims[l]);
@@ -1618,7 +1632,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
   // surrogate phase (own network per system) one set of launches per system
   auto step = [&](int par) -> int {
     const hipStream_t cs = sims[0]->ctx->stream;
-    hipError_t e = prod_group_pre(dP, dB, dsstart, L, sstart[L], par, cs);
+    hipError_t e = prod_group_pre(dP, dB, dsstart, L, sstart[L], par, any_outer, cs);
     if (e == hipSuccess)
       e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }

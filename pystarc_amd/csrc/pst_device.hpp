@@ -32,7 +32,7 @@ __host__ __device__ inline PhiloxOut philox4x32_10(uint32_t c0, uint32_t c1, uin
   return o;
 }
 
-enum { RNG_INIT = 1, RNG_ESC = 2, RNG_RETQ = 3, RNG_NOISE = 4, RNG_BRIDGE = 5 };
+enum { RNG_INIT = 1, RNG_ESC = 2, RNG_RETQ = 3, RNG_NOISE = 4, RNG_BRIDGE = 5, RNG_OUTER = 6 };
 
 // Force-kernel geometry shared by the direct and the binned gathers.
 constexpr int kForceBlock = 256;

@@ -103,6 +103,7 @@ struct ProdBufs {
   ObsBufs obs;                 // observables (obs.traj null: off)
   int* sm_state;               // per-slot reaction state (state-machine mode)
   unsigned int* sm_mask;       // per-slot reactions fired so far (bit r)
+  int* lmz_escaped;            // per slot: 1 when this step's outer excursion escaped
   int n_slots;
 };
 hipError_t prod_flush_observables(const StepParams& P, const ProdBufs& B, hipStream_t st);
@@ -114,7 +115,7 @@ hipError_t prod_step_post(const StepParams& P, const ForceSys& S, const ProdBufs
 hipError_t prod_step_bridge(const StepParams& P, const ProdBufs& B, int par, hipStream_t st);
 int prod_step_block();
 hipError_t prod_group_pre(const StepParams* Pg, const ProdBufs* Bg, const int* start, int L,
-                          int blocks, int par, hipStream_t st);
+                          int blocks, int par, bool outer, hipStream_t st);
 hipError_t prod_group_post(const StepParams* Pg, const ForceSys* Sg, const ProdBufs* Bg,
                            const int* start, int L, int blocks, bool obs, int par,
                            hipStream_t st);

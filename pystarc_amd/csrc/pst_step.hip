@@ -18,6 +18,7 @@
 #include "pst_device.hpp"
 #include "pst_kernels.hpp"
 #include "pst_rpy.hpp"
+#include "pst_outer.hpp"
 
 namespace pst {
 
@@ -599,7 +600,16 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
         continue;
       }
       const double r = norm3(pos[0], pos[1], pos[2]);
-      if (r >= P.r_esc && !fired) {
+      if (P.lmz_on) {
+        // the outer excursion ran in k_prod_outer this step (returned: the
+        // pose is already back on the b-sphere)
+        if (B.lmz_escaped[s]) {
+          B.lmz_escaped[s] = 0;
+          finish_traj(P, B, s, PST_ESCAPED);
+          if (B.status[s] != PST_RUNNING) break;
+          continue;
+        }
+      } else if (r >= P.r_esc && !fired) {
         double u, unused;
         philox_uniform2(P.seed, rng_key(P, t), ns, RNG_ESC, 0, u, unused);
         if (u < P.return_prob) {
@@ -656,6 +666,38 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
   if (hist[threadIdx.x]) atomicAdd(&B.bins[threadIdx.x], hist[threadIdx.x]);
 }
 
+// Outer excursions of the step (LMZ mode), before k_prod_pre: a running
+// trajectory that k_prod_pre would take to its escape test (no max-steps
+// end, no reaction at the check) and that sits at r >= 1.1 b is carried
+// through the outer region (nam_simulator.py:354-373).  Back on the b-sphere
+// its pose is updated; an escape is left for k_prod_pre to finish (and refill)
+// in the same step.  Its own kernel, so that the propagator's registers and
+// stack stay out of k_prod_pre.
+__device__ inline void prod_outer_body(const StepParams& P, const ProdBufs& B, int blk) {
+  const int s = blk * blockDim.x + threadIdx.x;
+  if (!P.lmz_on || s >= B.n_slots || B.ctr->stop != 0 || B.status[s] != PST_RUNNING) return;
+  const unsigned int ns = B.n_steps[s];
+  if ((long long)ns >= P.max_steps) return;
+  double* pos = B.pos + 3 * (size_t)s;
+  double* q = B.quat + 4 * (size_t)s;
+  if (P.sm_active) {
+    if (sm_check(P, pos, q, B.sm_state[s]) >= 0) return;
+  } else if (reacted_at(P, pos, q)) {
+    return;
+  }
+  if (norm3(pos[0], pos[1], pos[2]) < P.lmz_trigger) return;
+  if (outer_propagate(P, rng_key(P, B.traj[s]), ns, pos, q)) {
+    atomicAdd(&B.ctr->n_returns, 1ull);
+    if (B.obs.traj) B.obs.s_ret[s] += 1;
+  } else {
+    B.lmz_escaped[s] = 1;
+  }
+}
+
+__global__ void __launch_bounds__(kStepBlock) k_prod_outer(StepParams P, ProdBufs B) {
+  prod_outer_body(P, B, blockIdx.x);
+}
+
 // Several systems in one launch: workgroup b belongs to system l with
 // start[l] <= b < start[l + 1].
 __device__ inline int slice_of(const int* start, int L, int b) {
@@ -666,6 +708,13 @@ __device__ inline int slice_of(const int* start, int L, int b) {
 
 __global__ void __launch_bounds__(kStepBlock) k_prod_pre(StepParams P, ProdBufs B, int par) {
   prod_pre_body(P, B, par, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kStepBlock) k_prod_outer_group(const StepParams* Pg,
+                                                                 const ProdBufs* Bg,
+                                                                 const int* start, int L) {
+  const int l = slice_of(start, L, blockIdx.x);
+  prod_outer_body(Pg[l], Bg[l], blockIdx.x - start[l]);
 }
 
 __global__ void __launch_bounds__(kStepBlock) k_prod_pre_group(const StepParams* Pg,
@@ -941,6 +990,12 @@ hipError_t prod_step_pre(const StepParams& P, const ProdBufs& B, int par, hipStr
   hipLaunchKernelGGL(k_prod_stop, dim3(1), dim3(1), 0, st, P, B.ctr, par);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (P.lmz_on) {
+    hipLaunchKernelGGL(k_prod_outer, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
+                       dim3(kStepBlock), 0, st, P, B);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_prod_pre, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
                      dim3(kStepBlock), 0, st, P, B, par);
   return hipGetLastError();
@@ -970,10 +1025,16 @@ int prod_step_block() { return kStepBlock; }
 // ceil(n_slots / prod_step_block()) of them).  Each system's arithmetic is
 // that of its own single-system launches.
 hipError_t prod_group_pre(const StepParams* Pg, const ProdBufs* Bg, const int* start, int L,
-                          int blocks, int par, hipStream_t st) {
+                          int blocks, int par, bool outer, hipStream_t st) {
   hipLaunchKernelGGL(k_prod_stop_group, dim3((L + 63) / 64), dim3(64), 0, st, Pg, Bg, L, par);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (outer) {
+    hipLaunchKernelGGL(k_prod_outer_group, dim3(blocks), dim3(kStepBlock), 0, st, Pg, Bg, start,
+                       L);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_prod_pre_group, dim3(blocks), dim3(kStepBlock), 0, st, Pg, Bg, start, L, par);
   e = hipGetLastError();
   if (e != hipSuccess) return e;

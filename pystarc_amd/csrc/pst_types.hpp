@@ -137,6 +137,15 @@ struct StepParams {
   int sm_after[PST_MAX_PAIRS];
   int sm_terminal;       // bit mask over state indices (< 32 states)
   double min_core_rxn_dt;
+  // LMZ outer propagator (outer_propagator.py:259-401): a trajectory at
+  // r >= lmz_trigger is carried through b < r < q analytically, returning
+  // to the b-sphere or escaping (instead of the return probability at r_esc)
+  int lmz_on, lmz_hi;
+  double lmz_trigger, lmz_b, lmz_q, lmz_b_cover, lmz_q_cover;
+  double lmz_V, lmz_L;          // U/kT = lmz_V e^{-r/lmz_L} / r
+  double lmz_return_q;          // rate(b) / rate(q)
+  double lmz_Dt, lmz_dpre, lmz_ainv, lmz_a2;
+  double lmz_drot0, lmz_drot1;  // rotational diffusion of receptor and ligand
 };
 
 // Device-side run counters (production mode).

@@ -48,6 +48,12 @@ Keyword-only extensions (defaults keep the reference's physics):
                it draws in the whole run, so shards of one run (one per GPU,
                pystarc_amd.pipeline.multi_gpu) reproduce its trajectories;
                sim_data records are indexed from 0 within the shard.
+  outer        "return_prob" (default): the reference's escape test at
+               r_escape with the LMZ return probability.  "lmz": a trajectory
+               reaching 1.1 b is carried through the outer region (up to the
+               q-sphere at 20 x the larger molecular radius) by the LMZ outer
+               propagator of the reference's CPU simulator on the device,
+               returning to the b-sphere or escaping (philox mode).
 """
 from __future__ import annotations
 
@@ -270,7 +276,7 @@ class GPUBatchSimulator:
     def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
                  n_slots=None, precision=None, graph_steps=16, record=False, surrogate=None,
                  audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False, gather=None,
-                 traj_range=None):
+                 traj_range=None, outer="return_prob"):
         if rng not in ("philox", "numpy"):
             raise ValueError("rng must be 'philox' or 'numpy'")
         if traj_range is not None:
@@ -281,6 +287,12 @@ class GPUBatchSimulator:
                 raise ValueError("traj_range needs rng='philox' (replay shards split by seed)")
             traj_range = (first, count)
         self.traj_range = traj_range
+        if outer not in ("return_prob", "lmz"):
+            raise ValueError("outer must be 'return_prob' or 'lmz'")
+        if outer == "lmz" and rng != "philox":
+            raise ValueError("outer='lmz' needs rng='philox'")
+        self.outer = outer
+        self.outer_propagator = None
         if hydro not in ("scalar", "rpy12"):
             raise ValueError("hydro must be 'scalar' or 'rpy12'")
         if hydro == "rpy12" and rng != "philox":
@@ -350,6 +362,19 @@ class GPUBatchSimulator:
         self._k_b = 0.0
 
     # -- outer propagator: k_b and return probability (:320-404) ----------------
+    def _make_outer(self):
+        from .outer_propagator import OuterPropagator
+        q_rec, q_lig = float(self.mol1.total_charge()), float(self.mol2.total_charge())
+        kT_scale = float(getattr(self.params, "_kT_scale", 1.0))
+        V = q_rec * q_lig / (4.0 * math.pi * 78.0 * EPS0) / kT_scale
+        return OuterPropagator(
+            float(self.params.r_start),
+            max(self.mol1.bounding_radius(), self.mol2.bounding_radius()), V,
+            float(getattr(self.engine, "_debye", 7.858)),
+            float(self.mob.relative_translational_diffusion()), self._use_hi,
+            float(getattr(self.mob, "radius1", 0.0)), float(getattr(self.mob, "radius2", 0.0)),
+            float(self.mob.D_rot1), float(self.mob.D_rot2))
+
     def _compute_return_prob(self) -> float:
         b = float(self.params.r_start)
         q_out = float(self.params.r_escape)
@@ -392,6 +417,9 @@ class GPUBatchSimulator:
         N = int(p.n_trajectories)
         if self.traj_range is not None:
             P.traj_base, N = self.traj_range
+        if self.outer == "lmz":
+            self.outer_propagator = self._make_outer()
+            self.outer_propagator.fill(P)
         P.n_total = N
         P.n_slots = self.n_slots
         P.seed = int(p.seed if p.seed is not None else 0) & ((1 << 64) - 1)

@@ -64,6 +64,13 @@ class Molecule:
     def total_charge(self) -> float:
         return float(sum(a.charge for a in self.atoms))
 
+    def bounding_radius(self) -> float:
+        """Largest distance from the centroid to an atom's surface (molecules.py:79-84)."""
+        if not self.atoms:
+            return 0.0
+        d = np.linalg.norm(self.positions_array() - self.centroid(), axis=1)
+        return float((d + self.radii_array()).max())
+
 
 @dataclass
 class ContactPair:
