@@ -64,7 +64,7 @@ def make_sim(sysm, eng, args, surrogate=None, n_slots=None, autotune=False):
     return GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, eng,
                              n_slots=n_slots or args.slots, graph_steps=args.graph_steps,
                              surrogate=surrogate, audit_n=args.audit_n, audit_tol=args.audit_tol,
-                             hydro=args.hydro, autotune=autotune)
+                             hydro=args.hydro, autotune=autotune, outer=args.outer)
 
 
 def run_window(sysm, eng, args, surrogate=None):
@@ -377,6 +377,9 @@ def main():
     ap.add_argument("--pmc", action="store_true", help="counter-collection mode (no JSON line)")
     ap.add_argument("--hydro", default="rpy12", choices=("rpy12", "scalar"),
                     help="two-body RPY 12x12 mobility (config default) or scalar D_parallel")
+    ap.add_argument("--outer", default="return_prob", choices=("return_prob", "lmz"),
+                    help="escape handling: the reference's return probability at r_esc, or "
+                         "the LMZ outer propagator from 1.1 b")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the heuristic force launch instead of tuning it in the warm-up")
     ap.add_argument("--workload", default="protein_protein", choices=("protein_protein", "multi_ligand"),
@@ -487,7 +490,7 @@ def main():
                                   "Cholesky noise" if args.hydro == "rpy12" else "scalar HI")
                                + ", b=60 A, 2-of-5 contacts < 5.5 A",
                    "path": head, "slots": args.slots, "graph_steps": args.graph_steps,
-                   "hydro": args.hydro, "launch": best.get("launch")},
+                   "hydro": args.hydro, "outer": args.outer, "launch": best.get("launch")},
         "speedup_vs_plain_fp64": (best["value"] / plain["value"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
