@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(256) k_bin_gather(ForceSys S, ForceWork W, Bin
       const int a = e / (kTile * kTile), b = (e / kTile) % kTile, c = e % kTile;
       const int i = ti + a, j = tj + b, k = tk + c;
       if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz) {
-        const long gi = ((long)i * g.ny + j) * g.nz + k;
+        const long gi = global_acc(g).at(i, j, k);
         if (rev)
           ((float*)tile)[e] = ((const float*)g.data)[gi];
         else

@@ -386,7 +386,7 @@ static int update_pair(pst_ctx* ctx) {
   if (a.dtype != 0 || b.dtype != 0 || a.nx != b.nx || a.ny != b.ny || a.nz != b.nz) return 0;
   for (int i = 0; i < 3; ++i)
     if (a.o[i] != b.o[i] || a.sp[i] != b.sp[i]) return 0;
-  const long n = (long)a.nx * a.ny * a.nz;
+  const long n = brick_nodes(a.nx, a.ny, a.nz);  // bricked layout, padding included
   CK(hipMalloc(&ctx->pair.mem, (size_t)n * sizeof(float2)));
   CK(launch_interleave((const float*)a.data, (const float*)b.data, (float2*)ctx->pair.mem, n,
                        ctx->stream));
@@ -410,13 +410,21 @@ int pst_add_grid(pst_ctx* ctx, int kind, const void* host_data, int dtype, int n
     if (!(spacing3[i] > 0)) FAIL("pst_add_grid: spacing must be positive");
   CK(set_dev(ctx));
   DevGrid d;
+  // uploaded in C order, stored in the kernels' bricked layout (zero padding)
+  const int elem = dtype == PST_F64 ? 8 : 4;
   const size_t count = (size_t)nx * ny * nz;
-  const size_t bytes = count * (dtype == PST_F64 ? 8 : 4);
+  const size_t bytes = (size_t)brick_nodes(nx, ny, nz) * elem;
+  void* tmp = nullptr;
   CK(hipMalloc(&d.mem, bytes));
-  hipError_t e = hipMemcpy(d.mem, host_data, bytes, hipMemcpyHostToDevice);
+  hipError_t e = hipMemset(d.mem, 0, bytes);
+  if (e == hipSuccess) e = hipMalloc(&tmp, count * elem);
+  if (e == hipSuccess) e = hipMemcpy(tmp, host_data, count * elem, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_to_bricks(tmp, d.mem, elem, nx, ny, nz, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (tmp) hipFree(tmp);
   if (e != hipSuccess) {
     hipFree(d.mem);
-    ctx->err = std::string("pst_add_grid copy: ") + hipGetErrorString(e);
+    ctx->err = std::string("pst_add_grid upload: ") + hipGetErrorString(e);
     return 2;
   }
   d.g.data = d.mem;

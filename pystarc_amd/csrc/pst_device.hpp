@@ -155,20 +155,34 @@ template <> __device__ inline Pair<double> gld<Pair<double>, float2>(const float
 
 // Node addressing.  The interpolation arithmetic always works in the grid's
 // own index space (cell i0 = floor((p - o) * inv)); an accessor turns node
-// (i, j, k) into an offset into the array it reads: the whole grid in HBM, or
-// a tile of it staged in LDS whose first node is (ti, tj, tk).
+// (i, j, k) into an offset ox(i) + oy(j) + oz(k) into the array it reads: the
+// whole grid in HBM, or a tile of it staged in LDS whose first node is
+// (ti, tj, tk).
+//
+// In HBM every grid is stored in bricks of 4 x 4 x 4 nodes: bricks in C
+// order, nodes z-fastest inside a brick (one x-slice of a brick is 16 nodes,
+// 64 B of FP32 or 128 B of float2).  The 20 nodes of one atom's stencil then
+// sit in ~3-5 cache lines instead of the ~8-9 of a C-order array, where
+// every (i, j) column of the stencil is a line of its own.
 struct GlobalAcc {
-  long sx, sy;
-  __device__ long at(int i, int j, int k) const { return (long)i * sx + (long)j * sy + k; }
+  long bx, by;  // nodes per brick step along x and along y
+  __device__ long ox(int i) const { return (long)(i >> 2) * bx + ((i & 3) << 4); }
+  __device__ long oy(int j) const { return (long)(j >> 2) * by + ((j & 3) << 2); }
+  __device__ long oz(int k) const { return ((long)(k >> 2) << 6) + (k & 3); }
+  __device__ long at(int i, int j, int k) const { return ox(i) + oy(j) + oz(k); }
 };
 struct TileAcc {
   long sx, sy;
   int ti, tj, tk;
-  __device__ long at(int i, int j, int k) const {
-    return (long)(i - ti) * sx + (long)(j - tj) * sy + (k - tk);
-  }
+  __device__ long ox(int i) const { return (long)(i - ti) * sx; }
+  __device__ long oy(int j) const { return (long)(j - tj) * sy; }
+  __device__ long oz(int k) const { return (long)(k - tk); }
+  __device__ long at(int i, int j, int k) const { return ox(i) + oy(j) + oz(k); }
 };
-__device__ inline GlobalAcc global_acc(const GridDev& g) { return {(long)g.ny * g.nz, (long)g.nz}; }
+__device__ inline GlobalAcc global_acc(const GridDev& g) {
+  const long nbz = (g.nz + 3) >> 2, nby = (g.ny + 3) >> 2;
+  return {nby * nbz * 64, nbz * 64};
+}
 
 template <typename TG, typename TC, typename V, typename A>
 __device__ inline V grid_interp_a(const GridDev& g, const TG* __restrict__ d, const A& a, double x,
@@ -181,12 +195,12 @@ __device__ inline V grid_interp_a(const GridDev& g, const TG* __restrict__ d, co
   if (i0 < 0 || i0 >= g.nx - 1 || j0 < 0 || j0 >= g.ny - 1 || k0 < 0 || k0 >= g.nz - 1)
     return V{};
   const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
-  const long sxy = a.sx, sz = a.sy;
-  const long base = a.at(i0, j0, k0);
-  const V c000 = gld<V>(d, base), c001 = gld<V>(d, base + 1);
-  const V c010 = gld<V>(d, base + sz), c011 = gld<V>(d, base + sz + 1);
-  const V c100 = gld<V>(d, base + sxy), c101 = gld<V>(d, base + sxy + 1);
-  const V c110 = gld<V>(d, base + sxy + sz), c111 = gld<V>(d, base + sxy + sz + 1);
+  const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Y0 = a.oy(j0), Y1 = a.oy(j0 + 1);
+  const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1);
+  const V c000 = gld<V>(d, X0 + Y0 + Z0), c001 = gld<V>(d, X0 + Y0 + Z1);
+  const V c010 = gld<V>(d, X0 + Y1 + Z0), c011 = gld<V>(d, X0 + Y1 + Z1);
+  const V c100 = gld<V>(d, X1 + Y0 + Z0), c101 = gld<V>(d, X1 + Y0 + Z1);
+  const V c110 = gld<V>(d, X1 + Y1 + Z0), c111 = gld<V>(d, X1 + Y1 + Z1);
   const TC gx = (TC)1 - fx, gy = (TC)1 - fy, gz = (TC)1 - fz;
   return c000 * gx * gy * gz + c100 * fx * gy * gz + c010 * gx * fy * gz + c001 * gx * gy * fz +
          c110 * fx * fy * gz + c101 * fx * gy * fz + c011 * gx * fy * fz + c111 * fx * fy * fz;
@@ -248,21 +262,22 @@ __device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __res
   const double mi = floor(ix - 0.5), mj = floor(iy - 0.5), mk = floor(iz - 0.5);
   const TC ex = (TC)(ix - 0.5 - mi), ey = (TC)(iy - 0.5 - mj), ez = (TC)(iz - 0.5 - mk);
   const int ai = (int)mi - i0, aj = (int)mj - j0, ak = (int)mk - k0;  // -1 or 0
-  const long sy = a.sy, sx = a.sx;
-  const long b = a.at(i0, j0, k0);
+  // per-axis offsets of the centre cell's two planes and of the extra plane
+  // (index -1 when a = -1, +2 when a = 0)
+  const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Xe = a.ox(ai < 0 ? i0 - 1 : i0 + 2);
+  const long Y0 = a.oy(j0), Y1 = a.oy(j0 + 1), Ye = a.oy(aj < 0 ? j0 - 1 : j0 + 2);
+  const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1), Ze = a.oz(ak < 0 ? k0 - 1 : k0 + 2);
   // centre cell
-  const V c000 = gld<V>(d, b), c001 = gld<V>(d, b + 1), c010 = gld<V>(d, b + sy),
-          c011 = gld<V>(d, b + sy + 1);
-  const V c100 = gld<V>(d, b + sx), c101 = gld<V>(d, b + sx + 1), c110 = gld<V>(d, b + sx + sy),
-          c111 = gld<V>(d, b + sx + sy + 1);
-  // extra plane per axis: index -1 (a = -1) or +2 (a = 0)
-  const long xo = ai < 0 ? -sx : 2 * sx, yo = aj < 0 ? -sy : 2 * sy, zo = ak < 0 ? -1 : 2;
-  const V x00 = gld<V>(d, b + xo), x01 = gld<V>(d, b + xo + 1), x10 = gld<V>(d, b + xo + sy),
-          x11 = gld<V>(d, b + xo + sy + 1);
-  const V y00 = gld<V>(d, b + yo), y01 = gld<V>(d, b + yo + 1), y10 = gld<V>(d, b + yo + sx),
-          y11 = gld<V>(d, b + yo + sx + 1);
-  const V z00 = gld<V>(d, b + zo), z01 = gld<V>(d, b + zo + sy), z10 = gld<V>(d, b + zo + sx),
-          z11 = gld<V>(d, b + zo + sx + sy);
+  const V c000 = gld<V>(d, X0 + Y0 + Z0), c001 = gld<V>(d, X0 + Y0 + Z1),
+          c010 = gld<V>(d, X0 + Y1 + Z0), c011 = gld<V>(d, X0 + Y1 + Z1);
+  const V c100 = gld<V>(d, X1 + Y0 + Z0), c101 = gld<V>(d, X1 + Y0 + Z1),
+          c110 = gld<V>(d, X1 + Y1 + Z0), c111 = gld<V>(d, X1 + Y1 + Z1);
+  const V x00 = gld<V>(d, Xe + Y0 + Z0), x01 = gld<V>(d, Xe + Y0 + Z1),
+          x10 = gld<V>(d, Xe + Y1 + Z0), x11 = gld<V>(d, Xe + Y1 + Z1);
+  const V y00 = gld<V>(d, X0 + Ye + Z0), y01 = gld<V>(d, X0 + Ye + Z1),
+          y10 = gld<V>(d, X1 + Ye + Z0), y11 = gld<V>(d, X1 + Ye + Z1);
+  const V z00 = gld<V>(d, X0 + Y0 + Ze), z01 = gld<V>(d, X0 + Y1 + Ze),
+          z10 = gld<V>(d, X1 + Y0 + Ze), z11 = gld<V>(d, X1 + Y1 + Ze);
   const TC one = (TC)1;
   const TC gxw = one - fx, gyw = one - fy, gzw = one - fz;
   phi = c000 * gxw * gyw * gzw + c100 * fx * gyw * gzw + c010 * gxw * fy * gzw +

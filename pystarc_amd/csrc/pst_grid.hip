@@ -66,6 +66,35 @@ __global__ void k_interleave(const float* a, const float* b, float2* out, long n
   if (i < n) out[i] = make_float2(a[i], b[i]);
 }
 
+// C-order grid (z fastest) -> the bricked layout of the force kernels
+// (bricks of 4x4x4 nodes in C order, z fastest inside a brick).
+template <typename T>
+__global__ void k_to_bricks(const T* src, T* dst, int nx, int ny, int nz) {
+  const long n = (long)nx * ny * nz;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int k = (int)(idx % nz);
+  const long r = idx / nz;
+  const int j = (int)(r % ny), i = (int)(r / ny);
+  const long nbz = (nz + 3) >> 2, nby = (ny + 3) >> 2;
+  const long o = (long)(i >> 2) * nby * nbz * 64 + (long)(j >> 2) * nbz * 64 + ((long)(k >> 2) << 6) +
+                 ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
+  dst[o] = src[idx];
+}
+
+hipError_t launch_to_bricks(const void* src, void* dst, int elem_bytes, int nx, int ny, int nz,
+                            hipStream_t st) {
+  const long n = (long)nx * ny * nz;
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(k_to_bricks<unsigned long long>, grid, block, 0, st,
+                       (const unsigned long long*)src, (unsigned long long*)dst, nx, ny, nz);
+  else
+    hipLaunchKernelGGL(k_to_bricks<unsigned int>, grid, block, 0, st, (const unsigned int*)src,
+                       (unsigned int*)dst, nx, ny, nz);
+  return hipGetLastError();
+}
+
 hipError_t launch_interleave(const float* a, const float* b, float2* out, long n, hipStream_t st) {
   hipLaunchKernelGGL(k_interleave, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out,
                      n);

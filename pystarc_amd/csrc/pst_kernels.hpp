@@ -129,6 +129,12 @@ hipError_t launch_rpy_move(int n, double a0, double a1, double Df, int torque_mo
 
 // Interleave two FP32 grids of identical geometry into float2 nodes.
 hipError_t launch_interleave(const float* a, const float* b, float2* out, long n, hipStream_t st);
+hipError_t launch_to_bricks(const void* src, void* dst, int elem_bytes, int nx, int ny, int nz,
+                            hipStream_t st);
+// Nodes of a grid stored in 4x4x4 bricks, padding included.
+inline long brick_nodes(int nx, int ny, int nz) {
+  return (long)((nx + 3) >> 2) * ((ny + 3) >> 2) * ((nz + 3) >> 2) * 64;
+}
 
 // Synthetic grids.
 hipError_t launch_grid_dh(float* out, double* out64, int nx, int ny, int nz, const double* origin,

@@ -10,9 +10,10 @@
 // Trajectory status codes (pystarc/simulation/gpu_batch_simulator.py:514).
 enum { PST_RUNNING = 0, PST_REACTED = 1, PST_ESCAPED = 2, PST_MAXSTEPS = 3, PST_IDLE = 4 };
 
-// One APBS-style grid, C order (i*ny + j)*nz + k, z fastest
-// (pystarc/forces/electrostatic/grid_force.py:144).  lo/hi are the
-// 3-spacing BC-safe box of gpu_batch_engine.py:267-273.
+// One APBS-style grid (node (i, j, k) of an nx x ny x nz grid, DX order
+// (i*ny + j)*nz + k on upload, pystarc/forces/electrostatic/grid_force.py:144),
+// held on the device in bricks of 4x4x4 nodes (see GlobalAcc).  lo/hi are
+// the 3-spacing BC-safe box of gpu_batch_engine.py:267-273.
 struct GridDev {
   const void* data;
   int dtype;  // 0 = float32, 1 = float64
