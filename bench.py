@@ -39,7 +39,7 @@ METRIC = "trajectory-steps/sec on 1 MI355X, surrogate vs plain at equal k_on (Wi
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_TRAJ = 2 * (9 * 256 + 3 * 256 * 256 + 256 * 13)   # 404,480
-TUNE_STEPS = 60   # 2 rounds x 30 launch candidates x 1 step
+TUNE_STEPS = 100  # at most 2 rounds x 42 launch candidates x 1 step; the warm-up is topped up
 FORCE_TIMING_STEPS = 20
 
 
@@ -86,13 +86,21 @@ def run_window(sysm, eng, args, surrogate=None):
     # which are real steps under each candidate launch
     tune = None
     n_tune = TUNE_STEPS if args.autotune else 0
+    done = 0
     if args.warmup > n_tune:
         ctx.check(lib.pst_prod_run(h, args.warmup - n_tune, args.graph_steps, ptr(c16)), "warmup")
+        done = int(c16[8])
     if n_tune:
         tune = autotune_sim(lib, ctx, h)
+        done += tune["steps"]
         log(f"autotune: {tune['best']} after {tune['steps']} tuning steps")
+    if done < args.warmup:
+        # the tuner ran fewer steps than budgeted: the warm-up is exactly W steps
+        ctx.check(lib.pst_prod_run(h, args.warmup - done, args.graph_steps, ptr(c16)), "warmup")
     if surrogate is not None:
         ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
+    # trajectory-steps so far (warm-up and tuning steps), read right before timing
+    ctx.check(lib.pst_prod_counters(h, ptr(c16)), "pst_prod_counters")
     base_steps = c16[3]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -463,7 +471,8 @@ def main():
         rf = best["roof_force"]
         lc = best.get("launch")
         tc = "double" if head == "fp64" else "float"
-        kname = f"k_forces<{tc}, {lc['G']}, true>" if lc else "k_forces"
+        zp = "true" if lc and lc.get("mode") == "zpair" else "false"
+        kname = f"k_forces<{tc}, {lc['G']}, true, {zp}>" if lc else "k_forces"
         roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS,
                 "traffic": measured_traffic(head),

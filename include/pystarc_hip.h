@@ -202,6 +202,10 @@ int pst_prod_sm_counts(pst_sim* sim, unsigned long long* out48);
 int pst_prod_observables(pst_sim* sim, double* traj, unsigned long long* hist);
 /* Mean device time per step (ms) of the last pst_prod_run, HIP events. */
 int pst_prod_last_ms_per_step(pst_sim* sim, double* ms);
+/* The run counters as they stand, in pst_prod_run's 16-value layout (steps
+ * field 0), without stepping (the counters of gpu_batch_simulator.py:
+ * 1347-1348, 1524-1557 read between launches). */
+int pst_prod_counters(pst_sim* sim, double* c16);
 /* Autotuner of the production force launch (no reference counterpart: the
  * reference launches batch_force_kernel with a fixed (16,16) block,
  * gpu_batch_engine.py:328-332).  Candidates: the direct kernel with lanes per

@@ -106,6 +106,7 @@ _SIGS = {
     "pst_last_force_eval": ([_vp, _ip, _ip], C.c_int),
     "pst_prod_time_forces": ([_vp, C.c_int, _vp], C.c_int),
     "pst_prod_last_ms_per_step": ([_vp, _dp], C.c_int),
+    "pst_prod_counters": ([_vp, _vp], C.c_int),
     "pst_make_grid": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double,
                        C.c_double, C.c_double, C.c_int, _vp], C.c_int),
     "pst_mlp_param_count": ([], C.c_int),

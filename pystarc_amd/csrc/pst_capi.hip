@@ -691,6 +691,10 @@ static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const Force
     ctx->last_mode = 1;
     return launch_forces_binned(S, W, ctx->bins, precision, G, max_n, cap, ctx->stream);
   }
+  if (mode == 2 && precision == 1) {
+    ctx->last_mode = 2;
+    return launch_forces(S, W, precision, G, max_n, cap, ctx->stream, true);
+  }
   ctx->last_mode = 0;
   return launch_forces(S, W, precision, G, max_n, cap, ctx->stream);
 }
@@ -721,7 +725,7 @@ int pst_last_force_eval(const pst_ctx* ctx, int* mode, int* G) {
 
 int pst_set_gather_mode(pst_ctx* ctx, int mode, int max_traj) {
   if (!ctx) return 1;
-  if (mode != 0 && mode != 1) FAIL("pst_set_gather_mode: mode must be 0 or 1");
+  if (mode < 0 || mode > 2) FAIL("pst_set_gather_mode: mode must be 0, 1 or 2");
   if (mode == 1 && max_traj <= 0) FAIL("pst_set_gather_mode: max_traj must be positive");
   CK(set_dev(ctx));
   ctx->gather_mode = mode;
@@ -1417,6 +1421,11 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
     for (int g : Gs)
       if (g >= 8)
         for (int c : {1024, 4096}) cands.push_back({1, g, c, 1, 0.0, 0});
+  // FP32: the direct kernel with z-pair fetches
+  if (s->precision == 1)
+    for (int g : Gs)
+      for (int c : {2048, 8192})
+        for (int x = 0; x < 2; ++x) cands.push_back({2, g, c, x, 0.0, 0});
   CK(hipEventCreate(&s->tev0));
   CK(hipEventCreate(&s->tev1));
   RunCounters h;
@@ -1532,7 +1541,7 @@ int pst_prod_time_forces(pst_sim* s, int steps, double* out6) {
 int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
-  if (mode < -1 || mode > 1) FAIL("pst_prod_set_launch: mode must be -1, 0 or 1");
+  if (mode < -1 || mode > 2) FAIL("pst_prod_set_launch: mode must be -1, 0, 1 or 2");
   if (!(G == 0 || G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64))
     FAIL("pst_prod_set_launch: G must be 0 or a power of two up to 64");
   if (blocks_cap < 1) FAIL("pst_prod_set_launch: blocks_cap must be positive");
@@ -1780,6 +1789,23 @@ int pst_prod_outcomes(pst_sim* s, signed char* status, int* steps) {
   const size_t N = (size_t)s->P.n_total;
   if (status) CK(hipMemcpy(status, s->out_status, N, hipMemcpyDeviceToHost));
   if (steps) CK(hipMemcpy(steps, s->out_steps, N * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pst_prod_counters(pst_sim* s, double* c16) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready || !c16) FAIL("pst_prod_counters: call pst_prod_init first");
+  CK(set_dev(ctx));
+  RunCounters h;
+  CK(hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  const double v[16] = {(double)h.n_reacted, (double)h.n_escaped, (double)h.n_maxsteps,
+                        (double)h.total_steps, (double)h.next_traj, (double)h.n_returns,
+                        (double)h.n_overlap, (double)h.n_bridge, 0.0, (double)h.stop,
+                        (double)h.n_sr, (double)h.n_mlp, h.audit_err2, h.audit_ref2,
+                        (double)h.audit_n, 0};
+  memcpy(c16, v, sizeof(v));
   return 0;
 }
 

@@ -164,14 +164,20 @@ template <> __device__ inline Pair<double> gld<Pair<double>, float2>(const float
 // 64 B of FP32 or 128 B of float2).  The 20 nodes of one atom's stencil then
 // sit in ~3-5 cache lines instead of the ~8-9 of a C-order array, where
 // every (i, j) column of the stencil is a line of its own.
-struct GlobalAcc {
+// ZP: fetch the z-adjacent node pairs of a stencil with one wide load where
+// both nodes sit in one brick (the direct kernel's "z-pair" launch variant).
+template <bool ZP>
+struct GlobalAccT {
+  static constexpr bool kZPairs = ZP;
   long bx, by;  // nodes per brick step along x and along y
   __device__ long ox(int i) const { return (long)(i >> 2) * bx + ((i & 3) << 4); }
   __device__ long oy(int j) const { return (long)(j >> 2) * by + ((j & 3) << 2); }
   __device__ long oz(int k) const { return ((long)(k >> 2) << 6) + (k & 3); }
   __device__ long at(int i, int j, int k) const { return ox(i) + oy(j) + oz(k); }
 };
+using GlobalAcc = GlobalAccT<false>;
 struct TileAcc {
+  static constexpr bool kZPairs = false;
   long sx, sy;
   int ti, tj, tk;
   __device__ long ox(int i) const { return (long)(i - ti) * sx; }
@@ -179,7 +185,53 @@ struct TileAcc {
   __device__ long oz(int k) const { return (long)(k - tk); }
   __device__ long at(int i, int j, int k) const { return ox(i) + oy(j) + oz(k); }
 };
-__device__ inline GlobalAcc global_acc(const GridDev& g) {
+// Nodes (k0, k0 + 1) of one (i, j) column.  In a bricked grid they are
+// adjacent unless k0 ends a brick; with z-pair fetches they then come in one
+// 8- or 16-byte load (the values are those of two single loads).
+typedef float pst_f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float pst_f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef double pst_d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+template <typename V, typename TG> __device__ inline void gld_pair2(const TG* d, long o, V& lo, V& hi);
+template <> __device__ inline void gld_pair2<float, float>(const float* d, long o, float& lo, float& hi) {
+  const pst_f2a4 v = *(const pst_f2a4*)(d + o);
+  lo = v.x; hi = v.y;
+}
+template <> __device__ inline void gld_pair2<double, float>(const float* d, long o, double& lo, double& hi) {
+  const pst_f2a4 v = *(const pst_f2a4*)(d + o);
+  lo = v.x; hi = v.y;
+}
+template <> __device__ inline void gld_pair2<double, double>(const double* d, long o, double& lo, double& hi) {
+  const pst_d2a8 v = *(const pst_d2a8*)(d + o);
+  lo = v.x; hi = v.y;
+}
+template <> __device__ inline void gld_pair2<float, double>(const double* d, long o, float& lo, float& hi) {
+  const pst_d2a8 v = *(const pst_d2a8*)(d + o);
+  lo = (float)v.x; hi = (float)v.y;
+}
+template <> __device__ inline void gld_pair2<Pair<float>, float2>(const float2* d, long o, Pair<float>& lo,
+                                                                 Pair<float>& hi) {
+  const pst_f4a8 v = *(const pst_f4a8*)(d + o);
+  lo = {v.x, v.y}; hi = {v.z, v.w};
+}
+template <> __device__ inline void gld_pair2<Pair<double>, float2>(const float2* d, long o,
+                                                                  Pair<double>& lo, Pair<double>& hi) {
+  const pst_f4a8 v = *(const pst_f4a8*)(d + o);
+  lo = {(double)v.x, (double)v.y}; hi = {(double)v.z, (double)v.w};
+}
+template <typename V, typename TG, typename A>
+__device__ inline void gld_z(const TG* d, const A& a, long xy, int k0, long Z0, long Z1, V& lo, V& hi) {
+  if constexpr (A::kZPairs) {
+    if ((k0 & 3) != 3) {
+      gld_pair2<V, TG>(d, xy + Z0, lo, hi);
+      return;
+    }
+  }
+  lo = gld<V>(d, xy + Z0);
+  hi = gld<V>(d, xy + Z1);
+}
+
+template <bool ZP = false>
+__device__ inline GlobalAccT<ZP> global_acc(const GridDev& g) {
   const long nbz = (g.nz + 3) >> 2, nby = (g.ny + 3) >> 2;
   return {nby * nbz * 64, nbz * 64};
 }
@@ -267,15 +319,16 @@ __device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __res
   const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Xe = a.ox(ai < 0 ? i0 - 1 : i0 + 2);
   const long Y0 = a.oy(j0), Y1 = a.oy(j0 + 1), Ye = a.oy(aj < 0 ? j0 - 1 : j0 + 2);
   const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1), Ze = a.oz(ak < 0 ? k0 - 1 : k0 + 2);
-  // centre cell
-  const V c000 = gld<V>(d, X0 + Y0 + Z0), c001 = gld<V>(d, X0 + Y0 + Z1),
-          c010 = gld<V>(d, X0 + Y1 + Z0), c011 = gld<V>(d, X0 + Y1 + Z1);
-  const V c100 = gld<V>(d, X1 + Y0 + Z0), c101 = gld<V>(d, X1 + Y0 + Z1),
-          c110 = gld<V>(d, X1 + Y1 + Z0), c111 = gld<V>(d, X1 + Y1 + Z1);
-  const V x00 = gld<V>(d, Xe + Y0 + Z0), x01 = gld<V>(d, Xe + Y0 + Z1),
-          x10 = gld<V>(d, Xe + Y1 + Z0), x11 = gld<V>(d, Xe + Y1 + Z1);
-  const V y00 = gld<V>(d, X0 + Ye + Z0), y01 = gld<V>(d, X0 + Ye + Z1),
-          y10 = gld<V>(d, X1 + Ye + Z0), y11 = gld<V>(d, X1 + Ye + Z1);
+  // centre cell and the x- and y-extra planes, z-pairs (k0, k0 + 1)
+  V c000, c001, c010, c011, c100, c101, c110, c111, x00, x01, x10, x11, y00, y01, y10, y11;
+  gld_z<V, TG, A>(d, a, X0 + Y0, k0, Z0, Z1, c000, c001);
+  gld_z<V, TG, A>(d, a, X0 + Y1, k0, Z0, Z1, c010, c011);
+  gld_z<V, TG, A>(d, a, X1 + Y0, k0, Z0, Z1, c100, c101);
+  gld_z<V, TG, A>(d, a, X1 + Y1, k0, Z0, Z1, c110, c111);
+  gld_z<V, TG, A>(d, a, Xe + Y0, k0, Z0, Z1, x00, x01);
+  gld_z<V, TG, A>(d, a, Xe + Y1, k0, Z0, Z1, x10, x11);
+  gld_z<V, TG, A>(d, a, X0 + Ye, k0, Z0, Z1, y00, y01);
+  gld_z<V, TG, A>(d, a, X1 + Ye, k0, Z0, Z1, y10, y11);
   const V z00 = gld<V>(d, X0 + Y0 + Ze), z01 = gld<V>(d, X0 + Y1 + Ze),
           z10 = gld<V>(d, X1 + Y0 + Ze), z11 = gld<V>(d, X1 + Y1 + Ze);
   const TC one = (TC)1;
@@ -324,50 +377,39 @@ __device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __res
   }
 }
 
-template <typename TG, typename TC, typename V>
+template <typename TG, typename TC, typename V, bool ZP = false>
 __device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double py, double pz,
                                              V& phi, V& gx, V& gy, V& gz) {
-  grid_phi_grad_stencil_a<TG, TC, V>(g, (const TG*)g.data, global_acc(g), px, py, pz, phi, gx, gy,
-                                     gz);
+  grid_phi_grad_stencil_a<TG, TC, V>(g, (const TG*)g.data, global_acc<ZP>(g), px, py, pz, phi, gx,
+                                     gy, gz);
 }
 
 // FP64 follows the reference's seven interpolations literally (parity with
 // gpu_batch_engine.py to rounding); FP32 uses the node-difference stencil.
-template <typename TC>
+template <typename TC, bool ZP = false>
 __device__ inline void grid_phi_grad_any(const GridDev& g, double px, double py, double pz,
-                                         TC& phi, TC& gx, TC& gy, TC& gz);
-template <>
-__device__ inline void grid_phi_grad_any<double>(const GridDev& g, double px, double py, double pz,
-                                                 double& phi, double& gx, double& gy, double& gz) {
-  if (g.dtype == 1)
-    grid_phi_grad<double, double, double>(g, px, py, pz, phi, gx, gy, gz);
-  else
-    grid_phi_grad<float, double, double>(g, px, py, pz, phi, gx, gy, gz);
-}
-template <>
-__device__ inline void grid_phi_grad_any<float>(const GridDev& g, double px, double py, double pz,
-                                                float& phi, float& gx, float& gy, float& gz) {
-  if (g.dtype == 1)
-    grid_phi_grad_stencil<double, float, float>(g, px, py, pz, phi, gx, gy, gz);
-  else
-    grid_phi_grad_stencil<float, float, float>(g, px, py, pz, phi, gx, gy, gz);
+                                         TC& phi, TC& gx, TC& gy, TC& gz) {
+  if constexpr (std::is_same<TC, double>::value) {
+    if (g.dtype == 1)
+      grid_phi_grad<double, double, double>(g, px, py, pz, phi, gx, gy, gz);
+    else
+      grid_phi_grad<float, double, double>(g, px, py, pz, phi, gx, gy, gz);
+  } else {
+    if (g.dtype == 1)
+      grid_phi_grad_stencil<double, float, float, ZP>(g, px, py, pz, phi, gx, gy, gz);
+    else
+      grid_phi_grad_stencil<float, float, float, ZP>(g, px, py, pz, phi, gx, gy, gz);
+  }
 }
 
 // Interleaved potential + Born grid (float2 per node): both fields at once.
-template <typename TC>
+template <typename TC, bool ZP = false>
 __device__ inline void grid_phi_grad_pair(const GridDev& g, double px, double py, double pz,
-                                          Pair<TC>& phi, Pair<TC>& gx, Pair<TC>& gy, Pair<TC>& gz);
-template <>
-__device__ inline void grid_phi_grad_pair<double>(const GridDev& g, double px, double py, double pz,
-                                                  Pair<double>& phi, Pair<double>& gx,
-                                                  Pair<double>& gy, Pair<double>& gz) {
-  grid_phi_grad<float2, double, Pair<double>>(g, px, py, pz, phi, gx, gy, gz);
-}
-template <>
-__device__ inline void grid_phi_grad_pair<float>(const GridDev& g, double px, double py, double pz,
-                                                 Pair<float>& phi, Pair<float>& gx,
-                                                 Pair<float>& gy, Pair<float>& gz) {
-  grid_phi_grad_stencil<float2, float, Pair<float>>(g, px, py, pz, phi, gx, gy, gz);
+                                          Pair<TC>& phi, Pair<TC>& gx, Pair<TC>& gy, Pair<TC>& gz) {
+  if constexpr (std::is_same<TC, double>::value)
+    grid_phi_grad<float2, double, Pair<double>>(g, px, py, pz, phi, gx, gy, gz);
+  else
+    grid_phi_grad_stencil<float2, float, Pair<float>, ZP>(g, px, py, pz, phi, gx, gy, gz);
 }
 
 // The same two gathers through an accessor (a grid tile staged in LDS):

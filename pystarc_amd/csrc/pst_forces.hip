@@ -23,7 +23,7 @@ template <> struct Vec4<float> { typedef float4 type; };
 
 // Direct terms on one ligand atom at lab position p: receptor elec grids
 // finest-first with Yukawa fallback (:289-366) and receptor Born grids (:571-576).
-template <typename TC>
+template <typename TC, bool ZP = false>
 __device__ inline void atom_direct(const ForceSys& S, double px, double py, double pz, double q,
                                    bool grids, TC& fx, TC& fy, TC& fz, TC& e) {
   if (S.paired) {
@@ -31,7 +31,7 @@ __device__ inline void atom_direct(const ForceSys& S, double px, double py, doub
     if (grids && grid_inside(S.pair, px, py, pz)) {
       if (fabs(q) >= 1e-9) {
         Pair<TC> phi, gx, gy, gz;
-        grid_phi_grad_pair<TC>(S.pair, px, py, pz, phi, gx, gy, gz);
+        grid_phi_grad_pair<TC, ZP>(S.pair, px, py, pz, phi, gx, gy, gz);
         const TC qq = (TC)q, c = (TC)(S.alpha * q * q);
         fx -= qq * gx.a + c * gx.b;
         fy -= qq * gy.a + c * gy.b;
@@ -51,7 +51,7 @@ __device__ inline void atom_direct(const ForceSys& S, double px, double py, doub
       assigned = true;
       if (fabs(q) >= 1e-9) {
         TC phi, gx, gy, gz;
-        grid_phi_grad_any<TC>(S.elec[g], px, py, pz, phi, gx, gy, gz);
+        grid_phi_grad_any<TC, ZP>(S.elec[g], px, py, pz, phi, gx, gy, gz);
         const TC qq = (TC)q;
         fx -= qq * gx; fy -= qq * gy; fz -= qq * gz;
         e += qq * phi;
@@ -66,7 +66,7 @@ __device__ inline void atom_direct(const ForceSys& S, double px, double py, doub
       assigned = true;
       if (fabs(q) >= 1e-9) {
         TC phi, gx, gy, gz;
-        grid_phi_grad_any<TC>(S.born[g], px, py, pz, phi, gx, gy, gz);
+        grid_phi_grad_any<TC, ZP>(S.born[g], px, py, pz, phi, gx, gy, gz);
         const TC c = (TC)(S.alpha * q * q);
         fx -= c * gx; fy -= c * gy; fz -= c * gz;
         e += c * phi;
@@ -77,12 +77,12 @@ __device__ inline void atom_direct(const ForceSys& S, double px, double py, doub
 
 // Reverse Born on one receptor atom already expressed in the ligand frame
 // (:641-652); the returned force acts on the receptor atom (ligand frame).
-template <typename TC>
+template <typename TC, bool ZP = false>
 __device__ inline bool atom_reverse(const ForceSys& S, double x, double y, double z, double q,
                                     TC& fx, TC& fy, TC& fz) {
   if (!grid_inside(S.lig_born, x, y, z) || fabs(q) < 1e-9) return false;
   TC phi, gx, gy, gz;
-  grid_phi_grad_any<TC>(S.lig_born, x, y, z, phi, gx, gy, gz);
+  grid_phi_grad_any<TC, ZP>(S.lig_born, x, y, z, phi, gx, gy, gz);
   const TC c = (TC)(S.alpha * q * q);
   fx = -c * gx; fy = -c * gy; fz = -c * gz;
   return true;
@@ -91,7 +91,7 @@ __device__ inline bool atom_reverse(const ForceSys& S, double x, double y, doubl
 
 // One system's force evaluation by workgroup `blk` of `nblk` (the workgroups
 // of k_forces, or the slice of a grouped launch that belongs to the system).
-template <typename TC, int G, bool POSE>
+template <typename TC, int G, bool POSE, bool ZP = false>
 __device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int blk, int nblk) {
   typedef typename Vec4<TC>::type V4;
   extern __shared__ char smem_raw[];
@@ -186,7 +186,7 @@ __device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int bl
           px = lp[0]; py = lp[1]; pz = lp[2]; q = lq[i];
         }
         TC ax = 0, ay = 0, az = 0, ae = 0;
-        atom_direct<TC>(S, px, py, pz, q, grids, ax, ay, az, ae);
+        atom_direct<TC, ZP>(S, px, py, pz, q, grids, ax, ay, az, ae);
         fx += ax; fy += ay; fz += az; e += ae;
         if (S.torque_mode == 1) {
           const TC lx = (TC)(px - qx), ly = (TC)(py - qy), lz = (TC)(pz - qz);
@@ -221,7 +221,7 @@ __device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int bl
             const double y = R[1] * dx + R[4] * dy + R[7] * dz;
             const double z = R[2] * dx + R[5] * dy + R[8] * dz;
             TC ax, ay, az;
-            if (atom_reverse<TC>(S, x, y, z, S.rec_q[j], ax, ay, az)) {
+            if (atom_reverse<TC, ZP>(S, x, y, z, S.rec_q[j], ax, ay, az)) {
               rx += ax; ry += ay; rz += az;
               if (S.torque_mode == 1) {
                 const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
@@ -258,9 +258,11 @@ __device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int bl
   }
 }
 
-template <typename TC, int G, bool POSE>
+// ZP: z-pair fetches (one wide load for the two z-adjacent nodes of a
+// stencil column), chosen by the launch autotuner where it is faster.
+template <typename TC, int G, bool POSE, bool ZP>
 __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W) {
-  forces_body<TC, G, POSE>(S, W, blockIdx.x, gridDim.x);
+  forces_body<TC, G, POSE, ZP>(S, W, blockIdx.x, gridDim.x);
 }
 
 // Several systems (one ligand each, sharing the receptor grids) in one
@@ -331,7 +333,7 @@ __global__ void k_chunk_min_r(const double* pos, const int* list, int n, int fb,
 
 template <typename TC, int G>
 static hipError_t launch_g(const ForceSys& S, const ForceWork& W, int max_n, int blocks_cap,
-                           hipStream_t st) {
+                           hipStream_t st, bool zp) {
   const int gpb = kForceBlock / G;
   int blocks = (max_n + gpb - 1) / gpb;
   if (blocks > blocks_cap) blocks = blocks_cap;
@@ -340,10 +342,15 @@ static hipError_t launch_g(const ForceSys& S, const ForceWork& W, int max_n, int
   const bool pose = W.quat != nullptr && W.lig_pos == nullptr;
   size_t lds = 0;
   if (pose && S.M <= kLdsAtomsMax) lds = (size_t)S.M * sizeof(typename Vec4<TC>::type);
-  if (pose)
-    hipLaunchKernelGGL((k_forces<TC, G, true>), dim3(blocks), dim3(kForceBlock), lds, st, S, W);
+  if (pose && zp && std::is_same<TC, float>::value)
+    hipLaunchKernelGGL((k_forces<TC, G, true, true>), dim3(blocks), dim3(kForceBlock), lds, st, S,
+                       W);
+  else if (pose)
+    hipLaunchKernelGGL((k_forces<TC, G, true, false>), dim3(blocks), dim3(kForceBlock), lds, st, S,
+                       W);
   else
-    hipLaunchKernelGGL((k_forces<TC, G, false>), dim3(blocks), dim3(kForceBlock), 0, st, S, W);
+    hipLaunchKernelGGL((k_forces<TC, G, false, false>), dim3(blocks), dim3(kForceBlock), 0, st, S,
+                       W);
   return hipGetLastError();
 }
 
@@ -355,27 +362,27 @@ int pick_group(int M, int n_rec_active) {
 }
 
 hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, int G, int max_n,
-                         int blocks_cap, hipStream_t st) {
+                         int blocks_cap, hipStream_t st, bool zp) {
   if (max_n <= 0) return hipSuccess;
   if (precision == 0) {
     switch (G) {
-      case 1: return launch_g<double, 1>(S, W, max_n, blocks_cap, st);
-      case 2: return launch_g<double, 2>(S, W, max_n, blocks_cap, st);
-      case 4: return launch_g<double, 4>(S, W, max_n, blocks_cap, st);
-      case 8: return launch_g<double, 8>(S, W, max_n, blocks_cap, st);
-      case 16: return launch_g<double, 16>(S, W, max_n, blocks_cap, st);
-      case 32: return launch_g<double, 32>(S, W, max_n, blocks_cap, st);
-      default: return launch_g<double, 64>(S, W, max_n, blocks_cap, st);
+      case 1: return launch_g<double, 1>(S, W, max_n, blocks_cap, st, zp);
+      case 2: return launch_g<double, 2>(S, W, max_n, blocks_cap, st, zp);
+      case 4: return launch_g<double, 4>(S, W, max_n, blocks_cap, st, zp);
+      case 8: return launch_g<double, 8>(S, W, max_n, blocks_cap, st, zp);
+      case 16: return launch_g<double, 16>(S, W, max_n, blocks_cap, st, zp);
+      case 32: return launch_g<double, 32>(S, W, max_n, blocks_cap, st, zp);
+      default: return launch_g<double, 64>(S, W, max_n, blocks_cap, st, zp);
     }
   }
   switch (G) {
-    case 1: return launch_g<float, 1>(S, W, max_n, blocks_cap, st);
-    case 2: return launch_g<float, 2>(S, W, max_n, blocks_cap, st);
-    case 4: return launch_g<float, 4>(S, W, max_n, blocks_cap, st);
-    case 8: return launch_g<float, 8>(S, W, max_n, blocks_cap, st);
-    case 16: return launch_g<float, 16>(S, W, max_n, blocks_cap, st);
-    case 32: return launch_g<float, 32>(S, W, max_n, blocks_cap, st);
-    default: return launch_g<float, 64>(S, W, max_n, blocks_cap, st);
+    case 1: return launch_g<float, 1>(S, W, max_n, blocks_cap, st, zp);
+    case 2: return launch_g<float, 2>(S, W, max_n, blocks_cap, st, zp);
+    case 4: return launch_g<float, 4>(S, W, max_n, blocks_cap, st, zp);
+    case 8: return launch_g<float, 8>(S, W, max_n, blocks_cap, st, zp);
+    case 16: return launch_g<float, 16>(S, W, max_n, blocks_cap, st, zp);
+    case 32: return launch_g<float, 32>(S, W, max_n, blocks_cap, st, zp);
+    default: return launch_g<float, 64>(S, W, max_n, blocks_cap, st, zp);
   }
 }
 

@@ -6,7 +6,7 @@
 namespace pst {
 
 hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, int G, int max_n,
-                         int blocks_cap, hipStream_t st);
+                         int blocks_cap, hipStream_t st, bool zp = false);
 hipError_t launch_chunk_min_r(const double* pos, const int* list, int n, int fb, int cs, int cpf,
                               double* out, int n_chunks, hipStream_t st);
 int pick_group(int M, int n_rec_active);

@@ -143,20 +143,24 @@ class GPUBatchForceEngine:
         "binned" - gather requests sorted by 8^3-cell grid brick, each brick
         staged once in LDS; identical results; buffers for max_traj
         trajectories.  Without the float2 receptor pair grid the direct
-        kernel runs."""
-        if mode not in ("direct", "binned"):
-            raise ValueError("mode must be 'direct' or 'binned'")
-        m = 1 if mode == "binned" else 0
-        self.ctx.check(self.ctx.lib.pst_set_gather_mode(self.ctx.h, m, int(max_traj) if m else 0),
+        kernel runs.
+        "zpair"  - the direct kernel fetching the two z-adjacent nodes of a
+        stencil column with one wide load (FP32; identical results; the
+        FP64 engine runs the direct kernel)."""
+        if mode not in ("direct", "binned", "zpair"):
+            raise ValueError("mode must be 'direct', 'binned' or 'zpair'")
+        m = {"direct": 0, "binned": 1, "zpair": 2}[mode]
+        self.ctx.check(self.ctx.lib.pst_set_gather_mode(self.ctx.h, m,
+                                                        int(max_traj) if m == 1 else 0),
                        "pst_set_gather_mode")
         self.gather = mode
 
     def last_eval(self):
-        """(mode, G) of the last force evaluation: "direct" / "binned", lanes."""
+        """(mode, G) of the last force evaluation: "direct" / "binned" / "zpair", lanes."""
         m, g = C.c_int(), C.c_int()
         self.ctx.check(self.ctx.lib.pst_last_force_eval(self.ctx.h, C.byref(m), C.byref(g)),
                        "pst_last_force_eval")
-        return {-1: None, 0: "direct", 1: "binned"}[m.value], g.value
+        return {-1: None, 0: "direct", 1: "binned", 2: "zpair"}[m.value], g.value
 
     def set_ligand(self, body, charges):
         """Body-frame ligand coordinates (about the centroid) for the fused path."""

@@ -34,7 +34,7 @@ Keyword-only extensions (defaults keep the reference's physics):
                after which the production force launch (direct or
                brick-binned evaluation, lanes per trajectory, workgroup cap,
                XCD mapping) is tuned on live steps.
-  gather       None (the engine's), "direct" or "binned": force evaluation
+  gather       None (the engine's), "direct", "binned" or "zpair": force evaluation
                of the production step (identical results either way).
   hydro        "scalar" (default): the reference's scalar relative diffusion
                (D_t, or D_parallel(r) with hydrodynamic_interactions).
@@ -258,9 +258,10 @@ def autotune_sim(lib, ctx, h, rounds=2, steps_per_cand=1):
     best = (C.c_int * 5)()
     ctx.check(lib.pst_prod_autotune(h, int(rounds), int(steps_per_cand), ptr(table), 96,
                                     C.byref(n), best), "pst_prod_autotune")
-    rows = [{"mode": "binned" if r[0] else "direct", "G": int(r[1]), "cap": int(r[2]),
+    names = {0: "direct", 1: "binned", 2: "zpair"}
+    rows = [{"mode": names[int(r[0])], "G": int(r[1]), "cap": int(r[2]),
              "xcd": int(r[3]), "ms": float(r[4])} for r in table[:n.value]]
-    return {"best": {"mode": "binned" if best[0] else "direct", "G": best[1], "cap": best[2],
+    return {"best": {"mode": names[int(best[0])], "G": best[1], "cap": best[2],
                      "xcd": best[3]},
             "table": rows, "steps": int(best[4])}
 
@@ -301,8 +302,8 @@ class GPUBatchSimulator:
             raise ValueError("rpy12 needs both hydrodynamic radii (mob.radius1, mob.radius2)")
         self.hydro = hydro
         self.autotune = int(autotune) if autotune is not True else 64
-        if gather not in (None, "direct", "binned"):
-            raise ValueError("gather must be None, 'direct' or 'binned'")
+        if gather not in (None, "direct", "binned", "zpair"):
+            raise ValueError("gather must be None, 'direct', 'binned' or 'zpair'")
         self.gather = gather
         self.mol1, self.mol2, self.mob = mol1, mol2, mob
         self.pathway_set, self.params, self.engine = pathway_set, params, batch_engine
@@ -494,7 +495,8 @@ class GPUBatchSimulator:
                       "pst_sim_create")
         if self.gather is not None:
             eng.ctx.check(eng.ctx.lib.pst_prod_set_launch(
-                h, 1 if self.gather == "binned" else 0, 0, 2048, 1), "pst_prod_set_launch")
+                h, {"direct": 0, "binned": 1, "zpair": 2}[self.gather], 0, 2048, 1),
+                "pst_prod_set_launch")
         return h
 
     # -- run -----------------------------------------------------------------------

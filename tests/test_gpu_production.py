@@ -86,3 +86,13 @@ def test_autotune_measures_every_candidate_and_keeps_outcomes():
     both = (o1 > 0) & (o2 > 0)
     agree = (o1[both] == o2[both]) & (a.sim_data["n_steps"][both] == b.sim_data["n_steps"][both])
     assert both.sum() > 1000 and agree.mean() > 0.99
+
+
+def test_autotune_fp32_tries_zpair_launches():
+    sysm = small_protein_ligand(n_traj=600)
+    sysm.params.max_steps = 3000
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)), precision="fp32")
+    b = run(sysm, eng, record=True, autotune=16)
+    modes = {r["mode"] for r in b.sim_data["autotune"]["table"]}
+    assert {"direct", "zpair"} <= modes
+    assert b.n_reacted + b.n_escaped > 0
