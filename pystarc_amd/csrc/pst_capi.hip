@@ -119,6 +119,10 @@ struct pst_sim {
   int tune_xcd = 1;        // XCD-chunked mapping of the sorted grid-path list
   int tune_mode = -1;      // 0 direct, 1 brick-binned, -1 the context's gather mode
   hipEvent_t tev0 = nullptr, tev1 = nullptr;  // brackets the force launch while tuning
+  // the surrogate audit's grid-path launch runs beside the main force launch
+  // on a second stream (fork after the sort, join before the comparison)
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   unsigned long long* census = nullptr;       // gather census of timed force launches
   bool census_on = false;
 };
@@ -941,6 +945,9 @@ int pst_sim_destroy(pst_sim* s) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (void* p : s->obs_allocs) hipFree(p);
+  if (s->aux) { hipStreamSynchronize(s->aux); hipStreamDestroy(s->aux); }
+  if (s->fork_ev) hipEventDestroy(s->fork_ev);
+  if (s->join_ev) hipEventDestroy(s->join_ev);
   delete s;
   return 0;
 }
@@ -1264,22 +1271,54 @@ static int enqueue_pre(pst_sim* s, int par) {
 }
 
 // Surrogate forces of the step and their audit (after the grid-path forces).
-static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W) {
+static bool audits(const pst_sim* s) {
+  return s->mlp && s->P.surrogate_r > 0 && s->sp.audit_n > 0;
+}
+
+// The plain path judges the surrogate on the first audit_n surrogate
+// trajectories: their grid-path forces into audit_F.
+static hipError_t launch_audit_forces(pst_sim* s, const ForceSys& S, const ForceWork& W,
+                                      hipStream_t st) {
+  ForceWork A = W;
+  A.list = s->mlp_list;
+  A.n_dev = &s->ctr->n_mlp;
+  A.n_cap = s->sp.audit_n;
+  A.out_compact = 1;
+  A.xcd_chunks = 0;
+  A.F = s->audit_F;
+  A.T = nullptr;
+  return launch_forces(S, A, s->precision, group_for(s->ctx, S), s->sp.audit_n, 2048, st);
+}
+
+// Fork: the audit launch on the second stream, ordered after everything
+// enqueued so far on the context stream (captured graphs take it as a
+// parallel branch).
+static int fork_audit(pst_sim* s, const ForceSys& S, const ForceWork& W) {
+  pst_ctx* ctx = s->ctx;
+  if (!s->aux) {
+    CK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+    CK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+  }
+  CK(hipEventRecord(s->fork_ev, ctx->stream));
+  CK(hipStreamWaitEvent(s->aux, s->fork_ev, 0));
+  CK(launch_audit_forces(s, S, W, s->aux));
+  CK(hipEventRecord(s->join_ev, s->aux));
+  return 0;
+}
+
+// Surrogate forces of the step and their audit (after the grid-path forces).
+// forked: the audit's grid-path launch was issued by fork_audit.
+static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool forked = false) {
   pst_ctx* ctx = s->ctx;
   if (s->mlp && s->P.surrogate_r > 0) {
     CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
                  ctx->stream));
     if (s->sp.audit_n > 0) {
-      // the plain path judges the surrogate on the first audit_n surrogate trajectories
-      ForceWork A = W;
-      A.list = s->mlp_list;
-      A.n_dev = &s->ctr->n_mlp;
-      A.n_cap = s->sp.audit_n;
-      A.out_compact = 1;
-      A.xcd_chunks = 0;
-      A.F = s->audit_F;
-      A.T = nullptr;
-      CK(launch_forces(S, A, s->precision, group_for(ctx, S), s->sp.audit_n, 2048, ctx->stream));
+      if (forked)
+        CK(hipStreamWaitEvent(ctx->stream, s->join_ev, 0));
+      else
+        CK(launch_audit_forces(s, S, W, ctx->stream));
       CK(surr_audit(s->mlp_list, &s->ctr->n_mlp, s->sp.audit_n, s->F, s->audit_F, s->ctr,
                     ctx->stream));
     }
@@ -1289,10 +1328,11 @@ static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W) {
 
 // Step phases after the grid-path forces: surrogate forces and their audit,
 // the move, the Brownian bridge.
-static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par) {
+static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par,
+                        bool forked = false) {
   pst_ctx* ctx = s->ctx;
   ProdBufs B = prod_bufs(s);
-  const int r = enqueue_surr(s, S, W);
+  const int r = enqueue_surr(s, S, W, forked);
   if (r) return r;
   CK(prod_step_post(s->P, S, B, par, ctx->stream));
   CK(prod_step_bridge(s->P, B, par, ctx->stream));
@@ -1307,12 +1347,17 @@ static int enqueue_step(pst_sim* s, int par) {
   ForceSys S;
   fill_sys(ctx, S);
   const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
+  const bool forked = audits(s);
+  if (forked) {
+    const int r = fork_audit(s, S, W);
+    if (r) return r;
+  }
   if (s->census_on) CK(launch_census(S, W, s->census, ctx->stream));
   if (s->tev0) CK(hipEventRecord(s->tev0, ctx->stream));
   const int mode = s->tune_mode >= 0 ? s->tune_mode : ctx->gather_mode;
   CK(launch_force_eval(ctx, S, W, s->precision, G, s->S, s->tune_cap, mode));
   if (s->tev1) CK(hipEventRecord(s->tev1, ctx->stream));
-  return enqueue_post(s, S, W, par);
+  return enqueue_post(s, S, W, par, forked);
 }
 
 int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) {
