@@ -844,8 +844,8 @@ class GPUBatchGroup:
             if warmup > 0:
                 ctx.check(lib.pst_group_run(arr, L, int(warmup), self.graph_steps, ptr(c16),
                                             C.byref(ms)), "pst_group_run")
-            for h in handles:
-                if sims[0].surrogate is not None:
+            for s, h in zip(sims, handles):
+                if s.surrogate is not None:
                     ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
             base = c16[:, 3].copy()
             torch.cuda.synchronize()
