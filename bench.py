@@ -39,7 +39,7 @@ METRIC = "trajectory-steps/sec on 1 MI355X, surrogate vs plain at equal k_on (Wi
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_TRAJ = 2 * (9 * 256 + 3 * 256 * 256 + 256 * 13)   # 404,480
-TUNE_STEPS = 100  # at most 2 rounds x 42 launch candidates x 1 step; the warm-up is topped up
+TUNE_STEPS = 130  # at most 3 rounds x 42 launch candidates x 1 step; the warm-up is topped up
 FORCE_TIMING_STEPS = 20
 
 
@@ -375,8 +375,9 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--paths", default="fp64,fp32,surrogate")
     ap.add_argument("--kon-trajectories", type=int, default=200_000)
-    ap.add_argument("--train-poses", type=int, default=4_000_000)
-    ap.add_argument("--train-epochs", type=int, default=30)
+    # the surrogate workload's 2e7 (pose, force) training pairs
+    ap.add_argument("--train-poses", type=int, default=20_000_000)
+    ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--audit-n", type=int, default=64)
     ap.add_argument("--audit-tol", type=float, default=0.25)

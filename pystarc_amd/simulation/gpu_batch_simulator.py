@@ -250,7 +250,7 @@ def observables(lib, ctx, h, P, outcome):
     }
 
 
-def autotune_sim(lib, ctx, h, rounds=2, steps_per_cand=1):
+def autotune_sim(lib, ctx, h, rounds=3, steps_per_cand=1):
     """Runs pst_prod_autotune on a production sim handle; returns the table of
     candidates {G, cap, xcd, ms} and the chosen launch."""
     table = np.zeros((96, 5))
