@@ -68,7 +68,10 @@ int pst_set_alpha(pst_ctx* ctx, double alpha);
  * gpu_batch_simulator.py:775-777); host arrays (n,3) and (n,). */
 int pst_set_receptor(pst_ctx* ctx, int n, const double* pos, const double* charges);
 /* Ligand body-frame coordinates about the centroid and charges
- * (gpu_batch_simulator.py:171-176); host arrays (m,3) and (m,). */
+ * (gpu_batch_simulator.py:171-176); host arrays (m,3) and (m,).  A ligand of
+ * more than 64 atoms is held on the device in a spatial order (compact runs
+ * of 32 and 64 atoms); forces, torques and energies are sums over atoms, so
+ * only their summation order depends on it. */
 int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charges);
 /* 0 = reference torque (zero, gpu_batch_engine.py:597-599);
  * 1 = lever arm about the centroid of charged atoms (pystarc/forces/engine.py:144-183). */

@@ -241,6 +241,43 @@ static void rec_chunk_order(int n, const double* pos, std::vector<int>& order,
   }
 }
 
+// Device order of a ligand's atoms.  The pose-mode force loops hand atom i
+// to lane i mod G (G = 16..64), so the atoms one wave gathers together are an
+// aligned run of 64 or 32 entries.  Recursive bisection along the longest
+// edge of the bounding box, at multiples of 64 while a span holds more than
+// 64 atoms and at 32, then 16, below that (ties broken by atom index), makes
+// every such run a compact cluster: its stencils share grid bricks and cache
+// lines.  Only the order of each force sum changes.
+static void lig_spatial_order(int n, const double* pos, std::vector<int>& order) {
+  order.resize(n);
+  for (int j = 0; j < n; ++j) order[j] = j;
+  struct Span { int lo, hi; };
+  std::vector<Span> todo{{0, n}};
+  while (!todo.empty()) {
+    const Span s = todo.back();
+    todo.pop_back();
+    const int cnt = s.hi - s.lo;
+    if (cnt <= 16) continue;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int j = s.lo; j < s.hi; ++j)
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = fmin(mn[d], pos[3 * order[j] + d]);
+        mx[d] = fmax(mx[d], pos[3 * order[j] + d]);
+      }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d) if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+    const int run = cnt > 64 ? 64 : cnt > 32 ? 32 : 16;
+    const int mid = s.lo + (((cnt + run - 1) / run) / 2) * run;
+    std::nth_element(order.begin() + s.lo, order.begin() + mid, order.begin() + s.hi,
+                     [&](int a, int b) {
+                       const double pa = pos[3 * a + ax], pb = pos[3 * b + ax];
+                       return pa < pb || (pa == pb && a < b);
+                     });
+    todo.push_back({mid, s.hi});
+    todo.push_back({s.lo, mid});
+  }
+}
+
 static void fill_sys(pst_ctx* ctx, ForceSys& S) {
   memset(&S, 0, sizeof(S));
   S.n_elec = (int)ctx->elec.size();
@@ -569,8 +606,20 @@ int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charge
   ctx->M = m;
   CK(hipMalloc(&ctx->d_body, sizeof(double) * 3 * m));
   CK(hipMalloc(&ctx->d_q, sizeof(double) * m));
-  CK(hipMemcpy(ctx->d_body, body, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->d_q, charges, sizeof(double) * m, hipMemcpyHostToDevice));
+  // ligands larger than one wave's run are stored in spatial order; smaller
+  // ones (every atom in one pass of the force loop) keep the input order
+  std::vector<double> hb(body, body + 3 * (size_t)m), hq(charges, charges + m);
+  if (m > 64) {
+    std::vector<int> order;
+    lig_spatial_order(m, body, order);
+    for (int i = 0; i < m; ++i) {
+      const int a = order[i];
+      for (int d = 0; d < 3; ++d) hb[3 * (size_t)i + d] = body[3 * (size_t)a + d];
+      hq[i] = charges[a];
+    }
+  }
+  CK(hipMemcpy(ctx->d_body, hb.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->d_q, hq.data(), sizeof(double) * m, hipMemcpyHostToDevice));
   double c[3] = {0, 0, 0};
   int nc = 0;
   for (int i = 0; i < m; ++i)

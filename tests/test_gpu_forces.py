@@ -158,6 +158,32 @@ def test_protein_protein_bidirectional_born(prec, rel, frac):
     assert_close(F.cpu().numpy(), F0, sf, rel, frac, f"pp F {prec}")
 
 
+@pytest.mark.parametrize("prec,rel,frac,lanes", [("fp64", 1e-12, 1e-3, 16), ("fp64", 1e-12, 1e-3, 64),
+                                                 ("fp32", 1e-5, 5e-2, 32)])
+def test_pose_mode_spatially_ordered_ligand(prec, rel, frac, lanes):
+    """A 120-atom ligand is held on the device in spatial order: pose-mode
+    forces, lever-arm torques and energies still equal the oracle's sums in
+    input order (and every lane count walks the same set of atoms)."""
+    sysm = small_protein_protein()
+    assert len(sysm.mol2.charges_array()) > 64
+    g = numpy_grids(sysm)
+    eng = engine_for(sysm, g, precision=prec, torque_mode="physical", force_group=lanes)
+    body = sysm.mol2.positions_array() - sysm.mol2.centroid()
+    eng.set_ligand(body, sysm.mol2.charges_array())
+    rng = np.random.default_rng(18)
+    c, q = poses(150, rng, 8.0, 22.0)
+    F0, T0, E0, lab, R, sf, se = oracle_forces(sysm, g, c, q, torque_mode="physical", chunk=0)
+    eng_o = oracle_engine(sysm, g)
+    _, fa = OF.born_reverse(R, c, eng_o["rec_pos"], eng_o["rec_q"], eng_o["lig_born"][0],
+                            eng_o["alpha"], per_atom=True)
+    sf = sf + np.linalg.norm(fa, axis=2).sum(1)
+    Fp, Tp, Ep = eng.forces_pose(c, q)
+    lever = np.linalg.norm(lab - c[:, None, :], axis=2).max(1)
+    assert_close(Fp.cpu().numpy(), F0, sf, rel, frac, f"ordered F {prec}")
+    assert_close(Tp.cpu().numpy(), T0, sf * lever, rel, frac, f"ordered T {prec}")
+    assert_close(Ep.cpu().numpy()[:, None], E0[:, None], se, rel, frac, f"ordered E {prec}")
+
+
 @pytest.mark.parametrize("lanes", [1, 4, 16, 64])
 def test_force_group_sizes_agree(lanes):
     sysm = small_protein_ligand()
