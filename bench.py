@@ -39,7 +39,7 @@ METRIC = "trajectory-steps/sec on 1 MI355X, surrogate vs plain at equal k_on (Wi
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_TRAJ = 2 * (9 * 256 + 3 * 256 * 256 + 256 * 13)   # 404,480
-TUNE_STEPS = 130  # at most 3 rounds x 42 launch candidates x 1 step; the warm-up is topped up
+TUNE_STEPS = 150  # at most 3 rounds x 42 launch candidates + a 4 x 4 runoff; the warm-up is topped up
 FORCE_TIMING_STEPS = 20
 
 

@@ -216,11 +216,14 @@ int pst_prod_counters(pst_sim* sim, double* c16);
  * atoms, reverse-Born receptor atoms), at most 64), resident-workgroup cap in
  * {1024 ... 8192} and the XCD-chunked list mapping on/off; and, when the
  * receptor grids form a float2 pair, the brick-binned evaluation with G >= 8
- * and caps {1024, 4096}.  Each candidate runs `steps_per_cand` real
- * production steps per round, `rounds` round-robin passes, force evaluation
- * timed by HIP events; the fastest becomes the launch of later steps.
- * table (max_rows,5) = {mode, G, cap, xcd, mean ms} per candidate
- * (mode 0 direct, 1 binned); best5 = {mode, G, cap, xcd, steps run}.
+ * and caps {1024, 4096}; in FP32, the direct kernel with z-pair fetches at
+ * caps {2048, 8192}.  Each candidate runs `steps_per_cand` real production
+ * steps per round, `rounds` round-robin passes, its force launch timed by HIP
+ * events; the 4 fastest then run 4 more interleaved passes and the fastest of
+ * that runoff becomes the launch of later steps.
+ * table (max_rows,5) = {mode, G, cap, xcd, mean ms of the round-robin passes}
+ * per candidate (mode 0 direct, 1 binned, 2 z-pair);
+ * best5 = {mode, G, cap, xcd, steps run}.
  * Both may be NULL. */
 int pst_prod_autotune(pst_sim* sim, int rounds, int steps_per_cand, double* table, int max_rows,
                       int* n_rows, int* best5);

@@ -1506,21 +1506,21 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
     fill_sys(ctx, S);
     binned = bins_match(ctx, S);
   }
-  struct Cand { int mode, G, cap, xcd; double ms; int n; };
+  struct Cand { int mode, G, cap, xcd; double ms; int n; double ms2; int n2; };
   std::vector<Cand> cands;
   const int caps[] = {1024, 2048, 4096, 8192};
   for (int g : Gs)
     for (int c : caps)
-      for (int x = 0; x < 2; ++x) cands.push_back({0, g, c, x, 0.0, 0});
+      for (int x = 0; x < 2; ++x) cands.push_back({0, g, c, x, 0.0, 0, 0.0, 0});
   if (binned)
     for (int g : Gs)
       if (g >= 8)
-        for (int c : {1024, 4096}) cands.push_back({1, g, c, 1, 0.0, 0});
+        for (int c : {1024, 4096}) cands.push_back({1, g, c, 1, 0.0, 0, 0.0, 0});
   // FP32: the direct kernel with z-pair fetches
   if (s->precision == 1)
     for (int g : Gs)
       for (int c : {2048, 8192})
-        for (int x = 0; x < 2; ++x) cands.push_back({2, g, c, x, 0.0, 0});
+        for (int x = 0; x < 2; ++x) cands.push_back({2, g, c, x, 0.0, 0, 0.0, 0});
   CK(hipEventCreate(&s->tev0));
   CK(hipEventCreate(&s->tev1));
   RunCounters h;
@@ -1528,35 +1528,51 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
   bool stop = false;
   long long k = 0;
   int rc = 0;
-  for (int r = 0; r < rounds && !stop && !rc; ++r) {
-    for (auto& c : cands) {
-      if (stop || rc) break;
-      s->tune_mode = c.mode; s->tune_G = c.G; s->tune_cap = c.cap; s->tune_xcd = c.xcd;
-      for (int i = 0; i < steps_per_cand && !stop; ++i) {
-        rc = enqueue_step(s, (int)(k++ & 1));
-        if (rc) break;
-        hipError_t e = hipEventSynchronize(s->tev1);
-        float ms = 0;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->tev0, s->tev1);
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = 2; break; }
-        c.ms += ms;
-        c.n += 1;
-        if (h.stop || (h.n_sr + h.n_mlp) == 0) stop = true;
-      }
-    }
-  }
+  // one production step under candidate c; its force launch time is added to *acc
+  auto timed_step = [&](const Cand& c, double* acc, int* cnt) {
+    s->tune_mode = c.mode; s->tune_G = c.G; s->tune_cap = c.cap; s->tune_xcd = c.xcd;
+    rc = enqueue_step(s, (int)(k++ & 1));
+    if (rc) return;
+    hipError_t e = hipEventSynchronize(s->tev1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->tev0, s->tev1);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = 2; return; }
+    *acc += ms;
+    *cnt += 1;
+    if (h.stop || (h.n_sr + h.n_mlp) == 0) stop = true;
+  };
+  for (int r = 0; r < rounds && !stop && !rc; ++r)
+    for (auto& c : cands)
+      for (int i = 0; i < steps_per_cand && !stop && !rc; ++i) timed_step(c, &c.ms, &c.n);
+  // runoff: the kRunoff fastest candidates of the round-robin passes (those
+  // measured in every pass) are timed again over kRunoffRounds interleaved
+  // passes, and the choice rests on the runoff alone; the trajectory
+  // population drifts from step to step, so single-step timings of launches
+  // a few per cent apart can swap order
+  constexpr int kRunoff = 4, kRunoffRounds = 4;
+  int nmax = 0;
+  for (auto& c : cands) nmax = c.n > nmax ? c.n : nmax;
+  std::vector<Cand*> fin;
+  for (auto& c : cands)
+    if (c.n == nmax && nmax > 0) fin.push_back(&c);
+  std::stable_sort(fin.begin(), fin.end(), [](const Cand* a, const Cand* b) { return a->ms < b->ms; });
+  if (fin.size() > (size_t)kRunoff) fin.resize(kRunoff);
+  if (fin.size() > 1)
+    for (int r = 0; r < kRunoffRounds && !stop && !rc; ++r)
+      for (Cand* c : fin)
+        for (int i = 0; i < steps_per_cand && !stop && !rc; ++i) timed_step(*c, &c->ms2, &c->n2);
   hipEventDestroy(s->tev0);
   hipEventDestroy(s->tev1);
   s->tev0 = s->tev1 = nullptr;
-  // choose among candidates measured in every round-robin pass
-  int nmax = 0;
-  for (auto& c : cands) nmax = c.n > nmax ? c.n : nmax;
+  int n2max = 0;
+  for (Cand* c : fin) n2max = c->n2 > n2max ? c->n2 : n2max;
   const Cand* best = nullptr;
-  for (auto& c : cands)
-    if (c.n == nmax && nmax > 0 && (!best || c.ms < best->ms)) best = &c;
+  for (Cand* c : fin) {
+    const double a = n2max > 0 ? c->ms2 : c->ms, b = best ? (n2max > 0 ? best->ms2 : best->ms) : 0.0;
+    if ((n2max == 0 || c->n2 == n2max) && (!best || a < b)) best = c;
+  }
   if (best) {
     s->tune_mode = best->mode; s->tune_G = best->G; s->tune_cap = best->cap; s->tune_xcd = best->xcd;
   } else {

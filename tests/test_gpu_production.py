@@ -78,10 +78,12 @@ def test_autotune_measures_every_candidate_and_keeps_outcomes():
     tune = b.sim_data["autotune"]
     ms = [r["ms"] for r in tune["table"]]
     assert len(ms) >= 10 and min(ms) > 0
+    # the choice comes from a runoff among the 4 fastest of the round-robin passes
     best = tune["best"]
     key = ("mode", "G", "cap", "xcd")
     row = [r for r in tune["table"] if all(r[k] == best[k] for k in key)]
-    assert row and row[0]["ms"] == min(ms)
+    assert row and row[0]["ms"] <= sorted(ms)[3]
+    assert tune["steps"] <= 3 * len(ms) + 4 * 4
     o1, o2 = a.sim_data["outcome"], b.sim_data["outcome"]
     both = (o1 > 0) & (o2 > 0)
     agree = (o1[both] == o2[both]) & (a.sim_data["n_steps"][both] == b.sim_data["n_steps"][both])
