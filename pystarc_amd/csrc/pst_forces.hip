@@ -91,8 +91,11 @@ __device__ inline bool atom_reverse(const ForceSys& S, double x, double y, doubl
 
 // One system's force evaluation by workgroup `blk` of `nblk` (the workgroups
 // of k_forces, or the slice of a grouped launch that belongs to the system).
+// Always inlined: called as a function from k_forces_group it costs a stack
+// frame in scratch and the call convention's full register file (248-256
+// VGPRs, one wave per SIMD) against 168-198 VGPRs inlined.
 template <typename TC, int G, bool POSE, bool ZP = false>
-__device__ inline void forces_body(const ForceSys& S, const ForceWork& W, int blk, int nblk) {
+__device__ __forceinline__ void forces_body(const ForceSys& S, const ForceWork& W, int blk, int nblk) {
   typedef typename Vec4<TC>::type V4;
   extern __shared__ char smem_raw[];
   V4* s_lig = reinterpret_cast<V4*>(smem_raw);
@@ -269,13 +272,18 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
 // launch: workgroup b belongs to system l with start[l] <= b < start[l + 1]
 // (start[] multiples of 8, so the XCD of a workgroup is that of its slice).
 template <typename TC, int G>
-__global__ void __launch_bounds__(kForceBlock) k_forces_group(const ForceSys* Sg,
-                                                              const ForceWork* Wg,
-                                                              const int* start, int L) {
+__global__ void __launch_bounds__(kForceBlock) k_forces_group(const ForceSys* __restrict__ Sg,
+                                                              const ForceWork* __restrict__ Wg,
+                                                              const int* __restrict__ start, int L) {
   const int b = blockIdx.x;
   int l = 0;
   while (l + 1 < L && b >= start[l + 1]) ++l;
-  forces_body<TC, G, true>(Sg[l], Wg[l], b - start[l], start[l + 1] - start[l]);
+  // l is the same for the whole workgroup: made wave-uniform, the system's
+  // records are read with scalar loads instead of being held per lane
+  l = __builtin_amdgcn_readfirstlane(l);
+  const int b0 = __builtin_amdgcn_readfirstlane(start[l]);
+  const int b1 = __builtin_amdgcn_readfirstlane(start[l + 1]);
+  forces_body<TC, G, true>(Sg[l], Wg[l], b - b0, b1 - b0);
 }
 
 template <typename TC, int G>
