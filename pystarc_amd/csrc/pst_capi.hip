@@ -12,6 +12,7 @@
 #include "../../include/pystarc_hip.h"
 #include "pst_kernels.hpp"
 #include "pst_mlp.hpp"
+#include "pst_order.hpp"
 #include "pst_surrogate.hpp"
 
 using namespace pst;
@@ -183,98 +184,6 @@ static void rec_cells(int n, const double* pos, int* cn, double* co, std::vector
   for (int j = 0; j < n; ++j) {
     const int p = cur[cell[j]]++;
     for (int d = 0; d < 3; ++d) cpos[3 * (size_t)p + d] = pos[3 * j + d];
-  }
-}
-
-// Spatial order inside a span of atom indices.  The pose-mode force loops
-// hand atom i of a run to lane i mod G (G = 16..64), so the atoms one wave
-// gathers together are an aligned run of 64 or 32 entries.  Recursive
-// bisection along the longest edge of the bounding box, at multiples of 64
-// while a span holds more than 64 atoms and at 32, then 16, below that (ties
-// broken by atom index), makes every such run a compact cluster: its stencils
-// share grid bricks and cache lines.  Only the order of each sum changes.
-static void bisect_runs(const double* pos, int* idx, int n) {
-  struct Span { int lo, hi; };
-  std::vector<Span> todo{{0, n}};
-  while (!todo.empty()) {
-    const Span s = todo.back();
-    todo.pop_back();
-    const int cnt = s.hi - s.lo;
-    if (cnt <= 16) continue;
-    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int j = s.lo; j < s.hi; ++j)
-      for (int d = 0; d < 3; ++d) {
-        mn[d] = fmin(mn[d], pos[3 * idx[j] + d]);
-        mx[d] = fmax(mx[d], pos[3 * idx[j] + d]);
-      }
-    int ax = 0;
-    for (int d = 1; d < 3; ++d) if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
-    const int run = cnt > 64 ? 64 : cnt > 32 ? 32 : 16;
-    const int mid = s.lo + (((cnt + run - 1) / run) / 2) * run;
-    std::nth_element(idx + s.lo, idx + mid, idx + s.hi, [&](int a, int b) {
-      const double pa = pos[3 * a + ax], pb = pos[3 * b + ax];
-      return pa < pb || (pa == pb && a < b);
-    });
-    todo.push_back({mid, s.hi});
-    todo.push_back({s.lo, mid});
-  }
-}
-
-// Recursive bisection of the receptor atoms into runs of PST_REC_CHUNK
-// (the last run takes the remainder): split along the longest edge of the
-// bounding box at a multiple of PST_REC_CHUNK atoms, ties broken by atom
-// index; when there is more than one run, each is then put in the nested
-// spatial order of bisect_runs (its aligned halves of 32 and quarters of 16
-// atoms are compact as well), else kept in input order.  spheres gets (centre,
-// radius) per run, the radius rounded up so that |r_j - centre| <= radius
-// holds for every atom of the run in floating point.
-static void rec_chunk_order(int n, const double* pos, std::vector<int>& order,
-                            std::vector<double>& spheres) {
-  order.resize(n);
-  for (int j = 0; j < n; ++j) order[j] = j;
-  struct Span { int lo, hi; };
-  std::vector<Span> todo{{0, n}}, leaves;
-  while (!todo.empty()) {
-    const Span s = todo.back();
-    todo.pop_back();
-    const int cnt = s.hi - s.lo;
-    if (cnt <= PST_REC_CHUNK) { leaves.push_back(s); continue; }
-    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int j = s.lo; j < s.hi; ++j)
-      for (int d = 0; d < 3; ++d) {
-        mn[d] = fmin(mn[d], pos[3 * order[j] + d]);
-        mx[d] = fmax(mx[d], pos[3 * order[j] + d]);
-      }
-    int ax = 0;
-    for (int d = 1; d < 3; ++d) if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
-    const int chunks = (cnt + PST_REC_CHUNK - 1) / PST_REC_CHUNK;
-    const int mid = s.lo + (chunks / 2) * PST_REC_CHUNK;
-    std::nth_element(order.begin() + s.lo, order.begin() + mid, order.begin() + s.hi,
-                     [&](int a, int b) {
-                       const double pa = pos[3 * a + ax], pb = pos[3 * b + ax];
-                       return pa < pb || (pa == pb && a < b);
-                     });
-    todo.push_back({mid, s.hi});
-    todo.push_back({s.lo, mid});
-  }
-  std::sort(leaves.begin(), leaves.end(), [](const Span& a, const Span& b) { return a.lo < b.lo; });
-  spheres.assign(4 * leaves.size(), 0.0);
-  for (size_t k = 0; k < leaves.size(); ++k) {
-    const Span s = leaves[k];
-    std::sort(order.begin() + s.lo, order.begin() + s.hi);
-    if (n > PST_REC_CHUNK) bisect_runs(pos, order.data() + s.lo, s.hi - s.lo);
-    double c[3] = {0, 0, 0};
-    for (int j = s.lo; j < s.hi; ++j)
-      for (int d = 0; d < 3; ++d) c[d] += pos[3 * order[j] + d];
-    for (int d = 0; d < 3; ++d) c[d] /= (double)(s.hi - s.lo);
-    double r = 0;
-    for (int j = s.lo; j < s.hi; ++j) {
-      const double* p = pos + 3 * order[j];
-      r = fmax(r, sqrt((p[0] - c[0]) * (p[0] - c[0]) + (p[1] - c[1]) * (p[1] - c[1]) +
-                       (p[2] - c[2]) * (p[2] - c[2])));
-    }
-    spheres[4 * k] = c[0]; spheres[4 * k + 1] = c[1]; spheres[4 * k + 2] = c[2];
-    spheres[4 * k + 3] = r * (1.0 + 1e-12) + 1e-9;
   }
 }
 
@@ -606,18 +515,14 @@ int pst_set_ligand(pst_ctx* ctx, int m, const double* body, const double* charge
   ctx->M = m;
   CK(hipMalloc(&ctx->d_body, sizeof(double) * 3 * m));
   CK(hipMalloc(&ctx->d_q, sizeof(double) * m));
-  // ligands larger than one wave's run are stored in spatial order; smaller
-  // ones (every atom in one pass of the force loop) keep the input order
-  std::vector<double> hb(body, body + 3 * (size_t)m), hq(charges, charges + m);
-  if (m > 64) {
-    std::vector<int> order(m);
-    for (int i = 0; i < m; ++i) order[i] = i;
-    bisect_runs(body, order.data(), m);
-    for (int i = 0; i < m; ++i) {
-      const int a = order[i];
-      for (int d = 0; d < 3; ++d) hb[3 * (size_t)i + d] = body[3 * (size_t)a + d];
-      hq[i] = charges[a];
-    }
+  // device order of the atoms (spatial for ligands of more than 64 atoms)
+  std::vector<int> order;
+  ligand_order(m, body, order);
+  std::vector<double> hb(3 * (size_t)m), hq(m);
+  for (int i = 0; i < m; ++i) {
+    const int a = order[i];
+    for (int d = 0; d < 3; ++d) hb[3 * (size_t)i + d] = body[3 * (size_t)a + d];
+    hq[i] = charges[a];
   }
   CK(hipMemcpy(ctx->d_body, hb.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice));
   CK(hipMemcpy(ctx->d_q, hq.data(), sizeof(double) * m, hipMemcpyHostToDevice));
