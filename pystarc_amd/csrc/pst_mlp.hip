@@ -55,8 +55,22 @@ template <int AL, int BL, int EPI>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) __bf16 As[64][32 + kPad];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[64][32 + kPad];
-  const int tile_n = blockIdx.x * 64;
-  const int tile_m = blockIdx.y * 64;
+  // Workgroups go round-robin over the 8 XCDs by linear id.  With the M tiles
+  // a multiple of 8 (no K split), launch ids are regrouped so that the N
+  // tiles of one M tile (which read the same A rows) run on one XCD and share
+  // its L2, while consecutive M tiles still alternate over the XCDs (the
+  // rows past *m_dev stay spread evenly).  Each tile's arithmetic is
+  // unchanged.
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (gridDim.z == 1 && (gridDim.y & 7) == 0) {
+    const int L = blockIdx.y * gridDim.x + blockIdx.x;
+    const int per = 8 * gridDim.x;
+    const int grp = L / per, r = L - grp * per;
+    by = grp * 8 + (r & 7);
+    bx = r >> 3;
+  }
+  const int tile_n = bx * 64;
+  const int tile_m = by * 64;
   if (g.m_dev && tile_m >= *g.m_dev) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
