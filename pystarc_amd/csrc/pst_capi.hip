@@ -1230,7 +1230,7 @@ static bool audits(const pst_sim* s) {
   return s->mlp && s->P.surrogate_r > 0 && s->sp.audit_n > 0;
 }
 
-// The plain path judges the surrogate on the first audit_n surrogate
+// The plain path checks the surrogate on the first audit_n surrogate
 // trajectories: their grid-path forces into audit_F.
 static hipError_t launch_audit_forces(pst_sim* s, const ForceSys& S, const ForceWork& W,
                                       hipStream_t st) {

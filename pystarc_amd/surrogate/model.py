@@ -8,7 +8,7 @@ library's BF16-MFMA kernels (pst_mlp_forward / pst_mlp_backward); PyTorch
 holds the FP32 master parameters and the Adam state only.
 
 Targets are the plain FP64 grid path's own outputs on sampled poses (the
-plain path is the judge), divided by a screened-Coulomb envelope
+plain path is the reference), divided by a screened-Coulomb envelope
 env(r) = e^{-(r-r0)/lam} (r0/r)^2 (1 + r/lam)/(1 + r0/lam) and a per-group
 scale, so every target is O(1) across the shell.  The receptor force and
 torque follow from momentum and angular-momentum balance:

@@ -1,5 +1,5 @@
 """The learned surrogate in the production step on the protein-protein
-system: its force error against the FP64 grid path (the judge), the audit
+system: its force error against the FP64 grid path (its reference), the audit
 inside the step, k_on against the plain path's Wilson interval, and the
 throughput it gains over the plain path."""
 import time
