@@ -1,14 +1,18 @@
 """Host-side numerics of the package (CPU): Romberg quadrature
-(pystarc_amd/lib/numerical.py, used for k_b and the return probability) and
+(pystarc_amd/lib/numerical.py, used for k_b and the return probability),
 the receptor multipole moments of the Yukawa far-field
-(pystarc_amd/forces/multipole_farfield.py), each pinned by closed forms and
-invariants rather than by its own formula."""
+(pystarc_amd/forces/multipole_farfield.py) and the Stokes / RPY
+hydrodynamics (pystarc_amd/hydrodynamics/rotne_prager.py), each pinned by
+closed forms, limits and invariants rather than by its own formula."""
 import math
 
 import numpy as np
 import pytest
 
 from pystarc_amd.forces.multipole_farfield import MultipoleExpansion
+from pystarc_amd.hydrodynamics.rotne_prager import (MobilityTensor, _tt_coeffs, rpy_offdiagonal,
+                                                    stokes_rotational_diffusion,
+                                                    stokes_translational_diffusion)
 from pystarc_amd.lib.numerical import romberg_integrate
 
 
@@ -87,3 +91,39 @@ def test_neutral_dipole_does_not_depend_on_the_origin():
     q -= q.mean()                                        # net charge zero
     shift = np.array([10.0, -4.0, 3.0])
     assert np.allclose(moments(pos + shift, q).dipole, moments(pos, q).dipole)
+
+
+# --- host hydrodynamics (pystarc_amd/hydrodynamics/rotne_prager.py) -----------
+
+
+def test_stokes_reference_value_and_scaling():
+    # D_rel of two 1.005 A spheres is 0.43371 A^2/ps (reference tests:3992-3995)
+    m = MobilityTensor.from_radii(1.005, 1.005)
+    assert m.relative_translational_diffusion() == pytest.approx(0.43371, abs=5e-5)
+    # kT/(6 pi eta a) over kT/(8 pi eta a^3) = 4 a^2 / 3
+    for a in (1.0, 15.0, 17.0):
+        ratio = stokes_translational_diffusion(a) / stokes_rotational_diffusion(a)
+        assert ratio == pytest.approx(4 * a * a / 3, rel=1e-12)
+    assert stokes_translational_diffusion(10.0) == pytest.approx(
+        2 * stokes_translational_diffusion(20.0), rel=1e-12)
+
+
+def test_rpy_coupling_limits():
+    a, b = 17.0, 15.0
+    # far apart: the Oseen tensor, both coefficients 1/(8 pi r)
+    cI, cU = _tt_coeffs(1e6, a, b)
+    assert cI == pytest.approx(1 / (8 * math.pi * 1e6), rel=1e-9)
+    assert cU == pytest.approx(1 / (8 * math.pi * 1e6), rel=1e-9)
+    # continuous where the regimes meet (contact and full overlap)
+    for r0 in (a + b, abs(a - b)):
+        lo, hi = _tt_coeffs(r0 * (1 - 1e-10), a, b), _tt_coeffs(r0 * (1 + 1e-10), a, b)
+        assert lo == pytest.approx(hi, rel=1e-6, abs=1e-14)
+    # one sphere inside the other: the larger sphere's self mobility
+    assert _tt_coeffs(0.5, a, b) == (1 / (6 * math.pi * a), 0.0)
+    # symmetric, and reduces relative diffusion at contact
+    Da, Db = stokes_translational_diffusion(a), stokes_translational_diffusion(b)
+    M = rpy_offdiagonal([20.0, 25.0, 3.0], a, b, Da, Db)
+    assert np.allclose(M, M.T)
+    mt = MobilityTensor.from_radii(a, b)
+    assert mt.relative_translational_diffusion([32.0, 0, 0]) < Da + Db
+    assert mt.relative_translational_diffusion([1e7, 0, 0]) == pytest.approx(Da + Db, rel=1e-5)
