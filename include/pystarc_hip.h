@@ -101,6 +101,19 @@ int pst_last_force_eval(const pst_ctx* ctx, int* mode, int* G);
 int pst_forces_positions(pst_ctx* ctx, int n, int m, const double* lig_pos, const double* lig_q,
                          const double* rot, const double* cen, double* F, double* T, double* E,
                          int precision, int rev_skip);
+/* pst_forces_positions without allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as pst_forces_positions with the direct
+ * kernel (the context's gather mode is not used).  The reverse-Born chunk
+ * skip (rev_skip 1) keeps its per-chunk minimum centroid distances in a
+ * static device array of 65,536 doubles (512 KiB per GPU, held until the
+ * process ends): a call needing more chunks than that (more than 65,536 x
+ * floor(500 MiB / (24 N_rec)) trajectories) fails with an error.  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU race. */
+int pst_forces_positions_noalloc(pst_ctx* ctx, int n, int m, const double* lig_pos,
+                                 const double* lig_q, const double* rot, const double* cen,
+                                 double* F, double* T, double* E, int precision, int rev_skip);
 /* Fused variant: ligand atoms placed in registers from the pose (centroid
  * pos (n,3), quaternion quat (n,4) w,x,y,z) as gpu_batch_simulator.py:935-943
  * does, without materialising (n,m,3). */

@@ -79,6 +79,8 @@ _SIGS = {
     "pst_set_force_group": ([_vp, C.c_int], C.c_int),
     "pst_forces_positions": ([_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               C.c_int, C.c_int], C.c_int),
+    "pst_forces_positions_noalloc": ([_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      C.c_int, C.c_int], C.c_int),
     "pst_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int], C.c_int),
     "pst_time_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp],
                              C.c_int),

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/pystarc_hip.h"
 #include "pst_kernels.hpp"
@@ -658,20 +659,36 @@ static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const Force
   return launch_forces(S, W, precision, G, max_n, cap, ctx->stream);
 }
 
-static int forces_common(pst_ctx* ctx, ForceWork& W, int n, int precision, int fb) {
+// fixed_chunk != nullptr: the chunk-skip scratch is that buffer of
+// fixed_cap doubles (nothing is allocated; too many chunks is an error) and
+// the direct kernel runs whatever the context's gather mode.
+static int forces_common(pst_ctx* ctx, ForceWork& W, int n, int precision, int fb,
+                         double* fixed_chunk = nullptr, int fixed_cap = 0) {
   ForceSys S;
   fill_sys(ctx, S);
   if (W.rev_skip_mode == 1 && S.has_rev && n > 0) {
     int cs, cpf, nch;
     chunk_layout(ctx, n, fb, &cs, &cpf, &nch);
-    if (ensure_chunk(ctx, nch)) return 2;
-    CK(launch_chunk_min_r(W.pos, W.list, n, fb, cs, cpf, ctx->d_chunk, nch, ctx->stream));
-    W.chunk_min_r = ctx->d_chunk;
+    double* buf = fixed_chunk;
+    if (fixed_chunk) {
+      if (nch > fixed_cap) {
+        ctx->err = "pst_forces_positions_noalloc: " + std::to_string(n) + " trajectories need " +
+                   std::to_string(nch) + " reverse-Born chunks, more than the " +
+                   std::to_string(fixed_cap) + " the static scratch holds";
+        return 1;
+      }
+    } else {
+      if (ensure_chunk(ctx, nch)) return 2;
+      buf = ctx->d_chunk;
+    }
+    CK(launch_chunk_min_r(W.pos, W.list, n, fb, cs, cpf, buf, nch, ctx->stream));
+    W.chunk_min_r = buf;
     W.chunk_fb = fb;
     W.chunk_cs = cs;
     W.chunks_per_fb = cpf;
   }
-  CK(launch_force_eval(ctx, S, W, precision, group_for(ctx, S), n, 2048, ctx->gather_mode));
+  CK(launch_force_eval(ctx, S, W, precision, group_for(ctx, S), n, 2048,
+                       fixed_chunk ? 0 : ctx->gather_mode));
   return 0;
 }
 
@@ -714,6 +731,69 @@ int pst_forces_positions(pst_ctx* ctx, int n, int m, const double* lig_pos, cons
   W.rev_skip_mode = rev_skip;
   ctx->M = m;
   int rc = forces_common(ctx, W, n, precision, n);
+  ctx->M = saved_M;
+  return rc;
+}
+
+// ---- force operator without allocation ---------------------------------------
+// The reverse-Born chunk skip's per-chunk minimum centroid distance lives in a
+// file-scope device array (one copy per GPU, held until the process ends).
+// Host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name.  Every call on one GPU shares the array: calls that overlap
+// on one GPU race.  k_chunk_min_r writes every chunk it later reads, so the
+// array needs no clearing between calls.
+constexpr int kNoallocChunks = 65536;  // 512 KiB
+static __device__ __align__(256) double scratch[kNoallocChunks];
+constexpr int kNoallocMaxDevices = 64;
+static double* g_scratch_dev[kNoallocMaxDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+// Address of the device's copy of `scratch`, looked up once per device with
+// that device current (the caller has made it current).
+static int noalloc_scratch(pst_ctx* ctx, double** out) {
+  if (ctx->device < 0 || ctx->device >= kNoallocMaxDevices)
+    FAIL("pst_forces_positions_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ctx->err = std::string("pst_forces_positions_noalloc: hipGetSymbolAddress(scratch): ") +
+                 hipGetErrorString(e);
+      return 2;
+    }
+    g_scratch_dev[ctx->device] = (double*)p;
+  }
+  *out = g_scratch_dev[ctx->device];
+  return 0;
+}
+
+int pst_forces_positions_noalloc(pst_ctx* ctx, int n, int m, const double* lig_pos,
+                                 const double* lig_q, const double* rot, const double* cen,
+                                 double* F, double* T, double* E, int precision, int rev_skip) {
+  if (!ctx) return 1;
+  if (n < 0 || m <= 0) FAIL("pst_forces_positions_noalloc: bad sizes");
+  if (n == 0) return 0;
+  if (!lig_pos || !lig_q || !F) FAIL("pst_forces_positions_noalloc: null pointer");
+  if ((rot == nullptr) != (cen == nullptr))
+    FAIL("pst_forces_positions_noalloc: rot and cen go together");
+  if (precision != 0 && precision != 1)
+    FAIL("pst_forces_positions_noalloc: precision must be 0 or 1");
+  CK(set_dev(ctx));
+  double* chunk = nullptr;
+  if (noalloc_scratch(ctx, &chunk)) return 2;
+  const int saved_M = ctx->M;
+  ForceWork W;
+  memset(&W, 0, sizeof(W));
+  W.n = n;
+  W.pos = cen;
+  W.rot = rot;
+  W.lig_pos = lig_pos;
+  W.lig_q = lig_q;
+  W.F = F; W.T = T; W.E = E;
+  W.rev_skip_mode = rev_skip;
+  ctx->M = m;
+  int rc = forces_common(ctx, W, n, precision, n, chunk, kNoallocChunks);
   ctx->M = saved_M;
   return rc;
 }
