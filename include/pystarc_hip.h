@@ -119,6 +119,16 @@ int pst_forces_positions_noalloc(pst_ctx* ctx, int n, int m, const double* lig_p
  * does, without materialising (n,m,3). */
 int pst_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, double* F,
                     double* T, double* E, int precision);
+/* Several ligand contexts (receptor grids shared with pst_share_receptor_grids,
+ * each with its own ligand) evaluated in ONE launch of the grouped force
+ * kernel, as pst_group_run's step does: per context l, n[l] poses pos[l]
+ * (n,3) / quat[l] (n,4) and outputs F[l], T[l], E[l] (T / E arrays or their
+ * entries may be NULL).  Pointer arrays are host arrays of device pointers.
+ * Replaces one GPUBatchForceEngine.__call__ per ligand
+ * (gpu_batch_engine.py:531-605, one process per ligand in multi_GPU_runs.py). */
+int pst_forces_pose_group(pst_ctx* const* ctxs, int L, const int* n, const double* const* pos,
+                          const double* const* quat, double* const* F, double* const* T,
+                          double* const* E, int precision);
 /* Times the fused force kernel: `reps` launches on the context stream,
  * bracketed by HIP events; writes the mean launch time in ms and, when
  * census4 is non-NULL, {trajectories, elec-grid atoms, born-grid atoms,
@@ -252,6 +262,30 @@ int pst_prod_set_launch(pst_sim* sim, int mode, int G, int blocks_cap, int xcd);
  * receptor-Born-grid atoms, ligand-Born-grid (reverse) atoms}. */
 int pst_prod_time_forces(pst_sim* sim, int steps, double* out6);
 
+/* Checkpoint / resume of a production run (the reference saves pos, q,
+ * status, n_steps and counters to checkpoint.npz every checkpoint_interval
+ * completed trajectories and resumes from it, gpu_batch_simulator.py:548-575,
+ * :1400-1420).  The saved state is every device buffer that carries from one
+ * step to the next (slots, counters, outcomes, observables) behind a header
+ * and the run's StepParams; noise is keyed by (trajectory, own step), so a run
+ * resumed from it continues exactly as the uninterrupted run.  load requires
+ * a sim created with the same StepParams and initialised with the same record
+ * level.  bytes = pst_prod_state_bytes. */
+int pst_prod_state_bytes(pst_sim* sim, long long* bytes);
+int pst_prod_save_state(pst_sim* sim, void* host, long long bytes);
+int pst_prod_load_state(pst_sim* sim, const void* host, long long bytes);
+/* Path and energetics snapshots of the running trajectories that have moved
+ * (gpu_batch_simulator.py:1318-1340): path8 (n_slots,8) rows = trajectory,
+ * step, centroid x y z, q0 q1 q2; energy6 (n_slots,6) rows = trajectory, step,
+ * force x y z, dt; rows in trajectory order; host arrays (either may be
+ * NULL), n_rows written. */
+int pst_prod_snapshot(pst_sim* sim, long long step, double* path8, double* energy6, int* n_rows);
+/* Surrogate audit sums since the last pst_sim_reset_audit: out6 = {force
+ * squared error, squared grid-path force, audited rows, torque squared error,
+ * squared grid-path torque (torques audited when the surrogate applies its
+ * torque), 0}.  No reference counterpart. */
+int pst_prod_audit(pst_sim* sim, double* out6);
+
 /* ---- learned surrogate ---------------------------------------------------
  * No reference counterpart: a 9 -> 4x256 (SiLU) -> 13 MLP standing in for
  * the grid + Born + far-field interaction outside a contact shell, with
@@ -272,8 +306,10 @@ int pst_mlp_backward(pst_mlp* mlp, int B, const float* gy, float* gparams);
 /* Attaches (mlp != NULL) or detaches the surrogate branch of the production
  * step.  p9 = {r_on, r0, lambda_env, 1/b, scale_F, scale_T, scale_E,
  * use_torque, audit_n}: trajectories with centre distance > r_on take the
- * MLP force; the grid path re-evaluates audit_n of them per step and the
- * squared error / reference sums come back in counters16[12..14]. */
+ * MLP force; the grid path re-evaluates audit_n of them per step (a stride
+ * through the surrogate list from an offset that changes every step) and the
+ * squared error / reference sums come back in counters16[12..14] (forces) and
+ * pst_prod_audit (forces and torques). */
 int pst_sim_set_surrogate(pst_sim* sim, pst_mlp* mlp, const double* p9);
 int pst_sim_reset_audit(pst_sim* sim);
 

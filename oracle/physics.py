@@ -11,6 +11,10 @@ Restates, in plain NumPy / Python floats:
 * Stokes D_t / D_r              - pystarc/hydrodynamics/rotne_prager.py:43-64
 * RPY off-diagonal tt block     - rotne_prager.py:67-124
 * relative diffusion D_rel(r)   - rotne_prager.py:183-204
+* Ermak-McCammon translation    - gpu_batch_simulator.py:1085-1094
+* overlap rejection test        - gpu_batch_simulator.py:1102-1128
+* Brownian-bridge D_eff and crossing probability
+                                - gpu_batch_simulator.py:637-671, :1161-1181
 """
 from __future__ import annotations
 
@@ -199,3 +203,45 @@ def relative_translational_diffusion(D1, D2, a1, a2, r_vec=None):
     if r_vec is None or a1 <= 0 or a2 <= 0:
         return D0
     return max(D0 - (2.0 / 3.0) * np.trace(rpy_offdiagonal(r_vec, a1, a2, D1, D2)), 1e-12)
+
+
+def em_translate(pos, D, dt, F, xi):
+    """Ermak-McCammon move pos + D dt F + sqrt(2 D dt) xi
+    (gpu_batch_simulator.py:1085-1094); D, dt per row."""
+    D = np.asarray(D, dtype=np.float64)[:, None]
+    dt = np.asarray(dt, dtype=np.float64)[:, None]
+    return pos + D * dt * F + np.sqrt(2.0 * D * dt) * xi
+
+
+def overlap_rejected(centroids, rec_pos, thresh=1.5):
+    """True where the ligand centroid is closer than `thresh` to any receptor
+    atom (gpu_batch_simulator.py:1102-1128, threshold :774)."""
+    dmin = np.full(len(centroids), np.inf)
+    for c0 in range(0, rec_pos.shape[0], 4096):
+        dd = np.linalg.norm(centroids[:, None, :] - rec_pos[None, c0:c0 + 4096, :], axis=2)
+        dmin = np.minimum(dmin, dd.min(axis=1))
+    return dmin < thresh
+
+
+def bridge_d_eff(D_t, a_rec, a_lig, L_rec, L_lig):
+    """Diffusion coefficient of a reaction-pair distance
+    (gpu_batch_simulator.py:643-660): D_t + D_rot,rec L_rec^2 + D_rot,lig
+    L_lig^2, D_t split between the bodies in inverse proportion to their radii
+    and D_rot,k = D_trans,k * 3 / (4 a_k^2); L = lever of the pair atom from
+    its body's centre."""
+    a_sum = a_rec + a_lig if (a_rec + a_lig) > 0 else 1.0
+    Dtr = D_t * a_lig / a_sum
+    Dtl = D_t * a_rec / a_sum
+    Drr = Dtr * 3.0 / (4.0 * a_rec ** 2) if a_rec > 0 else 0.0
+    Drl = Dtl * 3.0 / (4.0 * a_lig ** 2) if a_lig > 0 else 0.0
+    return D_t + Drr * np.asarray(L_rec) ** 2 + Drl * np.asarray(L_lig) ** 2
+
+
+def bridge_crossing_prob(x0, x1, D_dt):
+    """Probability that a Brownian path from x0 to x1 over a step with
+    diffusion D dt touched 0 (gpu_batch_simulator.py:1161-1181): exp(-x0 x1 /
+    max(D dt, 1e-30)) where x0 > 0 and x1 > 0, else 0."""
+    D_dt = np.maximum(D_dt, 1e-30)
+    both = (x0 > 0) & (x1 > 0)
+    with np.errstate(over="ignore"):
+        return np.where(both, np.exp(-x0 * x1 / D_dt), 0.0)

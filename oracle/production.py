@@ -40,6 +40,7 @@ import numpy as np
 
 from . import forces as F_
 from . import philox as PH
+from . import physics as PHY
 from . import rpy12
 from .physics import apply_rotation, quats_to_rotmats, return_prob_and_kb, shoemake
 from .simulator import OracleSim
@@ -259,15 +260,11 @@ class ProductionOracle(OracleSim):
             f, g = PH.normal2(self.seed, run + self.traj_base, ns, PH.RNG_NOISE, 2)
             nt = np.stack([a, b, c], axis=1)
             nrot = np.stack([e, f, g], axis=1)
-            newp = pos0 + (D * dt)[:, None] * Fo + np.sqrt(2.0 * D * dt)[:, None] * nt
+            newp = PHY.em_translate(pos0, D, dt, Fo, nt)
             sr = np.sqrt(2.0 * self.D_r * dt)[:, None]
             newq = apply_rotation(q0, self.D_r * dt[:, None] * To + sr * nrot)
         if self.use_overlap:
-            dmin = np.full(n, np.inf)
-            for c0 in range(0, self.rec_pos.shape[0], 4096):
-                dd = np.linalg.norm(newp[:, None, :] - self.rec_pos[None, c0:c0 + 4096, :], axis=2)
-                dmin = np.minimum(dmin, dd.min(axis=1))
-            bad = dmin < 1.5
+            bad = PHY.overlap_rejected(newp, self.rec_pos)
             newp[bad] = pos0[bad]
             newq[bad] = q0[bad]
             self.n_overlap_rejected += int(bad.sum())
@@ -287,10 +284,7 @@ class ProductionOracle(OracleSim):
                 if p + 1 < P:
                     u[:, p + 1] = ub
             Dv = self.bb_D_eff if self.bb_D_eff is not None else np.full(P, self.D_t)
-            Ddt = np.maximum(Dv[None, :] * dt[:, None], 1e-30)
-            both = (x0 > 0) & (x1 > 0)
-            with np.errstate(over="ignore"):
-                pc = np.where(both, np.exp(-x0 * x1 / Ddt), 0.0)
+            pc = PHY.bridge_crossing_prob(x0, x1, Dv[None, :] * dt[:, None])
             pf = (u < pc) | (x1 < 0.0)
             if self.sm is not None:
                 rx = self._sm_lowest(run, pf)

@@ -23,7 +23,8 @@ import math
 import numpy as np
 
 from . import forces as F_
-from .physics import (apply_rotation, quats_to_rotmats, return_prob_and_kb, shoemake)
+from .physics import (apply_rotation, bridge_crossing_prob, bridge_d_eff, em_translate,
+                      overlap_rejected, quats_to_rotmats, return_prob_and_kb, shoemake)
 
 
 class OracleSim:
@@ -75,15 +76,9 @@ class OracleSim:
         # Brownian-bridge D_eff (:643-660)
         self.bb_D_eff = None
         if P:
-            a_r, a_l = float(s["a_rec"]), float(s["a_lig"])
-            a_sum = a_r + a_l if (a_r + a_l) > 0 else 1.0
-            Dtr = self.D_t * a_l / a_sum
-            Dtl = self.D_t * a_r / a_sum
-            Drr = Dtr * 3.0 / (4.0 * a_r ** 2) if a_r > 0 else 0.0
-            Drl = Dtl * 3.0 / (4.0 * a_l ** 2) if a_l > 0 else 0.0
-            L_rec = np.linalg.norm(self.rec_gho, axis=1)
-            L_lig = np.linalg.norm(self.lig_gho_body, axis=1)
-            self.bb_D_eff = self.D_t + Drr * L_rec ** 2 + Drl * L_lig ** 2
+            self.bb_D_eff = bridge_d_eff(self.D_t, float(s["a_rec"]), float(s["a_lig"]),
+                                         np.linalg.norm(self.rec_gho, axis=1),
+                                         np.linalg.norm(self.lig_gho_body, axis=1))
         # HI constants (:741-751)
         self.use_hi = bool(params.get("hydrodynamic_interactions", False))
         if self.use_hi:
@@ -226,22 +221,15 @@ class OracleSim:
         self.last["dt"] = dt
         n_sr = len(sr_idx)
         nt = rng.standard_normal((n_sr, 3))
-        sig = np.sqrt(2.0 * D * dt)[:, None]
-        drift = D[:, None] * dt[:, None] * Fo
         old_pos = pos[sr_idx].copy()
         old_q = q[sr_idx].copy()
-        pos[sr_idx] += drift + sig * nt
+        pos[sr_idx] = em_translate(old_pos, D, dt, Fo, nt)
         nr = rng.standard_normal((n_sr, 3))
         sig_r = np.sqrt(2.0 * self.D_r * dt)[:, None]
         omega = self.D_r * dt[:, None] * To + sig_r * nr
         q[sr_idx] = apply_rotation(q[sr_idx], omega)
         if self.use_overlap:
-            newp = pos[sr_idx]
-            dmin = np.full(n_sr, np.inf)
-            for c0 in range(0, self.rec_pos.shape[0], 4096):
-                dd = np.linalg.norm(newp[:, None, :] - self.rec_pos[None, c0:c0 + 4096, :], axis=2)
-                dmin = np.minimum(dmin, dd.min(axis=1))
-            inside = dmin < 1.5
+            inside = overlap_rejected(pos[sr_idx], self.rec_pos)
             if inside.any():
                 ii = np.where(inside)[0]
                 pos[sr_idx[ii]] = old_pos[ii]
@@ -259,8 +247,7 @@ class OracleSim:
                     Ddt = self.bb_D_eff[None, :] * dt[:, None]
                 else:
                     Ddt = self.D_t * dt[:, None]
-                Ddt = np.maximum(Ddt, 1e-30)
-                pc = np.where(both, np.exp(-x0 * x1 / Ddt), 0.0)
+                pc = bridge_crossing_prob(x0, x1, Ddt)
                 u = rng.random(pc.shape)
                 fired = (u < pc) | (new_d < self.cut[None, :])
                 bb = fired.sum(axis=1) >= self.n_needed

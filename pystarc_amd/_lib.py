@@ -82,6 +82,7 @@ _SIGS = {
     "pst_forces_positions_noalloc": ([_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       C.c_int, C.c_int], C.c_int),
     "pst_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int], C.c_int),
+    "pst_forces_pose_group": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int], C.c_int),
     "pst_time_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp],
                              C.c_int),
     "pst_rpy12_eval": ([_vp, C.c_int, _vp, C.c_double, C.c_double, _vp, _vp], C.c_int),
@@ -111,6 +112,11 @@ _SIGS = {
     "pst_prod_counters": ([_vp, _vp], C.c_int),
     "pst_make_grid": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double,
                        C.c_double, C.c_double, C.c_int, _vp], C.c_int),
+    "pst_prod_state_bytes": ([_vp, C.POINTER(C.c_longlong)], C.c_int),
+    "pst_prod_save_state": ([_vp, _vp, C.c_longlong], C.c_int),
+    "pst_prod_load_state": ([_vp, _vp, C.c_longlong], C.c_int),
+    "pst_prod_snapshot": ([_vp, C.c_longlong, _vp, _vp, _ip], C.c_int),
+    "pst_prod_audit": ([_vp, _vp], C.c_int),
     "pst_mlp_param_count": ([], C.c_int),
     "pst_mlp_create": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
     "pst_mlp_destroy": ([_vp], C.c_int),

@@ -114,7 +114,16 @@ struct pst_sim {
   pst_mlp* mlp = nullptr;
   SurrParams sp;
   double* audit_F = nullptr;
+  double* audit_T = nullptr;
+  int* audit_list = nullptr;    // slot ids of the audited surrogate trajectories
+  int* audit_count = nullptr;   // their number (device)
   int audit_cap = 0;
+  // the captured step graph of pst_prod_run, re-launched while everything it
+  // captured (launch configuration, parameters, buffers, system) is unchanged
+  hipGraphExec_t g_exec = nullptr;
+  hipGraph_t g_graph = nullptr;
+  hipStream_t g_cap = nullptr;
+  std::vector<unsigned char> g_key;
   // force-launch configuration of the production step (autotuned or default)
   int tune_G = 0;          // lanes per trajectory; 0 = pick_group heuristic
   int tune_cap = 2048;     // cap on resident workgroups (grid-stride beyond)
@@ -815,6 +824,80 @@ int pst_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, 
   return forces_common(ctx, W, n, precision, n);
 }
 
+int pst_forces_pose_group(pst_ctx* const* ctxs, int L, const int* n, const double* const* pos,
+                          const double* const* quat, double* const* F, double* const* T,
+                          double* const* E, int precision) {
+  if (!ctxs || L <= 0 || !ctxs[0]) return 1;
+  pst_ctx* ctx = ctxs[0];
+  if (L > 64) FAIL("pst_forces_pose_group: at most 64 systems");
+  if (!n || !pos || !quat || !F) FAIL("pst_forces_pose_group: null pointer");
+  if (precision != 0 && precision != 1) FAIL("pst_forces_pose_group: precision must be 0 or 1");
+  for (int l = 0; l < L; ++l) {
+    if (!ctxs[l] || ctxs[l]->device != ctx->device)
+      FAIL("pst_forces_pose_group: every context on the same device");
+    if (ctxs[l]->M <= 0) FAIL("pst_forces_pose_group: ligand not set");
+    if (n[l] < 0) FAIL("pst_forces_pose_group: bad size");
+    if (n[l] > 0 && (!pos[l] || !quat[l] || !F[l])) FAIL("pst_forces_pose_group: null pointer");
+  }
+  CK(set_dev(ctx));
+  for (int l = 1; l < L; ++l) CK(hipStreamSynchronize(ctxs[l]->stream));
+  // descriptors and the workgroup partition as the grouped step builds them
+  std::vector<ForceSys> S(L);
+  std::vector<ForceWork> W(L);
+  std::vector<int> start(L + 1, 0);
+  int G = 16;
+  size_t lds = 0;
+  for (int l = 0; l < L; ++l) {
+    fill_sys(ctxs[l], S[l]);
+    memset(&W[l], 0, sizeof(ForceWork));
+    W[l].n = n[l];
+    W[l].pos = pos[l];
+    W[l].quat = quat[l];
+    W[l].F = F[l];
+    W[l].T = T ? T[l] : nullptr;
+    W[l].E = E ? E[l] : nullptr;
+    W[l].xcd_chunks = 1;
+    G = std::max(G, group_for(ctxs[l], S[l]));
+    if (S[l].M <= 1536) lds = std::max(lds, (size_t)S[l].M * (precision == 0 ? 32 : 16));
+  }
+  G = G >= 64 ? 64 : (G >= 32 ? 32 : 16);
+  const int gpb = 256 / G;
+  for (int l = 0; l < L; ++l) {
+    int b = (n[l] + gpb - 1) / gpb;
+    if (b > 2048) b = 2048;
+    b = (b + 7) / 8 * 8;
+    start[l + 1] = start[l] + b;
+  }
+  ForceSys* dS = nullptr;
+  ForceWork* dW = nullptr;
+  int* dstart = nullptr;
+  auto release = [&]() {
+    if (dS) hipFree(dS);
+    if (dW) hipFree(dW);
+    if (dstart) hipFree(dstart);
+  };
+  hipError_t e = hipMalloc(&dS, sizeof(ForceSys) * L);
+  if (e == hipSuccess) e = hipMalloc(&dW, sizeof(ForceWork) * L);
+  if (e == hipSuccess) e = hipMalloc(&dstart, sizeof(int) * (L + 1));
+  if (e == hipSuccess) e = hipMemcpy(dS, S.data(), sizeof(ForceSys) * L, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dW, W.data(), sizeof(ForceWork) * L, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dstart, start.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_forces_group(dS, dW, dstart, L, start[L], lds, precision, G, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  release();
+  if (e != hipSuccess) {
+    ctx->err = std::string("pst_forces_pose_group: ") + hipGetErrorString(e);
+    return 2;
+  }
+  for (int l = 0; l < L; ++l) {
+    ctxs[l]->last_mode = 0;
+    ctxs[l]->last_G = G;
+  }
+  return 0;
+}
+
 int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, double* F,
                          double* T, double* E, int precision, int reps, double* ms_per_launch,
                          double* census4) {
@@ -975,8 +1058,11 @@ int pst_sim_destroy(pst_sim* s) {
   void* ptrs[] = {s->pos, s->quat, s->status, s->F, s->T, s->dt, s->x0, s->x1,
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
                   s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
-                  s->audit_F, s->key, s->bins, s->census, s->sm_state, s->sm_mask,
-                  s->lmz_escaped};
+                  s->audit_F, s->audit_T, s->audit_list, s->audit_count, s->key, s->bins,
+                  s->census, s->sm_state, s->sm_mask, s->lmz_escaped};
+  if (s->g_exec) hipGraphExecDestroy(s->g_exec);
+  if (s->g_graph) hipGraphDestroy(s->g_graph);
+  if (s->g_cap) hipStreamDestroy(s->g_cap);
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (void* p : s->obs_allocs) hipFree(p);
@@ -1310,18 +1396,23 @@ static bool audits(const pst_sim* s) {
   return s->mlp && s->P.surrogate_r > 0 && s->sp.audit_n > 0;
 }
 
-// The plain path checks the surrogate on the first audit_n surrogate
-// trajectories: their grid-path forces into audit_F.
+// The plain path checks the surrogate on audit_n surrogate trajectories per
+// step, picked by surr_audit_pick (a stride through the surrogate list from an
+// offset that moves with the step counter): their grid-path forces and
+// torques into audit_F / audit_T.
 static hipError_t launch_audit_forces(pst_sim* s, const ForceSys& S, const ForceWork& W,
                                       hipStream_t st) {
+  hipError_t e = surr_audit_pick(s->mlp_list, &s->ctr->n_mlp, s->sp.audit_n, s->ctr,
+                                 s->audit_list, s->audit_count, st);
+  if (e != hipSuccess) return e;
   ForceWork A = W;
-  A.list = s->mlp_list;
-  A.n_dev = &s->ctr->n_mlp;
+  A.list = s->audit_list;
+  A.n_dev = s->audit_count;
   A.n_cap = s->sp.audit_n;
   A.out_compact = 1;
   A.xcd_chunks = 0;
   A.F = s->audit_F;
-  A.T = nullptr;
+  A.T = s->audit_T;
   return launch_forces(S, A, s->precision, group_for(s->ctx, S), s->sp.audit_n, 2048, st);
 }
 
@@ -1354,8 +1445,8 @@ static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool 
         CK(hipStreamWaitEvent(ctx->stream, s->join_ev, 0));
       else
         CK(launch_audit_forces(s, S, W, ctx->stream));
-      CK(surr_audit(s->mlp_list, &s->ctr->n_mlp, s->sp.audit_n, s->F, s->audit_F, s->ctr,
-                    ctx->stream));
+      CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
+                    s->sp.use_torque ? s->audit_T : nullptr, s->ctr, ctx->stream));
     }
   }
   return 0;
@@ -1395,6 +1486,95 @@ static int enqueue_step(pst_sim* s, int par) {
   return enqueue_post(s, S, W, par, forked);
 }
 
+// Everything a captured step graph depends on: the launch configuration,
+// the step and surrogate parameters (kernel arguments by value), the system
+// record of the context and every buffer address a kernel argument holds.
+static void graph_key(pst_sim* s, int gs, std::vector<unsigned char>& key) {
+  pst_ctx* ctx = s->ctx;
+  key.clear();
+  auto put = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    key.insert(key.end(), b, b + n);
+  };
+  const int cfg[9] = {gs, s->tune_G, s->tune_cap, s->tune_xcd, s->tune_mode, ctx->gather_mode,
+                      (int)s->census_on, s->precision, s->audit_cap};
+  put(cfg, sizeof(cfg));
+  put(&s->P, sizeof(StepParams));
+  ForceSys S;
+  fill_sys(ctx, S);
+  put(&S, sizeof(S));
+  const void* ptrs[] = {s->mlp, s->mlp ? (const void*)s->mlp->m.x0 : nullptr, s->audit_F,
+                        s->audit_T, s->audit_list, s->audit_count, s->out_status, s->out_steps,
+                        s->mlp ? (const void*)s->mlp->m.y : nullptr, s->obs.traj, s->obs.hist,
+                        s->obs.s_min, s->obs.s_step, s->obs.s_time, s->obs.s_ret,
+                        s->obs.s_prev_r, s->obs.s_nm, s->aux, ctx->stream,
+                        ctx->bins.hist, ctx->bins.items, ctx->bins.sorted, ctx->bins.out,
+                        ctx->bins.out_rev, s->tev0, s->tev1};
+  put(ptrs, sizeof(ptrs));
+  if (s->mlp) {
+    put(&s->sp, sizeof(SurrParams));
+    put(&s->mlp->m, sizeof(MlpBufs));
+  }
+  const int binsz[3] = {ctx->bins.max_traj, ctx->bins.max_m, ctx->bins.max_rec};
+  put(binsz, sizeof(binsz));
+}
+
+static void drop_graph(pst_sim* s) {
+  if (s->g_exec) hipGraphExecDestroy(s->g_exec);
+  if (s->g_graph) hipGraphDestroy(s->g_graph);
+  s->g_exec = nullptr;
+  s->g_graph = nullptr;
+  s->g_key.clear();
+}
+
+// The step graph of gs steps: re-used while its key is unchanged, captured
+// again (on the sim's private capture stream) otherwise.
+static int step_graph(pst_sim* s, int gs) {
+  pst_ctx* ctx = s->ctx;
+  std::vector<unsigned char> key;
+  graph_key(s, gs, key);
+  if (s->g_exec && key == s->g_key) return 0;
+  drop_graph(s);
+  if (!s->g_cap) CK(hipStreamCreateWithFlags(&s->g_cap, hipStreamNonBlocking));
+  CK(hipStreamBeginCapture(s->g_cap, hipStreamCaptureModeThreadLocal));
+  hipStream_t saved = ctx->stream;
+  ctx->stream = s->g_cap;
+  int rc = 0;
+  for (int i = 0; i < gs && !rc; ++i) rc = enqueue_step(s, i & 1);
+  ctx->stream = saved;
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(s->g_cap, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    ctx->err = std::string("pst_prod_run: hipStreamEndCapture: ") + hipGetErrorString(ee);
+    return 2;
+  }
+  s->g_graph = g;
+  const hipError_t ei = hipGraphInstantiate(&s->g_exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    s->g_exec = nullptr;
+    drop_graph(s);
+    ctx->err = std::string("pst_prod_run: hipGraphInstantiate: ") + hipGetErrorString(ei);
+    return 2;
+  }
+  s->g_key.swap(key);
+  return 0;
+}
+
+namespace {
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+  }
+};
+}  // namespace
+
 int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
@@ -1402,37 +1582,22 @@ int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) 
   if (max_steps < 0 || graph_steps < 0) FAIL("pst_prod_run: bad arguments");
   CK(set_dev(ctx));
   if (sim_ensure_bins(s)) return 2;
-  hipEvent_t e0, e1;
-  CK(hipEventCreate(&e0));
-  CK(hipEventCreate(&e1));
+  EventPair ev;
+  CK(hipEventCreate(&ev.a));
+  CK(hipEventCreate(&ev.b));
   long long steps = 0;
   RunCounters h;
   memset(&h, 0, sizeof(h));
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  hipStream_t cap = nullptr;
   const int gs = graph_steps;
-  if (gs > 0) {
-    // capture on a dedicated stream (the legacy null stream cannot be captured)
-    CK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    CK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    hipStream_t saved = ctx->stream;
-    ctx->stream = cap;
-    int rc = 0;
-    for (int i = 0; i < gs && !rc; ++i) rc = enqueue_step(s, i & 1);
-    ctx->stream = saved;
-    hipGraph_t g = nullptr;
-    hipError_t ee = hipStreamEndCapture(cap, &g);
-    if (rc) { if (g) hipGraphDestroy(g); hipStreamDestroy(cap); return rc; }
-    if (ee != hipSuccess) { hipStreamDestroy(cap); ctx->err = hipGetErrorString(ee); return 2; }
-    graph = g;
-    CK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  if (gs > 0 && max_steps >= gs) {
+    const int rc = step_graph(s, gs);
+    if (rc) return rc;
   }
-  CK(hipEventRecord(e0, ctx->stream));
+  CK(hipEventRecord(ev.a, ctx->stream));
   bool stop = false;
   while (steps < max_steps && !stop) {
     if (gs > 0 && steps + gs <= max_steps) {
-      CK(hipGraphLaunch(exec, ctx->stream));
+      CK(hipGraphLaunch(s->g_exec, ctx->stream));
       steps += gs;
     } else {
       int rc = enqueue_step(s, (int)(steps & 1));
@@ -1443,16 +1608,11 @@ int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) 
     CK(hipStreamSynchronize(ctx->stream));
     if (h.stop || (h.n_sr + h.n_mlp) == 0) stop = true;
   }
-  CK(hipEventRecord(e1, ctx->stream));
-  CK(hipEventSynchronize(e1));
+  CK(hipEventRecord(ev.b, ctx->stream));
+  CK(hipEventSynchronize(ev.b));
   float ms = 0;
-  CK(hipEventElapsedTime(&ms, e0, e1));
+  CK(hipEventElapsedTime(&ms, ev.a, ev.b));
   s->last_ms_per_step = steps ? ms / steps : 0.0;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
-  if (cap) hipStreamDestroy(cap);
   if (c16) {
     double v[16] = {(double)h.n_reacted, (double)h.n_escaped, (double)h.n_maxsteps,
                     (double)h.total_steps, (double)h.next_traj, (double)h.n_returns,
@@ -1905,6 +2065,182 @@ int pst_prod_counters(pst_sim* s, double* c16) {
   return 0;
 }
 
+// ---- checkpoint / resume -------------------------------------------------------
+// Every device buffer whose contents carry from one production step to the
+// next, in a fixed order: slot state, run counters, per-trajectory outcomes
+// (record >= 1) and observables (record 2).  Noise is keyed by (trajectory,
+// own step), so restoring these restores the run exactly.
+namespace {
+struct Region {
+  void* dev;
+  size_t bytes;
+};
+constexpr long long kStateMagic = 0x3153544154535350ll;  // "PSSTATS1"
+constexpr int kStateHeader = 8;                           // int64 words
+}  // namespace
+
+static std::vector<Region> state_regions(pst_sim* s) {
+  const size_t S = s->S, N = (size_t)s->P.n_total;
+  std::vector<Region> r = {
+      {s->pos, S * 3 * 8},       {s->quat, S * 4 * 8},        {s->status, S * 4},
+      {s->n_steps32, S * 4},     {s->traj, S * 8},            {s->F, S * 3 * 8},
+      {s->T, S * 3 * 8},         {s->dt, S * 8},              {s->x0, S * PST_MAX_PAIRS * 8},
+      {s->x1, S * PST_MAX_PAIRS * 8}, {s->sm_state, S * 4},   {s->sm_mask, S * 4},
+      {s->lmz_escaped, S * 4},   {s->key, S * 4},             {s->bins, 512 * 4},
+      {s->ctr, sizeof(RunCounters)}};
+  if (s->out_status) {
+    r.push_back({s->out_status, N});
+    r.push_back({s->out_steps, N * 4});
+  }
+  if (s->obs.traj) {
+    const ObsBufs& O = s->obs;
+    r.push_back({O.s_min, S * 8});
+    r.push_back({O.s_step, S * 4});
+    r.push_back({O.s_time, S * 8});
+    r.push_back({O.s_ret, S * 4});
+    r.push_back({O.s_prev_r, S * 8});
+    r.push_back({O.s_nm, S * 7 * 8});
+    r.push_back({O.traj, N * kObsTraj * 8});
+    r.push_back({O.hist, (size_t)kObsHist * 8});
+  }
+  return r;
+}
+
+static long long state_bytes(pst_sim* s) {
+  long long b = kStateHeader * 8 + (long long)sizeof(StepParams);
+  for (const Region& r : state_regions(s)) b += (long long)r.bytes;
+  return b;
+}
+
+static int record_level(const pst_sim* s) { return s->obs.traj ? 2 : (s->out_status ? 1 : 0); }
+
+int pst_prod_state_bytes(pst_sim* s, long long* bytes) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_state_bytes: call pst_prod_init first");
+  if (!bytes) FAIL("pst_prod_state_bytes: null pointer");
+  *bytes = state_bytes(s);
+  return 0;
+}
+
+int pst_prod_save_state(pst_sim* s, void* host, long long bytes) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_save_state: call pst_prod_init first");
+  if (!host) FAIL("pst_prod_save_state: null pointer");
+  if (bytes != state_bytes(s)) FAIL("pst_prod_save_state: buffer size differs from pst_prod_state_bytes");
+  CK(set_dev(ctx));
+  CK(hipStreamSynchronize(ctx->stream));
+  long long* hdr = (long long*)host;
+  const long long h[kStateHeader] = {kStateMagic, 1, (long long)s->S, (long long)s->P.n_total,
+                                     record_level(s), (long long)sizeof(RunCounters),
+                                     (long long)sizeof(StepParams), bytes};
+  memcpy(hdr, h, sizeof(h));
+  unsigned char* p = (unsigned char*)host + kStateHeader * 8;
+  memcpy(p, &s->P, sizeof(StepParams));
+  p += sizeof(StepParams);
+  for (const Region& r : state_regions(s)) {
+    CK(hipMemcpy(p, r.dev, r.bytes, hipMemcpyDeviceToHost));
+    p += r.bytes;
+  }
+  return 0;
+}
+
+int pst_prod_load_state(pst_sim* s, const void* host, long long bytes) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_load_state: call pst_prod_init (same record level) first");
+  if (!host) FAIL("pst_prod_load_state: null pointer");
+  if (bytes < kStateHeader * 8 + (long long)sizeof(StepParams))
+    FAIL("pst_prod_load_state: buffer too small for a saved state");
+  long long h[kStateHeader];
+  memcpy(h, host, sizeof(h));
+  if (h[0] != kStateMagic || h[1] != 1) FAIL("pst_prod_load_state: not a saved production state");
+  if (h[2] != s->S || h[3] != s->P.n_total || h[4] != record_level(s))
+    FAIL("pst_prod_load_state: saved slots / trajectories / record level differ from this run");
+  if (h[5] != (long long)sizeof(RunCounters) || h[6] != (long long)sizeof(StepParams) ||
+      h[7] != bytes || bytes != state_bytes(s))
+    FAIL("pst_prod_load_state: saved layout differs from this library's");
+  const unsigned char* p = (const unsigned char*)host + kStateHeader * 8;
+  if (memcmp(p, &s->P, sizeof(StepParams)) != 0)
+    FAIL("pst_prod_load_state: saved step parameters differ from this run's");
+  p += sizeof(StepParams);
+  CK(set_dev(ctx));
+  CK(hipStreamSynchronize(ctx->stream));
+  for (const Region& r : state_regions(s)) {
+    CK(hipMemcpy(r.dev, p, r.bytes, hipMemcpyHostToDevice));
+    p += r.bytes;
+  }
+  return 0;
+}
+
+// Audit sums of the run counters (since the last pst_sim_reset_audit).
+int pst_prod_audit(pst_sim* s, double* out6) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->ctr) FAIL("pst_prod_audit: call pst_prod_init first");
+  if (!out6) FAIL("pst_prod_audit: null pointer");
+  CK(set_dev(ctx));
+  RunCounters h;
+  CK(hipMemcpyAsync(&h, s->ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  out6[0] = h.audit_err2; out6[1] = h.audit_ref2; out6[2] = (double)h.audit_n;
+  out6[3] = h.audit_terr2; out6[4] = h.audit_tref2; out6[5] = 0.0;
+  return 0;
+}
+
+// Path and energetics rows of the running trajectories that have moved
+// (gpu_batch_simulator.py:1318-1340): path8 (S,8) = trajectory, step,
+// centroid x y z, q0 q1 q2; energy6 (S,6) = trajectory, step, force x y z,
+// dt.  Rows in trajectory order.
+int pst_prod_snapshot(pst_sim* s, long long step, double* path8, double* energy6, int* n_rows) {
+  if (!s) return 1;
+  pst_ctx* ctx = s->ctx;
+  if (!s->prod_ready) FAIL("pst_prod_snapshot: call pst_prod_init first");
+  if (!n_rows) FAIL("pst_prod_snapshot: null pointer");
+  CK(set_dev(ctx));
+  CK(hipStreamSynchronize(ctx->stream));
+  const size_t S = s->S;
+  std::vector<double> pos(S * 3), q(S * 4), F(S * 3), dt(S);
+  std::vector<int> st(S);
+  std::vector<unsigned int> ns(S);
+  std::vector<long long> tr(S);
+  CK(hipMemcpy(st.data(), s->status, S * 4, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(ns.data(), s->n_steps32, S * 4, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(tr.data(), s->traj, S * 8, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(pos.data(), s->pos, S * 24, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(q.data(), s->quat, S * 32, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(F.data(), s->F, S * 24, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(dt.data(), s->dt, S * 8, hipMemcpyDeviceToHost));
+  // running trajectories that have moved (a slot refilled at the end of the
+  // step holds a new trajectory that has not), in trajectory order as the
+  // reference lists them (which slot a trajectory occupies depends on the
+  // order in which slots were refilled)
+  std::vector<size_t> order;
+  for (size_t i = 0; i < S; ++i)
+    if (st[i] == PST_RUNNING && ns[i] != 0) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tr[a] < tr[b]; });
+  int n = 0;
+  for (size_t i : order) {
+    const double t = (double)tr[i];
+    if (path8) {
+      double* r = path8 + 8 * (size_t)n;
+      r[0] = t; r[1] = (double)step;
+      r[2] = pos[3 * i]; r[3] = pos[3 * i + 1]; r[4] = pos[3 * i + 2];
+      r[5] = q[4 * i]; r[6] = q[4 * i + 1]; r[7] = q[4 * i + 2];
+    }
+    if (energy6) {
+      double* r = energy6 + 6 * (size_t)n;
+      r[0] = t; r[1] = (double)step;
+      r[2] = F[3 * i]; r[3] = F[3 * i + 1]; r[4] = F[3 * i + 2];
+      r[5] = dt[i];
+    }
+    ++n;
+  }
+  *n_rows = n;
+  return 0;
+}
+
 int pst_prod_last_ms_per_step(pst_sim* s, double* ms) {
   if (!s || !ms) return 1;
   *ms = s->last_ms_per_step;
@@ -2035,14 +2371,19 @@ int pst_sim_set_surrogate(pst_sim* s, pst_mlp* mlp, const double* p9) {
   sp.use_torque = p9[7] != 0.0; sp.audit_n = (int)p9[8];
   if (!(sp.r_on > 0) || !(sp.r0 > 0) || !(sp.lam > 0)) FAIL("pst_sim_set_surrogate: bad parameters");
   if (sp.audit_n < 0 || sp.audit_n > 65536) FAIL("pst_sim_set_surrogate: audit_n out of range");
-  if (sp.audit_n > s->audit_cap) {
+  if (sp.audit_n > s->audit_cap || !s->audit_count) {
     if (s->audit_F) hipFree(s->audit_F);
-    s->audit_F = nullptr;
+    if (s->audit_T) hipFree(s->audit_T);
+    if (s->audit_list) hipFree(s->audit_list);
+    s->audit_F = s->audit_T = nullptr;
+    s->audit_list = nullptr;
     s->audit_cap = 0;
-    if (sp.audit_n > 0) {
-      CK(hipMalloc(&s->audit_F, (size_t)sp.audit_n * 3 * 8));
-      s->audit_cap = sp.audit_n;
-    }
+    if (!s->audit_count) CK(hipMalloc(&s->audit_count, sizeof(int)));
+    const size_t n = sp.audit_n > 0 ? (size_t)sp.audit_n : 1;
+    CK(hipMalloc(&s->audit_F, n * 3 * 8));
+    CK(hipMalloc(&s->audit_T, n * 3 * 8));
+    CK(hipMalloc(&s->audit_list, n * 4));
+    s->audit_cap = (int)n;
   }
   s->sp = sp;
   s->mlp = mlp;
@@ -2057,6 +2398,7 @@ int pst_sim_reset_audit(pst_sim* s) {
   CK(set_dev(ctx));
   CK(hipMemsetAsync(&s->ctr->audit_n, 0, sizeof(unsigned long long), ctx->stream));
   CK(hipMemsetAsync(&s->ctr->audit_err2, 0, 2 * sizeof(double), ctx->stream));
+  CK(hipMemsetAsync(&s->ctr->audit_terr2, 0, 2 * sizeof(double), ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
   return 0;
 }

@@ -9,8 +9,9 @@
 //   env(r) = exp(-(r - r0)/lam) (r0/r)^2 (1 + r/lam) / (1 + r0/lam),
 // the shape of a screened-Coulomb force normalised to 1 at r0, so the
 // network learns O(1) targets over the whole shell.  An audit evaluates the
-// plain grid path on the first `audit_n` surrogate trajectories of every step
-// and accumulates the squared force error and the squared reference force.
+// plain grid path on `audit_n` surrogate trajectories of every step (a stride
+// through the list from a moving offset) and accumulates the squared force
+// and torque errors and the squared reference force and torque.
 #include <hip/hip_runtime.h>
 #include "pst_device.hpp"
 #include "pst_kernels.hpp"
@@ -57,11 +58,34 @@ __global__ void k_surr_apply(SurrParams S, const double* pos, const int* list, c
   }
 }
 
+// Audit sample of one step: k = min(n, cap) rows of the surrogate list at a
+// stride of n / k from an offset that a hash of the run's trajectory-step
+// counter moves every step (stable within the step: only the pre and post
+// phases change it), so the audit sees every part of the list over a run.
+__global__ void k_surr_audit_pick(const int* list, const int* n_dev, int cap,
+                                  const RunCounters* C, int* out, int* n_out) {
+  const int n = *n_dev;
+  const int k = min(n, cap);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *n_out = k;
+  if (i >= k) return;
+  unsigned long long h = (C->total_steps + 1) * 0x9E3779B97F4A7C15ull;
+  h ^= h >> 31;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 27;
+  const long long off = (long long)(h % (unsigned long long)n);
+  const long long stride = n / k;
+  out[i] = list[(off + (long long)i * stride) % n];
+}
+
+// Squared errors of the surrogate's forces (and, with Tref, torques) against
+// the grid path on the audited rows, summed into the run counters.
 __global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const double* F,
-                             const double* Fref, RunCounters* C) {
+                             const double* T, const double* Fref, const double* Tref,
+                             RunCounters* C) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = min(*n_dev, cap);
-  double e2 = 0, r2 = 0;
+  double e2 = 0, r2 = 0, te2 = 0, tr2 = 0;
   if (w < n) {
     const int s = list[w];
     for (int d = 0; d < 3; ++d) {
@@ -69,15 +93,28 @@ __global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const d
       e2 += (a - b) * (a - b);
       r2 += b * b;
     }
+    if (Tref) {
+      for (int d = 0; d < 3; ++d) {
+        const double a = T[3 * s + d], b = Tref[3 * w + d];
+        te2 += (a - b) * (a - b);
+        tr2 += b * b;
+      }
+    }
   }
   // wave reduction, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     e2 += __shfl_xor(e2, off);
     r2 += __shfl_xor(r2, off);
+    te2 += __shfl_xor(te2, off);
+    tr2 += __shfl_xor(tr2, off);
   }
   if (__lane_id() == 0 && (e2 > 0 || r2 > 0)) {
     atomicAdd(&C->audit_err2, e2);
     atomicAdd(&C->audit_ref2, r2);
+  }
+  if (__lane_id() == 0 && (te2 > 0 || tr2 > 0)) {
+    atomicAdd(&C->audit_terr2, te2);
+    atomicAdd(&C->audit_tref2, tr2);
   }
   if (w == 0) atomicAdd(&C->audit_n, (unsigned long long)n);
 }
@@ -97,10 +134,17 @@ hipError_t surr_step(const SurrParams& S, const MlpBufs& m, const double* pos, c
   return hipGetLastError();
 }
 
-hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* F,
-                      const double* Fref, RunCounters* C, hipStream_t st) {
+hipError_t surr_audit_pick(const int* list, const int* n_dev, int cap, const RunCounters* C,
+                           int* out, int* n_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_surr_audit_pick, dim3((cap + 255) / 256), dim3(256), 0, st, list, n_dev,
+                     cap, C, out, n_out);
+  return hipGetLastError();
+}
+
+hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* F, const double* T,
+                      const double* Fref, const double* Tref, RunCounters* C, hipStream_t st) {
   hipLaunchKernelGGL(k_surr_audit, dim3((cap + 255) / 256), dim3(256), 0, st, list, n_dev, cap, F,
-                     Fref, C);
+                     T, Fref, Tref, C);
   return hipGetLastError();
 }
 

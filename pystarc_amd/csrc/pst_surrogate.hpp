@@ -21,7 +21,9 @@ struct SurrParams {
 hipError_t surr_step(const SurrParams& S, const MlpBufs& m, const double* pos, const double* quat,
                      const int* list, const int* n_dev, int n_slots, double* F, double* T,
                      hipStream_t st);
-hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* F,
-                      const double* Fref, RunCounters* C, hipStream_t st);
+hipError_t surr_audit_pick(const int* list, const int* n_dev, int cap, const RunCounters* C,
+                           int* out, int* n_out, hipStream_t st);
+hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* F, const double* T,
+                      const double* Fref, const double* Tref, RunCounters* C, hipStream_t st);
 
 }  // namespace pst

@@ -172,4 +172,8 @@ struct RunCounters {
   unsigned long long sm_fired[PST_MAX_PAIRS];
   unsigned long long sm_fired_bridge[PST_MAX_PAIRS];
   unsigned long long sm_unique[PST_MAX_PAIRS];
+  // surrogate audit, torques (physical torque mode): squared error and
+  // squared grid-path torque summed over the audited trajectories
+  double audit_terr2;
+  double audit_tref2;
 };

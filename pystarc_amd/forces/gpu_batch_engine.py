@@ -230,3 +230,36 @@ class GPUBatchForceEngine:
                                                         ptr(T), ptr(E), self._prec(precision)),
                            "pst_forces_pose")
         return F, T, E
+
+
+def forces_pose_group(engines, centroids, quats, *, precision=None):
+    """Forces of several ligand engines (receptor grids shared through
+    ``receptor_from``) in one launch of the grouped force kernel, as the
+    several-ligand step evaluates them (pst_forces_pose_group).  centroids /
+    quats: one (n_l,3) / (n_l,4) array per engine; returns one (F, T, E)
+    tuple of device tensors per engine."""
+    if not engines or len(engines) != len(centroids) or len(engines) != len(quats):
+        raise ValueError("one centroid and one quaternion array per engine")
+    torch = engines[0]._torch
+    prec = engines[0]._prec(precision)
+    L = len(engines)
+    keep, outs = [], []
+    for e, c, q in zip(engines, centroids, quats):
+        if e._lig_key is None:
+            raise RuntimeError("call set_ligand(body, charges) on every engine first")
+        ct, qt = e._dev(c), e._dev(q)
+        n = int(ct.shape[0])
+        F = torch.zeros((n, 3), dtype=torch.float64, device=e.device)
+        T = torch.zeros((n, 3), dtype=torch.float64, device=e.device)
+        E = torch.zeros((n,), dtype=torch.float64, device=e.device)
+        keep += [ct, qt]
+        outs.append((F, T, E))
+    P = C.c_void_p * L
+    ctxs = P(*[e.ctx.h.value for e in engines])
+    ns = (C.c_int * L)(*[int(o[0].shape[0]) for o in outs])
+    pos = P(*[keep[2 * l].data_ptr() for l in range(L)])
+    qua = P(*[keep[2 * l + 1].data_ptr() for l in range(L)])
+    Fp, Tp, Ep = (P(*[o[k].data_ptr() for o in outs]) for k in range(3))
+    engines[0].ctx.check(engines[0].ctx.lib.pst_forces_pose_group(ctxs, L, ns, pos, qua, Fp, Tp, Ep,
+                                                                   prec), "pst_forces_pose_group")
+    return outs

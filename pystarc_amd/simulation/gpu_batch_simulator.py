@@ -54,13 +54,31 @@ Keyword-only extensions (defaults keep the reference's physics):
                q-sphere at 20 x the larger molecular radius) by the LMZ outer
                propagator of the reference's CPU simulator on the device,
                returning to the b-sphere or escaping (philox mode).
+  snapshots    path / energetics rows of the running trajectories every
+               _output_cfg.save_interval steps (sim_data "path_steps",
+               "energy_steps", gpu_batch_simulator.py:1318-1340); default: on
+               with record="observables", as the reference collects them
+               with its sim_data.
+  stall_interval  steps between the reference's stall checks (10,000).
+
+Run control of philox mode, as the reference's loop (:1342-1420): the 99.5 %
+early stop (on the device), the stall rule, live k_on at every
+convergence_interval percent (sim_data "convergence", logged), a checkpoint
+of the whole device state in _work_dir/checkpoint.npz every
+checkpoint_interval completed trajectories, resumed from when the file
+exists and removed when the run completes.  Each trajectory gets max_steps
+of its own steps however many slots refill it.
 """
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
+import logging
 import math
+import os
 import time
 from dataclasses import dataclass, field
+from pathlib import Path
 from typing import Dict
 
 import numpy as np
@@ -68,6 +86,15 @@ import numpy as np
 from .._lib import MAX_PAIRS, StepParams, ptr
 from ..constants import CONV, EPS0, MU_REF, OVERLAP_THRESHOLD, QB_FACTOR
 from ..lib.numerical import romberg_integrate
+
+log = logging.getLogger("pystarc_amd")
+
+# the reference checks for stalled runs every 10,000 loop steps
+# (gpu_batch_simulator.py:1349-1356)
+STALL_EVERY = 10_000
+# stated relative RMS force error of the surrogate: above it (audit per chunk
+# of steps) the grid path takes over every trajectory
+AUDIT_TOL = 0.25
 
 
 def _z_quantile(p):
@@ -273,11 +300,178 @@ def _shoemake_host(u):
                      np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], axis=1)
 
 
+def stall_stop(n_total, done, done_before):
+    """The reference's stall rule (gpu_batch_simulator.py:1349-1356), checked
+    every STALL_EVERY steps: stop when more than 95 % of the trajectories are
+    done and fewer than max(2, 0.1 % of them) finished since the last check."""
+    return done > int(n_total * 0.95) and done - done_before < max(2, int(n_total * 0.001))
+
+
+def is_stall_check(step, every=STALL_EVERY):
+    """The reference checks after loop index i > 0 with i % 10000 == 0, i.e.
+    after i + 1 moves; `step` here counts moves."""
+    return step > 1 and (step - 1) % every == 0
+
+
+def next_stall_check(step, every=STALL_EVERY):
+    """The first move count after `step` at which is_stall_check holds."""
+    return (max(step - 1, 0) // every + 1) * every + 1
+
+
+def next_snapshot(step, every):
+    """The first move count after `step` at which the reference records a
+    snapshot (loop index i with i % save_interval == 0, i.e. after i + 1 moves)."""
+    return (max(step - 1, -1) // every + 1) * every + 1
+
+
+def convergence_milestones(n_total, interval):
+    """Done counts at which the reference reports live k_on
+    (gpu_batch_simulator.py:753-761): every `interval` percent below 100."""
+    interval = int(interval or 0)
+    if interval <= 0:
+        return []
+    return sorted({int(n_total * q / 100) for q in range(interval, 100, interval)})
+
+
+def device_step_bound(max_steps, n_total, n_slots):
+    """Device steps that suffice for every trajectory to get its own
+    max_steps: while the pool still holds trajectories every slot is busy, so
+    all of them start within ceil(N / S) * max_steps steps, and the last one
+    ends max_steps later.  (The reference runs every trajectory from step 0,
+    so its loop bound max_steps is also each trajectory's.)"""
+    waves = -(-int(n_total) // max(1, int(n_slots)))
+    return int(max_steps) * (waves + 1)
+
+
+def snapshot_config(params):
+    """(save_interval, paths, energetics) of the reference's _output_cfg
+    (gpu_batch_simulator.py:578-581, :1318-1340): every 10 steps, both on,
+    when no output configuration is attached."""
+    cfg = getattr(params, "_output_cfg", None)
+    if cfg is None:
+        return 10, True, True
+    return (max(int(getattr(cfg, "save_interval", 10)), 1), bool(getattr(cfg, "full_paths", True)),
+            bool(getattr(cfg, "energetics", True)))
+
+
+_TUNE_CACHE = {}
+
+
+def _tune_cache_file():
+    return os.environ.get("PYSTARC_AMD_TUNE_CACHE")
+
+
+def tune_cache_get(key):
+    """Launch chosen earlier for this key (this process, or the JSON file
+    named by PYSTARC_AMD_TUNE_CACHE)."""
+    if key in _TUNE_CACHE:
+        return _TUNE_CACHE[key]
+    f = _tune_cache_file()
+    if f and os.path.exists(f):
+        import json
+        try:
+            with open(f) as fh:
+                d = json.load(fh)
+            v = d.get(key)
+            if v is not None:
+                _TUNE_CACHE[key] = v
+            return v
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def tune_cache_put(key, best):
+    _TUNE_CACHE[key] = best
+    f = _tune_cache_file()
+    if f:
+        import json
+        d = {}
+        try:
+            if os.path.exists(f):
+                with open(f) as fh:
+                    d = json.load(fh)
+        except (OSError, ValueError):
+            d = {}
+        d[key] = best
+        tmp = f + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(d, fh)
+        os.replace(tmp, f)
+
+
+_CONV_KEYS = ("milestone", "done", "pct", "P_rxn", "k_on", "n_reacted", "n_escaped", "step",
+              "elapsed_s")
+
+
+def _split_rows(rows, counts):
+    out, o = [], 0
+    for n in np.asarray(counts).tolist():
+        out.append(np.asarray(rows[o:o + n]))
+        o += n
+    return out
+
+
+class Checkpoint:
+    """checkpoint.npz of a production run in params._work_dir (default
+    "bd_sims"), with the reference's file contract
+    (gpu_batch_simulator.py:548-575 resume when the file exists, :764-768 and
+    :1400-1420 written whenever checkpoint_interval more trajectories have
+    completed, :1505-1510 removed when the run completes).  It holds the
+    library's saved device state (pst_prod_save_state: slots, counters,
+    outcomes, observables), the host loop's counters and the sim_data lists
+    collected so far, all as plain arrays (loaded with allow_pickle=False),
+    and a fingerprint of the run's parameters; a file of another run (or of
+    the reference) is ignored, as the reference ignores one it cannot load."""
+
+    def __init__(self, work_dir, fingerprint):
+        self.dir = Path(work_dir)
+        self.path = self.dir / "checkpoint.npz"
+        self.fingerprint = fingerprint
+
+    def exists(self):
+        return self.path.exists()
+
+    def save(self, lib, ctx, h, host):
+        nb = C.c_longlong()
+        ctx.check(lib.pst_prod_state_bytes(h, C.byref(nb)), "pst_prod_state_bytes")
+        blob = np.zeros(nb.value, dtype=np.uint8)
+        ctx.check(lib.pst_prod_save_state(h, ptr(blob), nb.value), "pst_prod_save_state")
+        self.dir.mkdir(parents=True, exist_ok=True)
+        arrays = {"state": blob, "fingerprint": np.array(self.fingerprint)}
+        for k, v in host.items():
+            arrays[k] = np.asarray(v)
+        tmp = self.dir / "checkpoint.tmp.npz"
+        np.savez_compressed(str(tmp), **arrays)
+        os.replace(tmp, self.path)
+
+    def load(self, lib, ctx, h):
+        """Restores the device state; returns the host entries, or None when
+        the file is not this run's (it is then left alone)."""
+        try:
+            with np.load(str(self.path), allow_pickle=False) as z:
+                if str(z["fingerprint"]) != self.fingerprint:
+                    log.warning("checkpoint %s belongs to another run; starting fresh", self.path)
+                    return None
+                host = {k: z[k] for k in z.files if k not in ("state", "fingerprint")}
+                blob = np.ascontiguousarray(z["state"])
+        except (OSError, ValueError, KeyError) as e:
+            log.warning("checkpoint load failed: %s. Starting fresh.", e)
+            return None
+        ctx.check(lib.pst_prod_load_state(h, ptr(blob), blob.size), "pst_prod_load_state")
+        return host
+
+    def remove(self):
+        if self.path.exists():
+            self.path.unlink()
+
+
 class GPUBatchSimulator:
     def __init__(self, mol1, mol2, mob, pathway_set, params, batch_engine, *, rng="philox",
                  n_slots=None, precision=None, graph_steps=16, record=False, surrogate=None,
-                 audit_n=64, audit_tol=0.05, hydro="scalar", autotune=False, gather=None,
-                 traj_range=None, outer="return_prob"):
+                 audit_n=64, audit_tol=AUDIT_TOL, hydro="scalar", autotune=False, gather=None,
+                 traj_range=None, outer="return_prob", snapshots=None,
+                 stall_interval=STALL_EVERY):
         if rng not in ("philox", "numpy"):
             raise ValueError("rng must be 'philox' or 'numpy'")
         if traj_range is not None:
@@ -317,6 +511,14 @@ class GPUBatchSimulator:
         if record not in (False, True, "observables"):
             raise ValueError("record must be False, True or 'observables'")
         self.record = 2 if record == "observables" else int(bool(record))
+        if snapshots not in (None, True, False):
+            raise ValueError("snapshots must be None, True or False")
+        if snapshots and rng != "philox":
+            raise ValueError("snapshots need rng='philox'")
+        self.snapshots = (self.record == 2) if snapshots is None else bool(snapshots)
+        self.stall_interval = int(stall_interval)
+        if self.stall_interval < 1:
+            raise ValueError("stall_interval must be positive")
         prec = precision if precision is not None else getattr(batch_engine, "precision", "fp64")
         if prec not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
@@ -500,12 +702,15 @@ class GPUBatchSimulator:
         return h
 
     # -- run -----------------------------------------------------------------------
-    def run(self) -> GPUBatchResult:
+    def run(self, max_device_steps=None) -> GPUBatchResult:
+        """Run to completion (the reference's stop rules).  max_device_steps
+        (philox mode) ends the run earlier after that many device steps, as an
+        interrupted run: its checkpoint, if one is being written, is kept."""
         self._return_prob = self._compute_return_prob()
         P = self._step_params()
         if self.rng_mode == "numpy":
             return self._run_replay(P)
-        return self._run_philox(P)
+        return self._run_philox(P, max_device_steps)
 
     def _result(self, N, n_r, n_e, n_m, total, elapsed, sim_data):
         return GPUBatchResult(
@@ -515,10 +720,32 @@ class GPUBatchSimulator:
             elapsed_sec=elapsed, steps_per_sec=total / elapsed if elapsed > 0 else 0.0,
             sim_data=sim_data, total_steps=int(total), k_b=self._k_b, return_prob=self._return_prob)
 
-    def _run_philox(self, P) -> GPUBatchResult:
+    def tune_key(self):
+        """Autotuner cache key: ligand and receptor atoms, grid dimensions,
+        precision, slots, hydrodynamics, torque mode and surrogate on/off."""
+        eng = self.engine
+        dims = [(g["nx"], g["ny"], g["nz"]) for g in getattr(eng, "_elec_grids_gpu", [])
+                + getattr(eng, "_born_grids_gpu", []) + getattr(eng, "_lig_born_grids_gpu", [])]
+        n_rec = 0 if getattr(eng, "_rec_pos", None) is None else len(eng._rec_pos)
+        return (f"M{self._N_lig}-R{n_rec}-G{dims}-{self.precision}-S{self.n_slots}-{self.hydro}-"
+                f"{getattr(eng, 'torque_mode', 'reference')}-surr{int(self.surrogate is not None)}")
+
+    def fingerprint(self, P):
+        """Identity of a run for its checkpoint: the step parameters (seed,
+        trajectory count, physics, pairs) and the options that change results."""
+        hsh = hashlib.sha256(bytes(P))
+        extra = (self.precision, self.record, self.n_slots, self.hydro, self.outer,
+                 None if self.surrogate is None else
+                 tuple(np.round(self.surrogate.sim_params(self.audit_n), 12).tolist()))
+        hsh.update(repr(extra).encode())
+        return hsh.hexdigest()
+
+    def _run_philox(self, P, max_device_steps=None) -> GPUBatchResult:
         eng = self.engine
         lib, ctx = eng.ctx.lib, eng.ctx
+        p = self.params
         h = self._make_sim(P)
+        N = int(P.n_total)
         try:
             ctx.check(lib.pst_prod_init(h, int(self.record)), "pst_prod_init")
             surr_on = False
@@ -527,46 +754,153 @@ class GPUBatchSimulator:
                 ctx.check(lib.pst_sim_set_surrogate(h, self.surrogate.h, ptr(p9)),
                           "pst_sim_set_surrogate")
                 surr_on = True
-            audit = {"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None}
+            audit = {"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None,
+                     "terr2": 0.0, "tref2": 0.0, "worst_torque": 0.0}
             c16 = np.zeros(16)
+            steps = stall_done = last_ckpt = 0
+            conv_done = 0
+            paths, energies, conv = [], [], []
+            save_every, want_paths, want_energy = snapshot_config(p)
+            snap = self.snapshots and (want_paths or want_energy)
+            ckpt_every = int(getattr(p, "checkpoint_interval", 0) or 0)
+            ckpt = Checkpoint(getattr(p, "_work_dir", "bd_sims"), self.fingerprint(P))
+            resumed = False
+            if ckpt.exists():
+                host = ckpt.load(lib, ctx, h)
+                if host is not None:
+                    resumed = True
+                    steps, stall_done, last_ckpt, conv_done = (int(x) for x in host["loop"])
+                    a = host["audit"]
+                    audit.update(err2=float(a[0]), ref2=float(a[1]), n=int(a[2]),
+                                 worst=float(a[3]), terr2=float(a[4]), tref2=float(a[5]),
+                                 worst_torque=float(a[6]),
+                                 fallback_step=None if a[7] < 0 else int(a[7]))
+                    if surr_on and audit["fallback_step"] is not None:
+                        ctx.check(lib.pst_sim_set_surrogate(h, None, None), "pst_sim_set_surrogate")
+                        surr_on = False
+                    if "paths" in host:
+                        paths = _split_rows(host["paths"], host["paths_n"])
+                    if "energies" in host:
+                        energies = _split_rows(host["energies"], host["energies_n"])
+                    if "conv" in host:
+                        conv = [dict(zip(_CONV_KEYS, r.tolist())) for r in host["conv"]]
+                    ctx.check(lib.pst_prod_counters(h, ptr(c16)), "pst_prod_counters")
+                    log.info("resumed from checkpoint: step=%d, %d/%d done", steps,
+                             int(c16[0] + c16[1] + c16[2]), N)
+            milestones = convergence_milestones(N, getattr(p, "convergence_interval", 10))
             t0 = time.time()
-            steps = 0
+            last_report = t0
             tuned = None
-            if self.autotune:
-                ctx.check(lib.pst_prod_run(h, self.autotune, self.graph_steps, ptr(c16)),
-                          "pst_prod_run")
-                steps += int(c16[8])
-                tuned = autotune_sim(lib, ctx, h)
-                steps += tuned["steps"]
+            if self.autotune and not resumed:
+                key = self.tune_key()
+                best = tune_cache_get(key)
+                if best is not None:
+                    ctx.check(lib.pst_prod_set_launch(h, {"direct": 0, "binned": 1, "zpair": 2}[
+                        best["mode"]], int(best["G"]), int(best["cap"]), int(best["xcd"])),
+                        "pst_prod_set_launch")
+                    tuned = {"best": best, "table": [], "steps": 0, "cached": True}
+                else:
+                    ctx.check(lib.pst_prod_run(h, self.autotune, self.graph_steps, ptr(c16)),
+                              "pst_prod_run")
+                    steps += int(c16[8])
+                    tuned = autotune_sim(lib, ctx, h)
+                    steps += tuned["steps"]
+                    tune_cache_put(key, tuned["best"])
+            limit = device_step_bound(P.max_steps, N, self.n_slots)
+            if max_device_steps is not None:
+                limit = min(limit, int(max_device_steps))
             chunk = max(self.graph_steps, 1) * 64
-            while steps < P.max_steps:
-                todo = int(min(chunk, P.max_steps - steps))
+            stop_reason = "step_limit"
+            while steps < limit:
+                todo = min(chunk, limit - steps)
+                # end chunks where the reference's loop checks for stalls and
+                # records snapshots
+                todo = min(todo, next_stall_check(steps, self.stall_interval) - steps)
+                if snap:
+                    todo = min(todo, next_snapshot(steps, save_every) - steps)
+                todo = max(int(todo), 1)
                 ctx.check(lib.pst_prod_run(h, todo, self.graph_steps, ptr(c16)), "pst_prod_run")
-                steps += int(c16[8])
+                ran = int(c16[8])
+                steps += ran
                 if surr_on and c16[13] > 0:
-                    err = math.sqrt(c16[12] / c16[13])
-                    audit["err2"] += c16[12]
-                    audit["ref2"] += c16[13]
-                    audit["n"] += int(c16[14])
+                    a6 = np.zeros(6)
+                    ctx.check(lib.pst_prod_audit(h, ptr(a6)), "pst_prod_audit")
+                    err = math.sqrt(a6[0] / a6[1])
+                    terr = math.sqrt(a6[3] / a6[4]) if a6[4] > 0 else 0.0
+                    audit["err2"] += a6[0]
+                    audit["ref2"] += a6[1]
+                    audit["n"] += int(a6[2])
+                    audit["terr2"] += a6[3]
+                    audit["tref2"] += a6[4]
                     audit["worst"] = max(audit["worst"], err)
+                    audit["worst_torque"] = max(audit["worst_torque"], terr)
                     ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
-                    if err > self.audit_tol:
+                    if err > self.audit_tol or terr > self.audit_tol:
                         # the grid path takes over every trajectory from here on
                         ctx.check(lib.pst_sim_set_surrogate(h, None, None), "pst_sim_set_surrogate")
                         surr_on = False
                         audit["fallback_step"] = steps
-                if c16[9] or (c16[10] + c16[11]) == 0 or int(c16[8]) < todo:
+                n_r, n_e, n_m = int(c16[0]), int(c16[1]), int(c16[2])
+                done = n_r + n_e + n_m
+                if snap and ran and (steps - 1) % save_every == 0:
+                    self._snapshot(lib, ctx, h, steps - 1, want_paths, want_energy, paths, energies)
+                for m in milestones:
+                    if conv_done < m <= done:
+                        conv_done = m
+                        conv.append(self._conv_report(m, n_r, n_e, done, N, steps,
+                                                      time.time() - t0))
+                now = time.time()
+                if now - last_report >= 10.0:
+                    last_report = now
+                    log.info("[T] %6.0fs done=%d/%d reacted=%d escaped=%d", now - t0, done, N,
+                             n_r, n_e)
+                if ckpt_every > 0 and done - last_ckpt >= ckpt_every:
+                    last_ckpt = done
+                    ckpt.save(lib, ctx, h, self._ckpt_host(steps, stall_done, last_ckpt, conv_done,
+                                                           audit, paths, energies, conv))
+                    log.info("checkpoint saved (%d/%d done) -> %s", done, N, ckpt.path)
+                if c16[9]:
+                    stop_reason = "done_99.5"
                     break
+                if (c16[10] + c16[11]) == 0:
+                    stop_reason = "complete"
+                    break
+                if ran < todo:
+                    stop_reason = "complete"
+                    break
+                if is_stall_check(steps, self.stall_interval):
+                    if stall_stop(N, done, stall_done):
+                        log.info("stall detected: %d trajectories trapped (+%d in the last %d "
+                                 "steps); stopping", N - done, done - stall_done,
+                                 self.stall_interval)
+                        stop_reason = "stall"
+                        break
+                    stall_done = done
             elapsed = time.time() - t0
-            N = int(P.n_total)
+            interrupted = (max_device_steps is not None and stop_reason == "step_limit"
+                           and steps >= int(max_device_steps))
+            if ckpt_every > 0 and interrupted:
+                ctx.check(lib.pst_prod_counters(h, ptr(c16)), "pst_prod_counters")
+                ckpt.save(lib, ctx, h, self._ckpt_host(steps, stall_done, last_ckpt, conv_done,
+                                                       audit, paths, energies, conv))
+            elif not interrupted and ckpt_every > 0:
+                ckpt.remove()
             n_r, n_e = int(c16[0]), int(c16[1])
             n_m = N - n_r - n_e
             sim_data = {"n_returns_total": int(c16[5]), "n_overlap_rejected": int(c16[6]),
-                        "n_bridge": int(c16[7]), "bd_steps": steps}
+                        "n_bridge": int(c16[7]), "bd_steps": steps, "stop_reason": stop_reason,
+                        "resumed": resumed}
+            if milestones:
+                sim_data["convergence"] = conv
+            if snap:
+                sim_data["path_steps"] = paths if want_paths else []
+                sim_data["energy_steps"] = energies if want_energy else []
             if tuned is not None:
                 sim_data["autotune"] = tuned
             if self.surrogate is not None:
                 audit["rel_rms"] = math.sqrt(audit["err2"] / audit["ref2"]) if audit["ref2"] > 0 else 0.0
+                audit["torque_rel_rms"] = (math.sqrt(audit["terr2"] / audit["tref2"])
+                                           if audit["tref2"] > 0 else None)
                 sim_data["surrogate_audit"] = audit
             if self.record:
                 st = np.zeros(N, dtype=np.int8)
@@ -581,6 +915,47 @@ class GPUBatchSimulator:
             return self._result(N, n_r, n_e, n_m, c16[3], elapsed, sim_data)
         finally:
             lib.pst_sim_destroy(h)
+
+    def _snapshot(self, lib, ctx, h, step, want_paths, want_energy, paths, energies):
+        S = self.n_slots
+        p8 = np.zeros((S, 8)) if want_paths else None
+        e6 = np.zeros((S, 6)) if want_energy else None
+        n = C.c_int()
+        ctx.check(lib.pst_prod_snapshot(h, int(step), ptr(p8), ptr(e6), C.byref(n)),
+                  "pst_prod_snapshot")
+        if want_paths:
+            paths.append(p8[:n.value].copy())
+        if want_energy:
+            energies.append(e6[:n.value].copy())
+
+    def _conv_report(self, milestone, n_r, n_e, done, N, steps, elapsed):
+        """Live k_on when `milestone` trajectories are done
+        (gpu_batch_simulator.py:1358-1372); the counts are those of the end of
+        the chunk of steps in which the milestone was passed."""
+        n_c = n_r + n_e
+        P_live = n_r / n_c if n_c else 0.0
+        k_live = CONV * self._k_b * P_live if self._k_b > 0 else 0.0
+        pct = milestone / N * 100
+        log.info("  -- %.0f%% converged --  P_rxn=%.5f  k_on=%.3e  (%d react / %d esc)  [%.0fs]",
+                 pct, P_live, k_live, n_r, n_e, elapsed)
+        return dict(zip(_CONV_KEYS, [float(milestone), float(done), pct, P_live, k_live,
+                                     float(n_r), float(n_e), float(steps), elapsed]))
+
+    def _ckpt_host(self, steps, stall_done, last_ckpt, conv_done, audit, paths, energies, conv):
+        a = [audit["err2"], audit["ref2"], audit["n"], audit["worst"], audit["terr2"],
+             audit["tref2"], audit["worst_torque"],
+             -1 if audit["fallback_step"] is None else audit["fallback_step"]]
+        host = {"loop": np.array([steps, stall_done, last_ckpt, conv_done], dtype=np.int64),
+                "audit": np.array(a, dtype=np.float64)}
+        if paths:
+            host["paths"] = np.concatenate(paths)
+            host["paths_n"] = np.array([len(x) for x in paths], dtype=np.int64)
+        if energies:
+            host["energies"] = np.concatenate(energies)
+            host["energies_n"] = np.array([len(x) for x in energies], dtype=np.int64)
+        if conv:
+            host["conv"] = np.array([[r[k] for k in _CONV_KEYS] for r in conv], dtype=np.float64)
+        return host
 
     def replay_driver(self) -> "ReplayDriver":
         """Step-by-step replay handle (run() drives it to completion)."""
@@ -799,7 +1174,7 @@ class GPUBatchGroup:
     grid path takes over that ligand's trajectories from then on (the other
     ligands keep theirs), as in a single run."""
 
-    def __init__(self, simulators, graph_steps=16):
+    def __init__(self, simulators, graph_steps=16, stall_interval=STALL_EVERY):
         if not simulators:
             raise ValueError("no simulators")
         for s in simulators:
@@ -809,6 +1184,7 @@ class GPUBatchGroup:
             raise ValueError("all simulators of a group need the same precision")
         self.sims = list(simulators)
         self.graph_steps = int(graph_steps)
+        self.stall_interval = int(stall_interval)
         self.last_ms_per_step = 0.0
 
     def _open(self, handles, Ps, record=None):
@@ -873,21 +1249,32 @@ class GPUBatchGroup:
         try:
             self._open(handles, Ps)
             L = len(sims)
-            arr = (C.c_void_p * L)(*[h.value for h in handles])
             c16 = np.zeros((L, 16))
             ms = C.c_double()
-            max_steps = max(int(P.max_steps) for P in Ps)
+            limit = max(device_step_bound(P.max_steps, P.n_total, s.n_slots)
+                        for s, P in zip(sims, Ps))
             surr_on = [s.surrogate is not None for s in sims]
             audits = [{"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None}
                       for _ in sims]
+            stall_done = [0] * L
+            stalled = [False] * L
             t0 = time.time()
             steps = 0
-            while steps < max_steps:
-                todo = int(min(max(self.graph_steps, 1) * 64, max_steps - steps))
-                ctx.check(lib.pst_group_run(arr, L, todo, self.graph_steps, ptr(c16), C.byref(ms)),
+            active = list(range(L))
+            while steps < limit and active:
+                todo = int(min(max(self.graph_steps, 1) * 64, limit - steps))
+                todo = max(1, min(todo, next_stall_check(steps, self.stall_interval) - steps))
+                La = len(active)
+                arr = (C.c_void_p * La)(*[handles[l].value for l in active])
+                ca = np.zeros((La, 16))
+                ctx.check(lib.pst_group_run(arr, La, todo, self.graph_steps, ptr(ca), C.byref(ms)),
                           "pst_group_run")
-                steps += int(c16[0, 8])
-                for l, s in enumerate(sims):
+                for k, l in enumerate(active):
+                    c16[l] = ca[k]
+                ran = int(ca[0, 8])
+                steps += ran
+                for l in active:
+                    s = sims[l]
                     if not surr_on[l] or c16[l, 13] <= 0:
                         continue
                     a, err = audits[l], math.sqrt(c16[l, 12] / c16[l, 13])
@@ -903,8 +1290,21 @@ class GPUBatchGroup:
                         a["fallback_step"] = steps
                 if progress is not None:
                     progress(steps, c16)
-                done = all(c16[l, 9] or (c16[l, 10] + c16[l, 11]) == 0 for l in range(L))
-                if done or int(c16[0, 8]) < todo:
+                if is_stall_check(steps, self.stall_interval):
+                    # the reference's stall rule, per ligand: a stalled ligand
+                    # leaves the group (its remaining trajectories end as
+                    # max-steps) while the others run on
+                    for l in list(active):
+                        P = Ps[l]
+                        d = int(c16[l, 0] + c16[l, 1] + c16[l, 2])
+                        if stall_stop(int(P.n_total), d, stall_done[l]):
+                            stalled[l] = True
+                            active.remove(l)
+                            log.info("ligand %d: stall detected, %d trajectories trapped", l,
+                                     int(P.n_total) - d)
+                        stall_done[l] = d
+                active = [l for l in active if not (c16[l, 9] or (c16[l, 10] + c16[l, 11]) == 0)]
+                if ran < todo:
                     break
             elapsed = time.time() - t0
             self.last_ms_per_step = ms.value
@@ -914,7 +1314,9 @@ class GPUBatchGroup:
                 N = int(P.n_total)
                 n_r, n_e = int(c[0]), int(c[1])
                 sd = {"n_returns_total": int(c[5]), "n_overlap_rejected": int(c[6]),
-                      "n_bridge": int(c[7]), "bd_steps": steps, "group_size": L}
+                      "n_bridge": int(c[7]), "bd_steps": steps, "group_size": L,
+                      "stop_reason": "stall" if stalled[l] else
+                      ("done_99.5" if c[9] else "complete")}
                 if s.surrogate is not None:
                     a = audits[l]
                     a["rel_rms"] = math.sqrt(a["err2"] / a["ref2"]) if a["ref2"] > 0 else 0.0

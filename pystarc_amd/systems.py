@@ -82,6 +82,25 @@ def lattice_cloud(n, semi_axes, rng, jitter=0.2):
     return pts, s
 
 
+def fcc_ball(n, nn, rng, jitter=0.03):
+    """n points of a face-centred cubic lattice with nearest-neighbour
+    distance nn, the n sites closest to the centre (every interior site
+    filled), jittered by U(-jitter, jitter) per coordinate, centroid at 0.
+    The largest empty sphere of the lattice is its octahedral hole, nn / sqrt 2
+    (1.45 A at nn = 2.05), so with a small jitter no ligand centroid fits
+    inside the ball under the 1.5 A overlap rule."""
+    a = nn * math.sqrt(2.0)
+    R = (n / (4.0 / a ** 3) * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) + 2.0 * a
+    m = int(math.ceil(R / a)) + 1
+    ax = np.arange(-m, m + 1) * a
+    g = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), axis=-1).reshape(-1, 3)
+    basis = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]]) * a
+    pts = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)
+    order = np.argsort(np.linalg.norm(pts, axis=1), kind="stable")[:n]
+    p = pts[order] + rng.uniform(-jitter, jitter, size=(n, 3))
+    return p - p.mean(axis=0)
+
+
 def charges_with_net(n, sigma, net, rng):
     q = rng.normal(0.0, sigma, size=n)
     return q + (net - q.sum()) / n
@@ -261,7 +280,8 @@ def make_engine(system: BDSystem, grids: Dict[str, list], **kw):
 
 def from_pqr(receptor_pqr, ligand_pqr, rxns_xml, *, b=None, n_traj=100_000, seed=1,
              grid=(193, 0.5), lig_grid=None, a_rec=None, a_lig=None, debye=DEBYE_015M,
-             max_steps=1_000_000, hi=True, overlap=True, dtype="float32", name=None):
+             max_steps=1_000_000, hi=True, overlap=True, dtype="float32", name=None,
+             alpha=ALPHA_DEFAULT, min_core_dt=0.2, r_escape=None, point_charges=False):
     """A batch system from the reference's input files: receptor and ligand
     PQR (pystarc/structures/pqr_io.py) and rxns.xml with 1-based atom indices
     (pystarc/pipeline/geometry.py:321-389).  Both molecules are centred on
@@ -271,8 +291,13 @@ def from_pqr(receptor_pqr, ligand_pqr, rxns_xml, *, b=None, n_traj=100_000, seed
     Debye-Hueckel and Born sums of the PQR charges and radii on cubic grids
     (``grid`` / ``lig_grid`` = (nodes, spacing)).  Hydrodynamic radii default
     to each molecule's bounding radius (farthest atom centre + its radius);
-    b defaults to 1.1 x (a_rec + a_lig + 15 A); r_escape = 2 b as the
-    pipeline passes it (geometry.py:291-292)."""
+    b defaults to 1.1 x (a_rec + a_lig + 15 A); r_escape to 2 b as the
+    pipeline passes it (geometry.py:291-292).  alpha is the Born desolvation
+    coefficient (desolvation_alpha, input_parser.py:75-77) and min_core_dt the
+    dt floor (minimum_core_dt), as an input.xml sets them.  point_charges:
+    the potential grid holds the screened Coulomb field of point charges
+    (ion-exclusion radius 0), the field of the analytic two-sphere solution
+    (examples/two_charged_spheres/analytical.py:39-40)."""
     from .pipeline.geometry import pathway_set_from_rxns
     from .structures.pqr_io import parse_pqr
     mol1, mol2 = parse_pqr(receptor_pqr), parse_pqr(ligand_pqr)
@@ -290,15 +315,19 @@ def from_pqr(receptor_pqr, ligand_pqr, rxns_xml, *, b=None, n_traj=100_000, seed
         b = 1.1 * (a_rec + a_lig + 15.0)
     ps = pathway_set_from_rxns(rxns_xml)
     mob = MobilityTensor.from_radii(a_rec, a_lig)
-    params = _params(n_traj, float(b), seed, max_steps, overlap=overlap, hi=hi)
+    params = _params(n_traj, float(b), seed, max_steps, overlap=overlap, hi=hi,
+                     min_core_dt=min_core_dt)
+    if r_escape is not None:
+        params.r_escape = float(r_escape)
     gn, gh = grid
     ln, lh = lig_grid if lig_grid is not None else grid
-    specs = [GridSpec("elec", 0, (gn,) * 3, _cube_origin(gn, gh), gh, dtype, _atoms5(rp, rq, rr), debye),
+    specs = [GridSpec("elec", 0, (gn,) * 3, _cube_origin(gn, gh), gh, dtype,
+                      _atoms5(rp, rq, 0.0 * rr if point_charges else rr), debye),
              GridSpec("born", 1, (gn,) * 3, _cube_origin(gn, gh), gh, dtype, _atoms5(rp, rq, rr), debye),
              GridSpec("lig_born", 1, (ln,) * 3, _cube_origin(ln, lh), lh, dtype,
                       _atoms5(lp, lq, lr), debye)]
     mp = MultipoleExpansion(rp, rq, debye, 78.0)
-    kw = dict(alpha=ALPHA_DEFAULT, receptor_charge=float(rq.sum()), debye_length=debye, sdie=78.0,
+    kw = dict(alpha=float(alpha), receptor_charge=float(rq.sum()), debye_length=debye, sdie=78.0,
               rec_positions=rp, rec_charges=rq, multipole_expansion=mp)
     return BDSystem(name or f"{mol1.name}-{mol2.name}", mol1, mol2, mob, ps, params, kw, specs,
                     {"a_rec": a_rec, "a_lig": a_lig, "files": [str(receptor_pqr), str(ligand_pqr),
@@ -307,18 +336,22 @@ def from_pqr(receptor_pqr, ligand_pqr, rxns_xml, *, b=None, n_traj=100_000, seed
 
 def multi_ligand(n_traj=1_000_000, seed=5015, rec_seed=5005, lig_seed=5006, n_rec=4700,
                  n_lig=16, lig_atoms=(12, 64), rec_grid=(257, 0.5), lig_grid=(65, 0.4),
-                 b=40.0, max_steps=1_000_000, rec_net=4.0, density=0.0826):
+                 b=40.0, max_steps=1_000_000, rec_net=4.0, density=0.0826, rec_nn=2.05):
     """Config 5: a 4,700-atom receptor on 257^3 FP32 potential + Born grids
     (0.5 A) and n_lig ligands of U{12..64} atoms (charges N(0, 0.3) e, integer
     nets in [-2, 2]), each with its own 65^3 Born grid at 0.4 A and its own
-    two-of-three contact criterion (4.0 A, docked pose); b = 40 A.  Atom
-    clouds are jittered lattices at the atom density of the 1,700-atom
+    two-of-three contact criterion (4.0 A, docked pose); b = 40 A.  The
+    receptor is a filled FCC ball (nearest neighbours rec_nn = 2.05 A, radius
+    ~19 A) whose interior holds no empty sphere of the 1.5 A overlap radius,
+    so ligand centroids stay outside it as they stay outside a real protein;
+    ligands are jittered lattices at the atom density of the 1,700-atom
     globule (0.0826 / A^3).  Returns one BDSystem per ligand: the first
     carries the receptor grids, the others only their ligand grid (their
     engines share the receptor grids, ``make_engine(..., receptor_from=)``)."""
     rr_ = np.random.default_rng(rec_seed)
-    r_rec = (n_rec / density * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
-    rp, s1 = lattice_cloud(n_rec, (r_rec,) * 3, rr_)
+    rp = fcc_ball(n_rec, rec_nn, rr_)
+    r_rec = float(np.linalg.norm(rp, axis=1).max())
+    s1 = rec_nn
     rq = charges_with_net(n_rec, 0.35, rec_net, rr_)
     rrad = rr_.uniform(1.4, 2.0, n_rec)
     mol1 = Molecule.from_arrays(rp, rq, rrad, name="receptor")

@@ -61,3 +61,31 @@ def test_files_run_writes_results_json(tmp_path):
     assert d["n_reacted"] + d["n_escaped"] >= int(0.995 * 20000)
     assert 0.0 < d["P_rxn_low"] <= d["P_rxn"] <= d["P_rxn_high"] < 1.0
     assert d["k_on"] > 0 and d["k_b"] == res.k_b and d["steps_per_sec"] > 0
+
+
+def example_system(n, seed=11111113):
+    """examples/two_charged_spheres with its own input.xml parameters
+    (input.xml:10-31): b = 10, q = 20 (2 b), Debye length 7.828, Born
+    desolvation off (alpha 0), HI and overlap off, no dt floor; hydrodynamic
+    radii 1.005 A (D_rel 0.43371, reference tests:3992-3995); the potential of
+    the point charges, as the analytic solution (analytical.py:39-40)."""
+    return S.from_pqr(F / "two_spheres_receptor.pqr", F / "two_spheres_ligand.pqr",
+                      F / "two_spheres_rxns.xml", b=10.0, n_traj=n, seed=seed, grid=(129, 0.5),
+                      a_rec=1.005, a_lig=1.005, debye=7.828, hi=False, overlap=False, alpha=0.0,
+                      min_core_dt=0.0, dtype="float64", point_charges=True)
+
+
+def test_two_spheres_example_reproduces_the_published_p_rxn():
+    """The exact P_rxn of the example, 0.4501
+    (examples/two_charged_spheres/README.md:18), lies inside the 95 % Wilson
+    interval of 100,000 trajectories run with the example's parameters."""
+    sysm = example_system(100_000)
+    assert sysm.params.r_escape == 20.0
+    eng = S.make_engine(sysm, dx_grids(numpy_grids(sysm)))
+    res = GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params,
+                            eng).run()
+    lo, hi = res.reaction_probability_ci(0.95)
+    print(f"two-sphere example: P={res.reaction_probability:.5f} [{lo:.5f}, {hi:.5f}], "
+          f"k_b={res.k_b:.3f}, {res.n_completed} completed", flush=True)
+    assert res.n_completed >= int(0.995 * 100_000)
+    assert lo <= 0.4501 <= hi

@@ -62,6 +62,6 @@ def test_surrogate_k_on_and_gain(pp, trained):
           f"audit rel rms {audit['rel_rms']:.3e}")
     assert audit["fallback_step"] is None
     assert audit["rel_rms"] < AUDIT_TOL
-    lo, hi = wilson_ci(plain.n_reacted, plain.n_escaped, 0.999)
-    assert lo - 0.005 <= surr.reaction_probability <= hi + 0.005
+    lo, hi = wilson_ci(plain.n_reacted, plain.n_escaped, 0.95)
+    assert lo <= surr.reaction_probability <= hi
     assert surr.steps_per_sec > plain.steps_per_sec

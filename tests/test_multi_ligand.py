@@ -25,3 +25,21 @@ def test_multi_ligand_spec():
     again = S.multi_ligand(n_traj=1000)
     assert [s.meta["atoms"] for s in again] == atoms
     np.testing.assert_array_equal(again[3].mol2.positions_array(), ls[3].mol2.positions_array())
+
+
+def test_receptor_interior_has_no_room_for_a_ligand_centroid():
+    """Probe the receptor on a 0.2 A grid: inside 2.5 A below its surface
+    every point lies within the 1.5 A overlap radius of some atom, so the
+    overlap rule keeps ligand centroids out of the interior."""
+    from scipy.spatial import cKDTree
+    rec = S.multi_ligand(n_traj=10, n_lig=1)[0].mol1.positions_array()
+    rec = rec - rec.mean(axis=0)
+    R = np.linalg.norm(rec, axis=1).max()
+    ax = np.arange(-R, R + 0.2, 0.2)
+    G = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), axis=-1).reshape(-1, 3)
+    inner = G[np.linalg.norm(G, axis=1) < R - 2.5]
+    d, _ = cKDTree(rec).query(inner)
+    assert len(inner) > 1_000_000
+    assert d.max() < 1.5
+    # and the atoms stay apart
+    assert cKDTree(rec).query(rec, k=2)[0][:, 1].min() > 1.9

@@ -51,19 +51,6 @@ def test_free_diffusion_fallback():
     assert rp == 0.5 and abs(kb - 4 * math.pi * 0.2 * 30) < 1e-12
 
 
-# --- constants / Yukawa (reference tests:3940-4070) ---------------------------
-def test_vfac_and_potential():
-    V = 1.0 * -1.0 / (4.0 * math.pi * 78.0 * ph.EPS0)
-    assert abs(V - (-7.1847)) < 0.01
-    assert abs(V * math.exp(-10.0 / 7.828) / 10.0 - (-0.200268)) < 1e-3
-
-
-def test_em_drift_noise_numbers():
-    D, F, dt = 0.43371, -0.04561, 0.2
-    assert abs(D * F * dt - (-0.00396)) < 1e-4
-    assert abs(math.sqrt(2 * D * dt) - 0.4169) < 1e-3
-
-
 def test_stokes_D():
     # D_rel of two 1.005 A spheres (reference tests:3992-3995), SI viscosity
     assert abs(2 * ph.stokes_dt(1.005) - 0.43371) < 0.002
