@@ -1,23 +1,34 @@
 """Benchmark: trajectory-steps/s of the batched BD step on one MI355X,
 surrogate vs plain, with k_on of both paths from the same run.
 
-Workload: the 1,700 / 1,400-atom protein-protein pair (193^3 FP32 receptor
-potential + Born grids and ligand Born grid at 0.5 A, bidirectional Born,
-two-body RPY 12x12 mobility with hydrodynamic radii 17 / 15 A and FP64
-Cholesky noise (--hydro scalar: the reference's scalar D_parallel), overlap
-rejection, 2-of-5 contact reaction < 5.5 A, b = 60 A, r_esc = 120 A).  The
-last TUNE_STEPS warm-up steps run the force-launch autotuner.  A "step" is one BD step of every live trajectory slot;
-value = trajectory-steps per second over the timed steps of the surrogate +
-FP32 path (inputs resident in HBM).  Paths timed on identical seeds:
+Workloads (--workload):
+  protein_protein  (default, the headline) the 1,700 / 1,400-atom pair:
+                   193^3 FP32 receptor potential + Born grids and ligand Born
+                   grid at 0.5 A, bidirectional Born, two-body RPY 12x12
+                   mobility with hydrodynamic radii 17 / 15 A and FP64
+                   Cholesky noise (--hydro scalar: the reference's scalar
+                   D_parallel), overlap rejection, 2-of-5 contacts < 5.5 A,
+                   b = 60 A, r_esc = 120 A.
+  protein_ligand   the 3,220 / 18-atom protein-small-ligand system (161^3 /
+                   65^3 grids, 2-of-3 contacts < 4 A, b = 40 A) and, as the
+                   surrogate path, the same system with the learned surrogate
+                   trained on plain-path trajectory poses plus shell samples.
+  multi_ligand     16 ligands of U{12..64} atoms against a 4,700-atom receptor
+                   on 257^3 grids, one grouped force launch per step.
+
+A "step" is one BD step of every live trajectory slot; value =
+trajectory-steps per second over the timed steps of the surrogate + FP32
+path (inputs resident in HBM; the step graph is captured before the timed
+window).  Paths timed on identical seeds:
 
   fp64       plain grid path, FP64 forces (the reference's arithmetic)
   fp32       plain grid path, FP32 forces
   surrogate  MLP beyond contact + 8 A, FP32 grid path inside, per-step audit
 
-k_on: each of fp64 and surrogate then runs --kon-trajectories complete
-trajectories; both P_rxn with Wilson 95 % intervals go in the JSON line.
+k_on: fp64 and surrogate each run --kon-trajectories complete trajectories;
+P_rxn with Wilson 95 % intervals go in the JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--slots S] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -50,7 +61,10 @@ def log(*a):
 def build_system(args, n_traj=None):
     from pystarc_amd import systems as S
     from pystarc_amd._lib import Context
-    sysm = S.protein_protein(n_traj=n_traj or args.trajectories, seed=3013)
+    if args.workload == "protein_ligand":
+        sysm = S.protein_ligand(n_traj=n_traj or args.trajectories, seed=2012)
+    else:
+        sysm = S.protein_protein(n_traj=n_traj or args.trajectories, seed=3013)
     t0 = time.time()
     ctx = Context(0)
     grids = S.build_grids(sysm, ctx)
@@ -97,6 +111,12 @@ def run_window(sysm, eng, args, surrogate=None):
     if done < args.warmup:
         # the tuner ran fewer steps than budgeted: the warm-up is exactly W steps
         ctx.check(lib.pst_prod_run(h, args.warmup - done, args.graph_steps, ptr(c16)), "warmup")
+        done = args.warmup
+    # the step graph of the chosen launch is captured and instantiated here,
+    # outside the timed window (pst_prod_run re-launches it while nothing it
+    # captured changes); these steps count as warm-up
+    ctx.check(lib.pst_prod_run(h, args.graph_steps, args.graph_steps, ptr(c16)), "capture")
+    done += int(c16[8])
     if surrogate is not None:
         ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
     # trajectory-steps so far (warm-up and tuning steps), read right before timing
@@ -125,6 +145,7 @@ def run_window(sysm, eng, args, surrogate=None):
     traj_steps = c16[3] - base_steps
     audit = math.sqrt(c16[12] / c16[13]) if c16[13] > 0 else None
     return {"wall_s": wall, "device_ms_per_step": ms.value, "steps": int(c16[8]),
+            "warmup_steps_run": done, "tune_steps": tune["steps"] if tune else 0,
             "traj_steps": traj_steps, "value": traj_steps / wall if wall > 0 else 0.0,
             "pos": pos[live], "quat": q[live], "counters": c16.tolist(), "audit_rel_rms": audit,
             "n_surrogate_last_step": int(c16[11]),
@@ -166,45 +187,172 @@ def full_run(sysm, eng, args, surrogate=None):
     return out
 
 
-def cpu_baseline(sysm, grids, pos, quat, hydro, seconds=15.0, n_traj=128):
-    """The NumPy FP64 oracle of the same production step (same hydrodynamics)
-    on a bounded sample: n_traj live poses taken from the GPU run, stepped for
-    about `seconds` of wall time."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _cpu_worker(task):
+    """One host core: the NumPy FP64 production oracle on its own poses for
+    about `seconds` of wall time; returns (trajectory-steps, seconds, loops)."""
+    import os as _os
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        _os.environ[k] = "1"
+    root, npz, system_kind, hydro, seconds, wid = task
+    import sys as _sys
+    if root not in _sys.path:
+        _sys.path.insert(0, root)
+    import numpy as _np
+    import time as _time
     from oracle.production import ProductionOracle
+    from pystarc_amd import systems as S
     from tests._systems import oracle_params, oracle_system
-    np_grids = {k: [(np.asarray(g.data), np.asarray(g.origin), float(g.delta[0, 0])) for g in v]
-                for k, v in grids.items()}
-    sel = np.random.default_rng(0).choice(len(pos), size=min(n_traj, len(pos)), replace=False)
-    ora = ProductionOracle(oracle_system(sysm, np_grids, chunk=0),
-                           oracle_params(sysm, n_trajectories=len(sel)), hydro=hydro)
-    ora.pos[:] = pos[sel]
-    ora.q[:] = quat[sel]
-    w0 = time.perf_counter()
+    z = _np.load(npz, allow_pickle=False)
+    sysm = (S.protein_ligand(n_traj=100) if system_kind == "protein_ligand"
+            else S.protein_protein(n_traj=100))
+    grids = {}
+    for role in ("elec", "born", "lig_born"):
+        grids[role] = [(z[f"{role}{i}_data"], z[f"{role}{i}_origin"], float(z[f"{role}{i}_h"]))
+                       for i in range(int(z[f"{role}_n"]))]
+    pos, quat = z["pos"][wid::int(z["n_workers"])], z["quat"][wid::int(z["n_workers"])]
+    ora = ProductionOracle(oracle_system(sysm, grids, chunk=0),
+                           oracle_params(sysm, n_trajectories=len(pos)), hydro=hydro)
+    ora.pos[:] = pos
+    ora.q[:] = quat
+    w0 = _time.perf_counter()
     steps = loops = 0
-    while time.perf_counter() - w0 < seconds:
+    while _time.perf_counter() - w0 < seconds:
         before = ora.total_steps
         if not ora.step():
             break
         loops += 1
         steps += ora.total_steps - before
-    wall = time.perf_counter() - w0
-    return {"value": steps / wall if wall > 0 else 0.0, "unit": "trajectory-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{len(sel)} live poses from the GPU run stepped {loops} times "
-                      f"({steps} trajectory-steps, {wall:.1f}s, single-threaded NumPy FP64 oracle, "
-                      f"hydro {hydro})"}
+    return steps, _time.perf_counter() - w0, loops
 
 
-def measured_traffic(path):
+def cpu_baseline(sysm, grids, pos, quat, hydro, args, seconds=15.0, per_worker=32):
+    """The NumPy FP64 oracle of the same production step (same
+    hydrodynamics) on a bounded sample of live poses from the GPU run: one
+    host core alone, then every core of the box's CPU share at once (one
+    process per core, each on its own poses), each for about `seconds`."""
+    import multiprocessing as mp
+    import tempfile
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    n_workers = max(1, min(share, 16))
+    sel = np.random.default_rng(0).choice(len(pos), size=min(per_worker * n_workers, len(pos)),
+                                          replace=False)
+    arrays = {"pos": pos[sel], "quat": quat[sel], "n_workers": np.array(n_workers)}
+    for role, lst in grids.items():
+        arrays[f"{role}_n"] = np.array(len(lst))
+        for i, g in enumerate(lst):
+            arrays[f"{role}{i}_data"] = np.asarray(g.data)
+            arrays[f"{role}{i}_origin"] = np.asarray(g.origin)
+            arrays[f"{role}{i}_h"] = np.array(float(np.asarray(g.delta)[0, 0]))
+    tmp = tempfile.NamedTemporaryFile(suffix=".npz", delete=False)
+    tmp.close()
+    try:
+        np.savez(tmp.name, **arrays)
+        ctx = mp.get_context("spawn")
+        base = (ROOT, tmp.name, args.workload, hydro, seconds)
+        with ctx.Pool(1) as pool:
+            s1, w1, l1 = pool.map(_cpu_worker, [base + (0,)])[0]
+        log(f"cpu baseline, 1 core: {s1 / w1:.4g} traj-steps/s")
+        with ctx.Pool(n_workers) as pool:
+            res = pool.map(_cpu_worker, [base + (w,) for w in range(n_workers)])
+    finally:
+        os.unlink(tmp.name)
+    total = sum(r[0] / r[1] for r in res)
+    return {"value": total, "unit": "trajectory-steps/s", "cores": n_workers, "kind": "port",
+            "cpu_model": _cpu_model(), "one_core_value": s1 / w1,
+            "sample": f"{len(sel)} live poses from the GPU run, {per_worker} per process, stepped "
+                      f"for ~{seconds:.0f}s by the NumPy FP64 production oracle (hydro {hydro}): "
+                      f"{n_workers} single-threaded processes at once, summed "
+                      f"({sum(r[0] for r in res)} trajectory-steps); one core alone "
+                      f"{s1} trajectory-steps in {w1:.1f}s"}
+
+
+def gather_ceiling(eng, n_points=1 << 24, reps=10):
+    """Gather ceiling of the receptor grids (pst_gather_ceiling): uniform
+    and clustered points, algorithmic GB/s of the 160-byte stencil."""
+    out = {}
+    for mode, name in ((0, "uniform"), (1, "clustered")):
+        o4 = np.zeros(4)
+        eng.ctx.check(eng.ctx.lib.pst_gather_ceiling(eng.ctx.h, mode, n_points, reps, ptr_np(o4)),
+                      "pst_gather_ceiling")
+        out[name] = {"ms": float(o4[0]), "gbs": float(o4[1]), "points": int(o4[2])}
+    return out
+
+
+def ptr_np(a):
+    from pystarc_amd._lib import ptr
+    return ptr(a)
+
+
+def harvest_poses(sysm, eng, args, n_want, r_on, every=4):
+    """Poses of the plain FP64 path's own trajectories beyond r_on: a fresh
+    production run whose live slots are read every `every` steps."""
+    from pystarc_amd._lib import ptr
+    sim = make_sim(sysm, eng, args, n_slots=args.slots)
+    sim._return_prob = sim._compute_return_prob()
+    P = sim._step_params()
+    lib, ctx = eng.ctx.lib, eng.ctx
+    h = sim._make_sim(P)
+    S_ = sim.n_slots
+    c16 = np.zeros(16)
+    pos = np.zeros((S_, 3)); q = np.zeros((S_, 4)); st = np.zeros(S_, dtype=np.int32)
+    got_p, got_q, n = [], [], 0
+    try:
+        ctx.check(lib.pst_prod_init(h, 0), "pst_prod_init")
+        while n < n_want:
+            ctx.check(lib.pst_prod_run(h, every, 0, ptr(c16)), "harvest")
+            ctx.check(lib.pst_sim_get_state(h, ptr(pos), ptr(q), ptr(st), None), "get_state")
+            keep = (st == 0) & (np.linalg.norm(pos, axis=1) > r_on)
+            got_p.append(pos[keep].copy())
+            got_q.append(q[keep].copy())
+            n += int(keep.sum())
+            if c16[9] or int(c16[8]) < every:
+                break
+    finally:
+        lib.pst_sim_destroy(h)
+    return np.concatenate(got_p)[:n_want], np.concatenate(got_q)[:n_want]
+
+
+def measured_traffic(workload, path):
     """HBM bytes per force launch of `path` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, calibrated), or None."""
+    summary of the workload (FETCH_SIZE and WRITE_SIZE, calibrated), or None."""
     f = os.path.join(ROOT, "profiles", "k_forces_traffic.json")
     try:
         with open(f) as fh:
             d = json.load(fh)
+        if d.get("workload_key", "protein_protein") != workload:
+            return None
         return d["paths"][path]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def workload_text(args):
+    if args.workload == "protein_ligand":
+        return ("protein-small-ligand 3220/18 atoms, 161^3 FP32 receptor potential+Born grids "
+                "(0.5 A), 65^3 ligand Born grid (0.4 A), bidirectional Born, "
+                + ("two-body RPY 12x12 mobility" if args.hydro == "rpy12" else "scalar diffusion")
+                + ", b=40 A, 2-of-3 contacts < 4 A; surrogate path = the same system with the "
+                "MLP surrogate")
+    return ("protein-protein 1700/1400 atoms, 193^3 FP32 grids (receptor potential+Born, ligand "
+            "Born), bidirectional Born, "
+            + ("two-body RPY 12x12 mobility (radii 17/15 A) with FP64 Cholesky noise"
+               if args.hydro == "rpy12" else "scalar HI")
+            + ", b=60 A, 2-of-5 contacts < 5.5 A")
 
 
 def train_surrogate(sysm, eng64, args, extra_poses=None):
@@ -384,15 +532,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kon", action="store_true")
     ap.add_argument("--pmc", action="store_true", help="counter-collection mode (no JSON line)")
-    ap.add_argument("--hydro", default="rpy12", choices=("rpy12", "scalar"),
-                    help="two-body RPY 12x12 mobility (config default) or scalar D_parallel")
+    ap.add_argument("--hydro", default=None, choices=("rpy12", "scalar"),
+                    help="two-body RPY 12x12 mobility (the protein pair's default) or scalar "
+                         "D_parallel (the protein-ligand system's default)")
+    ap.add_argument("--train-from-trajectories", type=int, default=None,
+                    help="training poses taken from the plain FP64 path's own trajectories "
+                         "(default: 10,000,000 for protein_ligand, 0 for protein_protein)")
     ap.add_argument("--outer", default="return_prob", choices=("return_prob", "lmz"),
                     help="escape handling: the reference's return probability at r_esc, or "
                          "the LMZ outer propagator from 1.1 b")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the heuristic force launch instead of tuning it in the warm-up")
-    ap.add_argument("--workload", default="protein_protein", choices=("protein_protein", "multi_ligand"),
-                    help="protein_protein (the headline) or multi_ligand (16 ligands, one launch)")
+    ap.add_argument("--workload", default="protein_protein",
+                    choices=("protein_protein", "protein_ligand", "multi_ligand"),
+                    help="protein_protein (the headline), protein_ligand (with the surrogate as "
+                         "its surrogate path) or multi_ligand (16 ligands, one launch)")
     ap.add_argument("--ligand-slots", type=int, default=8192, help="multi_ligand: slots per ligand")
     ap.add_argument("--ml-train-poses", type=int, default=1_000_000)
     ap.add_argument("--ml-train-epochs", type=int, default=12)
@@ -406,6 +560,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(2)
+    if args.hydro is None:
+        args.hydro = "rpy12" if args.workload == "protein_protein" else "scalar"
+    if args.train_from_trajectories is None:
+        args.train_from_trajectories = 10_000_000 if args.workload == "protein_ligand" else 0
     if args.pmc:
         pmc_mode(args)
         return
@@ -418,6 +576,7 @@ def main():
     engines = {p: S.make_engine(sysm, grids, precision="fp64" if p == "fp64" else "fp32")
                for p in ("fp64", "fp32")}
     surr = None
+    traj_poses = 0
     results = {}
     # plain paths first: their live poses join the surrogate's training set
     order = [p for p in paths if p != "surrogate"] + [p for p in paths if p == "surrogate"]
@@ -427,6 +586,17 @@ def main():
             if results:
                 extra = (np.concatenate([r["pos"] for r in results.values()]),
                          np.concatenate([r["quat"] for r in results.values()]))
+            if args.train_from_trajectories > 0:
+                mob = sysm.mob
+                r_on = float(mob.radius1 + mob.radius2) + 8.0
+                t0 = time.time()
+                hp, hq = harvest_poses(sysm, engines["fp64"], args, args.train_from_trajectories,
+                                       r_on)
+                log(f"harvested {len(hp)} poses beyond {r_on:.1f} A from plain FP64 trajectories "
+                    f"in {time.time() - t0:.1f}s")
+                extra = (hp, hq) if extra is None else (np.concatenate([extra[0], hp]),
+                                                        np.concatenate([extra[1], hq]))
+                traj_poses = len(hp)
             surr = train_surrogate(sysm, engines["fp64"], args, extra)
         eng = engines["fp32" if p == "surrogate" else p]
         log(f"path {p}: warmup {args.warmup}, timed {args.steps} steps, {args.slots} slots")
@@ -457,8 +627,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(sysm, grids, best["pos"], best["quat"], args.hydro)
-        log(f"cpu baseline {cpu['value']:.4g} traj-steps/s")
+        cpu = cpu_baseline(sysm, grids, best["pos"], best["quat"], args.hydro, args)
+        log(f"cpu baseline {cpu['value']:.4g} traj-steps/s on {cpu['cores']} cores "
+            f"({cpu['cpu_model']}), {cpu['one_core_value']:.4g} on one")
+    # gather ceiling of the receptor grids (the force kernel's bound below HBM)
+    ceiling = gather_ceiling(engines["fp32"])
+    log(f"gather ceiling: uniform {ceiling['uniform']['gbs']:.0f} GB/s, clustered "
+        f"{ceiling['clustered']['gbs']:.0f} GB/s (algorithmic, 160 B per stencil)")
     # dominant kernel of the headline path: MFMA roofline of the MLP when it
     # takes longer than the force kernel on the near trajectories, else HBM
     if head == "surrogate" and (best["roof_force"] is None
@@ -476,9 +651,11 @@ def main():
         kname = f"k_forces<{tc}, {lc['G']}, true, {zp}>" if lc else "k_forces"
         roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS,
-                "traffic": measured_traffic(head),
+                "traffic": measured_traffic(args.workload, head),
                 "kernel": kname, "ms_per_launch": rf["ms"], "launches": rf["launches"],
-                "census": rf["census"], "algorithmic_bytes": rf["bytes"]}
+                "census": rf["census"], "algorithmic_bytes": rf["bytes"],
+                "gather_ceiling_gbs": ceiling["clustered"]["gbs"],
+                "frac_of_gather_ceiling": rf["achieved_gbs"] / ceiling["clustered"]["gbs"]}
     plain = results.get("fp64")
     out = {
         "metric": METRIC,
@@ -494,11 +671,7 @@ def main():
         "dtype": {"surrogate": "f64 state, f32 grid forces, bf16 MLP (f32 accumulate)",
                   "fp32": "f64 state, f32 forces", "fp64": "f64"}[head],
         "data": "synthetic",
-        "config": {"workload": "protein-protein 1700/1400 atoms, 193^3 FP32 grids (receptor "
-                               "potential+Born, ligand Born), bidirectional Born, "
-                               + ("two-body RPY 12x12 mobility (radii 17/15 A) with FP64 "
-                                  "Cholesky noise" if args.hydro == "rpy12" else "scalar HI")
-                               + ", b=60 A, 2-of-5 contacts < 5.5 A",
+        "config": {"workload": workload_text(args),
                    "path": head, "slots": args.slots, "graph_steps": args.graph_steps,
                    "hydro": args.hydro, "outer": args.outer, "launch": best.get("launch")},
         "speedup_vs_plain_fp64": (best["value"] / plain["value"]) if plain else None,
@@ -506,9 +679,16 @@ def main():
         "cpu_baseline": cpu,
         "k_on": kon,
         "surrogate": None if surr is None else {
-            "validation_force_rel_rms": surr.val_error, "audit_tol": args.audit_tol,
-            "r_on": surr.r_on, "train_poses": args.train_poses, "epochs": args.train_epochs},
+            "validation_force_rel_rms": surr.val_error,
+            "validation_torque_rel_rms": getattr(surr, "val_torque_error", None),
+            "audit_tol": args.audit_tol, "r_on": surr.r_on, "train_poses": args.train_poses,
+            "trajectory_poses": traj_poses, "epochs": args.train_epochs,
+            "k_on_shift": (kon["surrogate"]["P"] / kon["fp64"]["P"] - 1.0)
+            if "surrogate" in kon and "fp64" in kon and kon["fp64"]["P"] > 0 else None},
+        "gather_ceiling": ceiling,
         "paths": {k: {"value": v["value"], "device_ms_per_step": v["device_ms_per_step"],
+                      "wall_ms_per_step": v["wall_s"] * 1e3 / max(v["steps"], 1),
+                      "tune_steps": v["tune_steps"], "warmup_steps_run": v["warmup_steps_run"],
                       "audit_rel_rms": v["audit_rel_rms"],
                       "force_ms": v["roof_force"]["ms"] if v.get("roof_force") else None,
                       "force_gbs": v["roof_force"]["achieved_gbs"] if v.get("roof_force") else None,

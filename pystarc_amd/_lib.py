@@ -85,6 +85,7 @@ _SIGS = {
     "pst_forces_pose_group": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int], C.c_int),
     "pst_time_forces_pose": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp],
                              C.c_int),
+    "pst_gather_ceiling": ([_vp, C.c_int, C.c_longlong, C.c_int, _vp], C.c_int),
     "pst_rpy12_eval": ([_vp, C.c_int, _vp, C.c_double, C.c_double, _vp, _vp], C.c_int),
     "pst_rpy12_move": ([_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp, _vp, _vp,
                         _vp, _vp, _vp, _vp], C.c_int),

@@ -153,6 +153,16 @@ struct pst_sim {
     return 1;              \
   } while (0)
 
+namespace {
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+  }
+};
+}  // namespace
+
 static hipError_t set_dev(pst_ctx* ctx) { return hipSetDevice(ctx->device); }
 
 static GridDev grid_record(const DevGrid& d) { return d.g; }
@@ -975,6 +985,42 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
   return 0;
 }
 
+int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, double* out4) {
+  if (!ctx) return 1;
+  if (!ctx->pair.mem) FAIL("pst_gather_ceiling: needs the FP32 potential + Born pair grid");
+  if (mode != 0 && mode != 1) FAIL("pst_gather_ceiling: mode must be 0 or 1");
+  if (n_points <= 0 || n_points > (1ll << 30) || reps <= 0 || !out4)
+    FAIL("pst_gather_ceiling: bad arguments");
+  CK(set_dev(ctx));
+  float* out = nullptr;
+  CK(hipMalloc(&out, (size_t)n_points * 4));
+  EventPair ev;
+  hipError_t e = hipEventCreate(&ev.a);
+  if (e == hipSuccess) e = hipEventCreate(&ev.b);
+  const long long nb = (n_points + 255) / 256;
+  const int blocks = (int)(nb < 65536 ? nb : 65536);
+  if (e == hipSuccess) e = launch_gather_probe(ctx->pair.g, n_points, mode, 12345, out, blocks,
+                                               ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev.a, ctx->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    e = launch_gather_probe(ctx->pair.g, n_points, mode, 777 + r, out, blocks, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev.b, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev.b);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev.a, ev.b);
+  hipFree(out);
+  if (e != hipSuccess) {
+    ctx->err = std::string("pst_gather_ceiling: ") + hipGetErrorString(e);
+    return 2;
+  }
+  const double per = ms / reps;
+  out4[0] = per;
+  out4[1] = 160.0 * (double)n_points / (per * 1e-3) / 1e9;
+  out4[2] = (double)n_points;
+  out4[3] = 160.0;
+  return 0;
+}
+
 // ----------------------------------------------------------------------------
 // two-body RPY mobility
 // ----------------------------------------------------------------------------
@@ -1564,16 +1610,6 @@ static int step_graph(pst_sim* s, int gs) {
   s->g_key.swap(key);
   return 0;
 }
-
-namespace {
-struct EventPair {
-  hipEvent_t a = nullptr, b = nullptr;
-  ~EventPair() {
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
-  }
-};
-}  // namespace
 
 int pst_prod_run(pst_sim* s, long long max_steps, int graph_steps, double* c16) {
   if (!s) return 1;

@@ -20,6 +20,9 @@ hipError_t launch_census(const ForceSys& S, const ForceWork& W, unsigned long lo
                          hipStream_t st);
 // Empty marker kernel bracketing a timing window in a kernel trace.
 hipError_t launch_window_mark(hipStream_t st);
+// Gather ceiling: n points, the FP32 pair-grid stencil at each (pst_probe.hip).
+hipError_t launch_gather_probe(const GridDev& g, long long n, int mode, unsigned long long seed,
+                               float* out, int blocks, hipStream_t st);
 
 // Brick-binned force evaluation: request counts / order per brick of the
 // receptor float2 pair grid and of the ligand Born grid, and per-request results.

@@ -1,0 +1,58 @@
+// Gather ceiling of the receptor grids (measurement only, no reference
+// counterpart).  Each thread gathers the 20-node potential + Born stencil of
+// the FP32 force path (grid_phi_grad_pair: float2 nodes in 4x4x4 bricks,
+// 160 algorithmic bytes per point) at one point and writes one float, so the
+// kernel does the force kernel's gathers with none of its arithmetic:
+//   mode 0: points uniform over the grid's valid box (every gather far from
+//           the last: the line traffic is served by the Infinity Cache / HBM);
+//   mode 1: the points of one workgroup inside a 12 A cube at a random place
+//           (the gathers of a ligand's atoms: neighbouring stencils share
+//           bricks and lines in L2).
+// Its rate is the ceiling a gather-bound force kernel can reach on the same
+// grids.
+#include <hip/hip_runtime.h>
+#include "pst_device.hpp"
+#include "pst_kernels.hpp"
+
+namespace pst {
+
+__device__ inline double hash_unit(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void __launch_bounds__(256) k_gather_probe(GridDev g, long long n, int mode,
+                                                      unsigned long long seed, float* out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const double ext[3] = {g.hi[0] - g.lo[0], g.hi[1] - g.lo[1], g.hi[2] - g.lo[2]};
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    double x[3];
+    if (mode == 0) {
+      for (int d = 0; d < 3; ++d)
+        x[d] = g.lo[0 + d] + ext[d] * hash_unit(seed ^ (unsigned long long)(3 * p + d));
+    } else {
+      // the workgroup's cluster (one per 256 consecutive points)
+      const long long c = p >> 8;
+      for (int d = 0; d < 3; ++d) {
+        const double half = 6.0 < 0.5 * ext[d] ? 6.0 : 0.5 * ext[d];
+        const double cen = g.lo[d] + half + (ext[d] - 2.0 * half) *
+                                                hash_unit(seed ^ (0x5555ull + 3 * (unsigned long long)c + d));
+        x[d] = cen + half * (2.0 * hash_unit(seed ^ (unsigned long long)(3 * p + d + 0x777)) - 1.0);
+      }
+    }
+    Pair<float> phi, gx, gy, gz;
+    grid_phi_grad_pair<float>(g, x[0], x[1], x[2], phi, gx, gy, gz);
+    out[p] = phi.a + phi.b + gx.a + gx.b + gy.a + gy.b + gz.a + gz.b;
+  }
+}
+
+hipError_t launch_gather_probe(const GridDev& g, long long n, int mode, unsigned long long seed,
+                               float* out, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, st, g, n, mode, seed, out);
+  return hipGetLastError();
+}
+
+}  // namespace pst

@@ -224,6 +224,18 @@ class Surrogate:
         err = np.linalg.norm(Fp - F, axis=1)
         return float(np.sqrt((err ** 2).sum() / (np.linalg.norm(F, axis=1) ** 2).sum())), Fp
 
+    def torque_error(self, c, q, T):
+        """Relative RMS torque error of the surrogate against T (FP64 plain
+        path, physical torque mode)."""
+        torch = self.torch
+        X = torch.as_tensor(self.features(c, q), device=self.device)
+        Y = self.predict(X).double().cpu().numpy()
+        env = envelope(np.linalg.norm(c, axis=1), self.r0, self.lam)[:, None]
+        Tp = Y[:, 3:6] * self.scale[1] * env
+        err = np.linalg.norm(Tp - T, axis=1)
+        ref = (np.linalg.norm(T, axis=1) ** 2).sum()
+        return float(np.sqrt((err ** 2).sum() / ref)) if ref > 0 else 0.0
+
     def sim_params(self, audit_n=64):
         return np.array([self.r_on, self.r0, self.lam, 1.0 / self.b, self.scale[0], self.scale[1],
                          self.scale[4], 1.0 if self.use_torque else 0.0, float(audit_n)])
@@ -250,8 +262,10 @@ def build_surrogate(engine, system, n_train=2_000_000, epochs=4, batch=8192, con
     if log:
         log(f"surrogate data: {len(c)} poses labelled by the FP64 grid path in {time.time() - t0:.1f}s")
     s.train(X, Y, epochs=epochs, batch=batch, log=log, W=Wt, lr=lr)
-    cv, qv, Fv, _, _ = s.make_dataset(20000, seed=seed + 1, near_frac=0.0)
+    cv, qv, Fv, Tv, _ = s.make_dataset(20000, seed=seed + 1, near_frac=0.0)
     s.val_error, _ = s.force_error(cv, qv, Fv)
+    s.val_torque_error = s.torque_error(cv, qv, Tv) if s.use_torque else None
     if log:
-        log(f"surrogate validation force RMS relative error {s.val_error:.3e}")
+        log(f"surrogate validation force RMS relative error {s.val_error:.3e}"
+            + (f", torque {s.val_torque_error:.3e}" if s.use_torque else ""))
     return s

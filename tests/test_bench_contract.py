@@ -25,9 +25,11 @@ def test_traffic_summary_covers_every_path():
     sys.path.insert(0, ROOT)
     import bench
     for path in ("surrogate", "fp32", "fp64"):
-        v = bench.measured_traffic(path)
+        v = bench.measured_traffic("protein_protein", path)
         assert isinstance(v, int) and v > 0
-    assert bench.measured_traffic("no_such_path") is None
+    assert bench.measured_traffic("protein_protein", "no_such_path") is None
+    # a summary measured on another workload is not reported for this one
+    assert bench.measured_traffic("protein_ligand", "surrogate") is None
 
 
 @pytest.mark.parametrize("path", ["surrogate", "fp32", "fp64"])
@@ -49,3 +51,14 @@ def test_surrogate_traffic_exceeds_its_algorithmic_bytes():
     e = json.load(open(TRAFFIC))["paths"]["surrogate"]
     alg = 4100 * 104 + (5.6e6 + 5.6e6 + 6.8e6) * 80
     assert e["hbm_bytes_per_launch"] > alg
+
+
+def test_cpu_model_and_workload_text():
+    sys.path.insert(0, ROOT)
+    import argparse
+    import bench
+    assert isinstance(bench._cpu_model(), str) and bench._cpu_model()
+    a = argparse.Namespace(workload="protein_ligand", hydro="scalar")
+    assert "3220/18" in bench.workload_text(a)
+    a = argparse.Namespace(workload="protein_protein", hydro="rpy12")
+    assert "1700/1400" in bench.workload_text(a) and "RPY" in bench.workload_text(a)
