@@ -298,11 +298,11 @@ def ptr_np(a):
     return ptr(a)
 
 
-def harvest_poses(sysm, eng, args, n_want, r_on, every=4):
+def harvest_poses(sysm, eng, args, n_want, r_on, every=4, n_slots=None):
     """Poses of the plain FP64 path's own trajectories beyond r_on: a fresh
     production run whose live slots are read every `every` steps."""
     from pystarc_amd._lib import ptr
-    sim = make_sim(sysm, eng, args, n_slots=args.slots)
+    sim = make_sim(sysm, eng, args, n_slots=n_slots or args.slots)
     sim._return_prob = sim._compute_return_prob()
     P = sim._step_params()
     lib, ctx = eng.ctx.lib, eng.ctx
@@ -360,7 +360,7 @@ def train_surrogate(sysm, eng64, args, extra_poses=None):
     t0 = time.time()
     s = build_surrogate(eng64, sysm, n_train=args.train_poses, epochs=args.train_epochs,
                         batch=args.train_batch, max_rows=max(args.slots, args.train_batch),
-                        log=log, extra_poses=extra_poses)
+                        log=log, extra_poses=extra_poses, lr=args.train_lr)
     log(f"surrogate trained in {time.time() - t0:.1f}s, validation force error {s.val_error:.3f}")
     return s
 
@@ -422,10 +422,15 @@ def multi_ligand_bench(args):
     surr = []
     t0 = time.time()
     for k, (s, e) in enumerate(zip(ls, engines["fp64"]) if "surrogate" in paths else ()):
+        # training poses: shell samples plus poses of the ligand's own plain
+        # FP64 trajectories
+        r_on = float(s.mob.radius1 + s.mob.radius2) + 8.0
+        extra = harvest_poses(s, e, args, args.ml_train_poses, r_on, n_slots=args.ligand_slots)
         surr.append(build_surrogate(e, s, n_train=args.ml_train_poses, epochs=args.ml_train_epochs,
                                     batch=args.train_batch, max_rows=max(args.ligand_slots, args.train_batch),
-                                    seed=k))
-        log(f"ligand {k}: surrogate validation force error {surr[-1].val_error:.3f}")
+                                    seed=k, extra_poses=extra, lr=args.train_lr))
+        log(f"ligand {k}: surrogate validation force error {surr[-1].val_error:.3f} "
+            f"({len(extra[0])} trajectory poses)")
     t_train = time.time() - t0
     if surr:
         log(f"{len(surr)} surrogates trained in {t_train:.1f}s")
@@ -525,8 +530,9 @@ def main():
     ap.add_argument("--kon-trajectories", type=int, default=200_000)
     # the surrogate workload's 2e7 (pose, force) training pairs
     ap.add_argument("--train-poses", type=int, default=20_000_000)
-    ap.add_argument("--train-epochs", type=int, default=6)
-    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--train-batch", type=int, default=16384)
+    ap.add_argument("--train-lr", type=float, default=4e-3)
     ap.add_argument("--audit-n", type=int, default=64)
     ap.add_argument("--audit-tol", type=float, default=0.25)
     ap.add_argument("--no-cpu", action="store_true")
@@ -549,7 +555,7 @@ def main():
                          "its surrogate path) or multi_ligand (16 ligands, one launch)")
     ap.add_argument("--ligand-slots", type=int, default=8192, help="multi_ligand: slots per ligand")
     ap.add_argument("--ml-train-poses", type=int, default=1_000_000)
-    ap.add_argument("--ml-train-epochs", type=int, default=12)
+    ap.add_argument("--ml-train-epochs", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU in this build (asked for {args.gpus}); "

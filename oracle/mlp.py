@@ -1,11 +1,14 @@
 """FP64 forward / backward of the 9 -> 4x256 (SiLU) -> 13 surrogate MLP
 (oracle, test infrastructure only; no reference counterpart).
 
-``bf16=True`` rounds at the same points as the gfx950 kernels (inputs,
-weights, hidden activations and backward gradients to BF16, round to nearest
-even) and accumulates in FP64, so the kernels can be compared to it up to
-accumulation order; ``bf16=False`` is the exact FP64 network, pinned in the
-tests by torch autograd (float64, CPU) and by finite differences.
+``bf16=True`` rounds at the same points as the gfx950 kernels (the weights
+of layers 1-4, every hidden activation and the backward gradients to BF16,
+round to nearest even; the first layer's inputs and weights stay unrounded,
+as the kernels run that layer in FP32; its weight gradient uses BF16 inputs,
+as the kernels' backward does) and accumulates in FP64, so the kernels can
+be compared to it up to accumulation order; ``bf16=False`` is the exact FP64
+network, pinned in the tests by torch autograd (float64, CPU) and by finite
+differences.
 """
 from __future__ import annotations
 
@@ -55,10 +58,14 @@ def forward(p, x, bf=True):
     r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     fb = (lambda b: b.astype(np.float32).astype(np.float64)) if bf else (lambda b: b)
     layers = unflatten(np.asarray(p, dtype=np.float64))
-    h = r(x)
-    cache = {"h": [h], "z": []}
-    for W, b in layers[:4]:
-        z = h @ r(W).T + fb(b)
+    h = np.asarray(x, dtype=np.float64)
+    if bf:
+        h = h.astype(np.float32).astype(np.float64)
+    # the first layer's weight gradient reads BF16 inputs
+    cache = {"h": [r(x)], "z": []}
+    for l, (W, b) in enumerate(layers[:4]):
+        Wl = (W.astype(np.float32).astype(np.float64) if bf else W) if l == 0 else r(W)
+        z = h @ Wl.T + fb(b)
         h = r(silu(z))
         cache["z"].append(z)
         cache["h"].append(h)

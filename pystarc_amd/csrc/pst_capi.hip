@@ -1498,6 +1498,20 @@ static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool 
   return 0;
 }
 
+// The part of enqueue_surr after the network's forward (a grouped forward
+// of several simulators ran it): forces applied, then the audit.
+static int enqueue_surr_apply(pst_sim* s, const ForceSys& S, const ForceWork& W) {
+  pst_ctx* ctx = s->ctx;
+  CK(surr_apply(s->sp, s->mlp->m, s->pos, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
+                ctx->stream));
+  if (s->sp.audit_n > 0) {
+    CK(launch_audit_forces(s, S, W, ctx->stream));
+    CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
+                  s->sp.use_torque ? s->audit_T : nullptr, s->ctr, ctx->stream));
+  }
+  return 0;
+}
+
 // Step phases after the grid-path forces: surrogate forces and their audit,
 // the move, the Brownian bridge.
 static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par,
@@ -1903,12 +1917,28 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     b = (b + 7) / 8 * 8;
     start[l + 1] = start[l] + b;
   }
+  // the simulators whose surrogate is attached: one grouped network forward
+  // per step over all of them
+  std::vector<int> surr_l;
+  std::vector<MlpInfer> MI;
+  std::vector<int> mstart(1, 0);
+  for (int l = 0; l < L; ++l) {
+    pst_sim* sl = sims[l];
+    if (!sl->mlp || !(sl->P.surrogate_r > 0)) continue;
+    const int Sp = (sl->S + 63) / 64 * 64;
+    surr_l.push_back(l);
+    MI.push_back(mlp_infer_args(sl->mlp->m, Sp, &sl->ctr->n_mlp, sl->mlp->m.y));
+    mstart.push_back(mstart.back() + Sp / 64);
+  }
+  const int n_surr = (int)surr_l.size();
   ForceSys* dS = nullptr;
   ForceWork* dW = nullptr;
   StepParams* dP = nullptr;
   ProdBufs* dB = nullptr;
   int* dstart = nullptr;
   int* dsstart = nullptr;
+  MlpInfer* dMI = nullptr;
+  int* dmstart = nullptr;
   auto cleanup = [&]() {
     for (int l = 0; l < L; ++l) sims[l]->ctx->stream = saved[l];
     if (dS) hipFree(dS);
@@ -1917,15 +1947,26 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     if (dB) hipFree(dB);
     if (dstart) hipFree(dstart);
     if (dsstart) hipFree(dsstart);
+    if (dMI) hipFree(dMI);
+    if (dmstart) hipFree(dmstart);
   };
   if (hipMalloc(&dS, sizeof(ForceSys) * L) != hipSuccess ||
       hipMalloc(&dW, sizeof(ForceWork) * L) != hipSuccess ||
       hipMalloc(&dP, sizeof(StepParams) * L) != hipSuccess ||
       hipMalloc(&dB, sizeof(ProdBufs) * L) != hipSuccess ||
       hipMalloc(&dstart, sizeof(int) * (L + 1)) != hipSuccess ||
-      hipMalloc(&dsstart, sizeof(int) * (L + 1)) != hipSuccess) {
+      hipMalloc(&dsstart, sizeof(int) * (L + 1)) != hipSuccess ||
+      (n_surr > 0 && (hipMalloc(&dMI, sizeof(MlpInfer) * n_surr) != hipSuccess ||
+                      hipMalloc(&dmstart, sizeof(int) * (n_surr + 1)) != hipSuccess))) {
     cleanup();
     FAIL("pst_group_run: out of device memory");
+  }
+  if (n_surr > 0 &&
+      (hipMemcpy(dMI, MI.data(), sizeof(MlpInfer) * n_surr, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dmstart, mstart.data(), sizeof(int) * (n_surr + 1), hipMemcpyHostToDevice) !=
+           hipSuccess)) {
+    cleanup();
+    FAIL("pst_group_run: upload failed");
   }
   if (hipMemcpy(dS, S.data(), sizeof(ForceSys) * L, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dW, W.data(), sizeof(ForceWork) * L, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1945,9 +1986,19 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     if (e == hipSuccess)
       e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
-    for (int l = 0; l < L; ++l) {
-      const int r = enqueue_surr(sims[l], S[l], W[l]);
-      if (r) return r;
+    if (n_surr > 0) {
+      for (int l : surr_l) {
+        pst_sim* sl = sims[l];
+        e = surr_features(sl->sp, sl->mlp->m, sl->pos, sl->quat, sl->mlp_list, &sl->ctr->n_mlp,
+                          sl->S, cs);
+        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
+      }
+      e = launch_mlp_infer_group(dMI, dmstart, n_surr, mstart[n_surr], cs);
+      if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
+      for (int l : surr_l) {
+        const int r = enqueue_surr_apply(sims[l], S[l], W[l]);
+        if (r) return r;
+      }
     }
     e = prod_group_post(dP, dS, dB, dsstart, L, sstart[L], obs, par, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
@@ -2338,6 +2389,8 @@ int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out) {
   CK(al((void**)&m.b3, H * 4));
   CK(al((void**)&m.b4, 64 * 4));
   CK(al((void**)&m.x0, R * 64 * 2));
+  CK(al((void**)&m.w0f, H * kMlpIn * 4));
+  CK(al((void**)&m.x0f, R * kMlpInLd * 4));
   __bf16** hs[4] = {&m.h1, &m.h2, &m.h3, &m.h4};
   float** zs[4] = {&m.z1, &m.z2, &m.z3, &m.z4};
   __bf16** gs[4] = {&m.g1, &m.g2, &m.g3, &m.g4};
@@ -2350,6 +2403,7 @@ int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out) {
   CK(al((void**)&m.gy, R * 64 * 2));
   CK(al((void**)&m.gw, (H * 64 * 2 + H * H * 3) * 4));
   CK(hipMemsetAsync(m.x0, 0, R * 64 * 2, ctx->stream));
+  CK(hipMemsetAsync(m.x0f, 0, R * kMlpInLd * 4, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
   return 0;
 }

@@ -4,10 +4,13 @@
 // accumulate).  No reference counterpart.
 //
 // Layout: parameters arrive as one FP32 vector in [out][in] order
-// (W0 256x9, b0, W1 256x256, b1, W2, b2, W3, b3, W4 13x256, b4).  The kernels
-// use padded BF16 copies: W0p [256][64] (inputs padded 9 -> 64), W1..W3
-// [256][256], W4p [64][256] (outputs padded 13 -> 64); activations are BF16
-// [rows][256] (rows padded to 64), pre-activations FP32.
+// (W0 256x9, b0, W1 256x256, b1, W2, b2, W3, b3, W4 13x256, b4).  The first
+// layer (9 inputs) runs on the FP32 units with FP32 inputs and weights
+// (BF16 inputs would round the pose to 1 part in 256: ~0.2 A at the b-sphere),
+// the other layers on MFMA with padded BF16 copies: W1..W3 [256][256], W4p
+// [64][256] (outputs padded 13 -> 64); activations are BF16 [rows][256]
+// (rows padded to 64), pre-activations FP32.  The backward keeps a BF16
+// copy of the inputs, W0p [256][64], for the first layer's weight gradient.
 //
 // One GEMM kernel serves every product: a 256-thread workgroup owns a 64x64
 // output tile, stages 64x32 slices of A and B in LDS as [row][k] with k
@@ -31,6 +34,35 @@ __device__ inline float silu_f(float z) { return z / (1.0f + __expf(-z)); }
 __device__ inline float dsilu_f(float z) {
   const float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
+}
+
+// First layer on the FP32 units: z = sum_k x_k W0[n][k] (k = 0..8 in order,
+// fused multiply-adds) + b0[n].  Every path computes it with this function,
+// so inference and training see the same bits.
+__device__ __forceinline__ float layer0_z(const float* x, const float* w, float b) {
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) acc = fmaf(x[k], w[k], acc);
+  return acc + b;
+}
+
+// Training-path first layer: Z (train) and H = bf16(SiLU(Z)) for rows < *m_dev.
+template <bool TRAIN>
+__global__ void k_mlp_layer0(const float* x0f, const float* w0f, const float* b0, int rows,
+                             const int* m_dev, float* Z, __bf16* H) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(t / kMlpWidth), n = (int)(t % kMlpWidth);
+  const int live = m_dev ? *m_dev : rows;
+  if (row >= rows || row >= ((live + 63) & ~63)) return;
+  float x[kMlpIn], w[kMlpIn];
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) {
+    x[k] = x0f[(size_t)row * kMlpInLd + k];
+    w[k] = w0f[n * kMlpIn + k];
+  }
+  const float z = layer0_z(x, w, b0[n]);
+  if (TRAIN) Z[(size_t)row * kMlpWidth + n] = z;
+  H[(size_t)row * kMlpWidth + n] = (__bf16)silu_f(z);
 }
 
 enum { LAY_MK = 0, LAY_KM = 1 };  // A: [m][k] or [k][m];  B: [n][k] (=MK) or [k][n] (=KM)
@@ -148,12 +180,158 @@ static hipError_t gemm(const GemmArgs& g, hipStream_t st, int splits = 1) {
   return hipGetLastError();
 }
 
+// ---- fused inference forward ---------------------------------------------------------
+// All five layers of the inference forward in one kernel: a 256-thread
+// workgroup owns 64 rows and keeps their activations in LDS as BF16
+// [64][256 + 8] (one buffer: a layer's outputs wait in registers until every
+// wave has read its inputs).  Wave w computes output columns 64w..64w+63 of a
+// hidden layer as a 4 x 4 block of 16x16x32 MFMA tiles per 32-wide k step;
+// A fragments come from LDS, B fragments (weights, [out][in] with k
+// contiguous, shared by every workgroup) straight from L2 / L1 with the next
+// k step's fragments loaded while the current step's MFMAs run.  Layer 0
+// runs one k step (the 9 inputs padded to the MFMA's K of 32, not 64), the
+// output layer one n tile (13 outputs padded to 16, one m tile per wave).
+// The MFMA sequence (k order, fragments, rounding of every activation to
+// BF16 after bias + SiLU in FP32) is that of the layer GEMMs, so outputs are
+// the same bit for bit; rows past *m_dev skip their 64-row tile.
+constexpr int kActLd = kMlpWidth + 8;
+
+__device__ inline void act_store(__bf16 (*act)[kActLd], const f32x4 (&acc)[4][4],
+                                 const float (&bias)[4], int wn, int lane) {
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        act[mt * 16 + (lane >> 4) * 4 + i][wn + nt * 16 + (lane & 15)] =
+            (__bf16)silu_f(acc[mt][nt][i] + bias[nt]);
+}
+
+// Tile `tile` (rows 64 tile .. 64 tile + 63) of one network's inference.
+__device__ __forceinline__ void mlp_infer_tile(const MlpInfer& p, int tile) {
+  __shared__ __attribute__((aligned(16))) __bf16 act[64][kActLd];
+  const int tile_m = tile * 64;
+  const int n_rows = p.m_dev ? *p.m_dev : p.rows;
+  if (tile_m >= n_rows) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+  const int wn = wave * 64;
+  f32x4 acc[4][4];
+  float bias[4];
+  bf16x8 a[4], b[4], bn[4];
+  (void)n_rows;
+  // layer 0 on the FP32 units: lane l of wave w computes column 64w + l for
+  // the tile's 64 rows (features staged once in LDS, read as broadcasts)
+  __shared__ float xs[64][kMlpIn];
+  for (int i = threadIdx.x; i < 64 * kMlpIn; i += 256)
+    xs[i / kMlpIn][i % kMlpIn] = p.x0f[(size_t)(tile_m + i / kMlpIn) * kMlpInLd + i % kMlpIn];
+  {
+    const int n = wn + lane;
+    float w[kMlpIn];
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) w[k] = p.w0f[n * kMlpIn + k];
+    const float bn0 = p.b0[n];
+    __syncthreads();
+#pragma unroll 4
+    for (int row = 0; row < 64; ++row) act[row][n] = (__bf16)silu_f(layer0_z(xs[row], w, bn0));
+  }
+  __syncthreads();
+  // hidden layers 1..3: [64][256] x [256][256]^T
+  const __bf16* ws[3] = {p.w1, p.w2, p.w3};
+  const float* bs[3] = {p.b1, p.b2, p.b3};
+#pragma unroll 1
+  for (int l = 0; l < 3; ++l) {
+    const __bf16* W = ws[l];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      b[nt] = *(const bf16x8*)&W[(size_t)(wn + nt * 16 + r) * kMlpWidth + kq];
+      bias[nt] = bs[l][wn + nt * 16 + r];
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kMlpWidth / 32; ++ks) {
+      const int k0 = ks * 32;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) a[mt] = *(const bf16x8*)&act[mt * 16 + r][k0 + kq];
+      if (ks + 1 < kMlpWidth / 32) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+          bn[nt] = *(const bf16x8*)&W[(size_t)(wn + nt * 16 + r) * kMlpWidth + k0 + 32 + kq];
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+      if (ks + 1 < kMlpWidth / 32) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) b[nt] = bn[nt];
+      }
+    }
+    __syncthreads();   // every wave has read this layer's inputs
+    act_store(act, acc, bias, wn, lane);
+    __syncthreads();
+  }
+  // output layer: one 16-column n tile (13 outputs), wave w takes rows 16w..16w+15
+  f32x4 o = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < kMlpWidth / 32; ++ks) {
+    const bf16x8 av = *(const bf16x8*)&act[wave * 16 + r][ks * 32 + kq];
+    const bf16x8 bv = *(const bf16x8*)&p.w4[(size_t)r * kMlpWidth + ks * 32 + kq];
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, o, 0, 0, 0);
+  }
+  const float b4 = p.b4[lane & 15];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = tile_m + wave * 16 + (lane >> 4) * 4 + i;
+    p.y[(size_t)row * 64 + (lane & 15)] = o[i] + b4;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(MlpInfer p) { mlp_infer_tile(p, blockIdx.x); }
+
+// Several networks (one per ligand) in one launch: workgroup b runs tile
+// b - start[l] of network l for start[l] <= b < start[l + 1].
+__global__ void __launch_bounds__(256) k_mlp_infer_group(const MlpInfer* __restrict__ P,
+                                                         const int* __restrict__ start, int L) {
+  const int b = blockIdx.x;
+  int l = 0;
+  while (l + 1 < L && b >= start[l + 1]) ++l;
+  l = __builtin_amdgcn_readfirstlane(l);
+  mlp_infer_tile(P[l], b - __builtin_amdgcn_readfirstlane(start[l]));
+}
+
+MlpInfer mlp_infer_args(const MlpBufs& m, int Bp, const int* m_dev, float* y) {
+  MlpInfer p;
+  p.x0f = m.x0f;
+  p.w0f = m.w0f;
+  p.w1 = m.w1; p.w2 = m.w2; p.w3 = m.w3; p.w4 = m.w4;
+  p.b0 = m.b0; p.b1 = m.b1; p.b2 = m.b2; p.b3 = m.b3; p.b4 = m.b4;
+  p.m_dev = m_dev;
+  p.rows = Bp;
+  p.y = y;
+  return p;
+}
+
+hipError_t launch_mlp_infer_group(const MlpInfer* P, const int* start, int L, int blocks,
+                                  hipStream_t st) {
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_infer_group, dim3(blocks), dim3(256), 0, st, P, start, L);
+  return hipGetLastError();
+}
+
 // ---- small element-wise kernels --------------------------------------------------
 // FP32 parameters -> padded BF16 weights + FP32 biases.
 __global__ void k_mlp_cast(const float* p, __bf16* w0, __bf16* w1, __bf16* w2, __bf16* w3,
-                           __bf16* w4, float* b0, float* b1, float* b2, float* b3, float* b4) {
+                           __bf16* w4, float* b0, float* b1, float* b2, float* b3, float* b4,
+                           float* w0f) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int H = kMlpWidth;
+  if (t < H * kMlpIn) w0f[t] = p[kOffW0 + t];
   // W0p [256][64]
   if (t < H * 64) {
     const int o = t / 64, i = t % 64;
@@ -177,12 +355,15 @@ __global__ void k_mlp_cast(const float* p, __bf16* w0, __bf16* w1, __bf16* w2, _
   if (t < 64) b4[t] = t < kMlpOut ? p[kOffB4 + t] : 0.0f;
 }
 
-// (B,9) FP32 features -> [Bp][64] BF16, zero padded.
-__global__ void k_mlp_pack_x(const float* x, int B, int Bp, __bf16* x0) {
+// (B,9) FP32 features -> [Bp][16] FP32 (the first layer's input) and
+// [Bp][64] BF16 (the first layer's weight gradient), zero padded.
+__global__ void k_mlp_pack_x(const float* x, int B, int Bp, __bf16* x0, float* x0f) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Bp * 64) return;
   const int r = t / 64, c = t % 64;
-  x0[t] = (__bf16)((r < B && c < kMlpIn) ? x[r * kMlpIn + c] : 0.0f);
+  const float v = (r < B && c < kMlpIn) ? x[r * kMlpIn + c] : 0.0f;
+  x0[t] = (__bf16)v;
+  if (c < kMlpInLd) x0f[(size_t)r * kMlpInLd + c] = v;
 }
 
 // [Bp][64] FP32 outputs -> (B,13).
@@ -239,22 +420,43 @@ static inline unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 
 // ---- host-side orchestration ----------------------------------------------------------
 hipError_t mlp_cast(MlpBufs& m, const float* params, hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_cast, dim3(nblk(kMlpWidth * kMlpWidth)), dim3(256), 0, st, params,
-                     m.w0, m.w1, m.w2, m.w3, m.w4, m.b0, m.b1, m.b2, m.b3, m.b4);
+                     m.w0, m.w1, m.w2, m.w3, m.w4, m.b0, m.b1, m.b2, m.b3, m.b4, m.w0f);
   return hipGetLastError();
 }
 
-// Forward on padded BF16 input x0 [Bp][64]; rows >= *m_dev skipped when m_dev is set.
+// Forward on the FP32 features x0f [Bp][16] (training also reads the BF16
+// copy x0 in backward); rows >= *m_dev skipped when m_dev is set.
+// Inference runs the fused kernel; training the layer GEMMs, which keep the
+// pre-activations for the backward pass.
 hipError_t mlp_forward_packed(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
+                              hipStream_t st) {
+  if (!train) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3(Bp / 64), dim3(256), 0, st,
+                       mlp_infer_args(m, Bp, m_dev, y));
+    return hipGetLastError();
+  }
+  return mlp_forward_layers(m, Bp, m_dev, train, y, st);
+}
+
+// The same forward as five layer GEMMs (training mode keeps Z for backward).
+hipError_t mlp_forward_layers(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
                               hipStream_t st) {
   const int H = kMlpWidth;
   __bf16* hs[4] = {m.h1, m.h2, m.h3, m.h4};
   float* zs[4] = {m.z1, m.z2, m.z3, m.z4};
   const __bf16* ws[4] = {m.w0, m.w1, m.w2, m.w3};
   const float* bs[4] = {m.b0, m.b1, m.b2, m.b3};
-  const __bf16* in = m.x0;
-  int ld_in = 64;
-  hipError_t e;
-  for (int l = 0; l < 4; ++l) {
+  if (train)
+    hipLaunchKernelGGL((k_mlp_layer0<true>), dim3(nblk((size_t)Bp * H)), dim3(256), 0, st, m.x0f,
+                       m.w0f, m.b0, Bp, m_dev, m.z1, m.h1);
+  else
+    hipLaunchKernelGGL((k_mlp_layer0<false>), dim3(nblk((size_t)Bp * H)), dim3(256), 0, st, m.x0f,
+                       m.w0f, m.b0, Bp, m_dev, m.z1, m.h1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const __bf16* in = m.h1;
+  int ld_in = H;
+  for (int l = 1; l < 4; ++l) {
     GemmArgs g = {};
     g.A = in; g.lda = ld_in; g.B = ws[l]; g.ldb = ld_in;
     g.M = Bp; g.N = H; g.K = ld_in; g.m_dev = m_dev;
@@ -274,7 +476,8 @@ hipError_t mlp_forward_packed(const MlpBufs& m, int Bp, const int* m_dev, bool t
 hipError_t mlp_forward(const MlpBufs& m, int B, const float* x, float* y_out, bool train,
                        hipStream_t st) {
   const int Bp = (B + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_mlp_pack_x, dim3(nblk((size_t)Bp * 64)), dim3(256), 0, st, x, B, Bp, m.x0);
+  hipLaunchKernelGGL(k_mlp_pack_x, dim3(nblk((size_t)Bp * 64)), dim3(256), 0, st, x, B, Bp, m.x0,
+                     m.x0f);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = mlp_forward_packed(m, Bp, nullptr, train, m.y, st);

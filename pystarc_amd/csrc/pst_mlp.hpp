@@ -20,9 +20,13 @@ constexpr int kOffW4 = kOffB3 + kMlpWidth;
 constexpr int kOffB4 = kOffW4 + kMlpOut * kMlpWidth;
 constexpr int kMlpParams = kOffB4 + kMlpOut;  // 203,277
 
+constexpr int kMlpInLd = 16;   // FP32 feature row stride (9 used)
+
 struct MlpBufs {
   int max_rows;  // padded to 64
   __bf16 *w0, *w1, *w2, *w3, *w4;
+  float* w0f;                 // [256][9] FP32 first-layer weights
+  float* x0f;                 // [rows][16] FP32 features (9 used)
   float *b0, *b1, *b2, *b3, *b4;
   __bf16* x0;                 // [rows][64]
   __bf16 *h1, *h2, *h3, *h4;  // [rows][256]
@@ -33,10 +37,28 @@ struct MlpBufs {
   float* gw;                  // padded weight gradients
 };
 
+// Operands of the fused inference forward of one network.
+struct MlpInfer {
+  const float* x0f;                 // [rows][16] FP32 features
+  const float* w0f;                 // [256][9] FP32 first-layer weights
+  const __bf16 *w1, *w2, *w3, *w4;
+  const float *b0, *b1, *b2, *b3, *b4;
+  const int* m_dev;                 // live rows, or null
+  int rows;                         // rows when m_dev is null
+  float* y;                         // [rows][64] outputs (columns 0..15 written)
+};
+MlpInfer mlp_infer_args(const MlpBufs& m, int Bp, const int* m_dev, float* y);
+// L networks' inference in one launch: device descriptors P[L], workgroup
+// partition start[L + 1] (64-row tiles), blocks = start[L].
+hipError_t launch_mlp_infer_group(const MlpInfer* P, const int* start, int L, int blocks,
+                                  hipStream_t st);
+
 hipError_t mlp_cast(MlpBufs& m, const float* params, hipStream_t st);
 hipError_t mlp_forward(const MlpBufs& m, int B, const float* x, float* y_out, bool train,
                        hipStream_t st);
 hipError_t mlp_forward_packed(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
+                              hipStream_t st);
+hipError_t mlp_forward_layers(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
                               hipStream_t st);
 hipError_t mlp_backward(const MlpBufs& m, int B, const float* gy, float* gparams, hipStream_t st);
 

@@ -2,8 +2,8 @@
 // distance exceeds r_on are listed by k_prod_pre in the surrogate list; for
 // them the MLP replaces the grid + Born + far-field force evaluation.
 //
-// Features (9): centroid in the receptor frame divided by b, and the first
-// two columns of R(q) (a 6-D rotation representation).  Outputs (13):
+// Features (9, FP32): centroid in the receptor frame divided by b, and the
+// first two columns of R(q) (a 6-D rotation representation).  Outputs (13):
 // ligand force and torque, receptor force and torque, energy, each divided
 // by a per-group scale and by the envelope
 //   env(r) = exp(-(r - r0)/lam) (r0/r)^2 (1 + r/lam) / (1 + r0/lam),
@@ -25,20 +25,20 @@ __device__ inline double envelope(const SurrParams& S, double r) {
   return exp(-(r - S.r0) / S.lam) * x * x * (1.0 + r / S.lam) / (1.0 + S.r0 / S.lam);
 }
 
+// FP32 features of list entry w into row w of x0f (the first layer runs in FP32).
 __global__ void k_surr_features(SurrParams S, const double* pos, const double* quat,
-                                 const int* list, const int* n_dev, __bf16* x0) {
+                                 const int* list, const int* n_dev, float* x0f) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= *n_dev) return;
   const int s = list[w];
   double R[9];
   quat_to_rot(quat + 4 * (size_t)s, R);
-  __bf16* row = x0 + (size_t)w * 64;
-  row[0] = (__bf16)(float)(pos[3 * s] * S.inv_b);
-  row[1] = (__bf16)(float)(pos[3 * s + 1] * S.inv_b);
-  row[2] = (__bf16)(float)(pos[3 * s + 2] * S.inv_b);
-  row[3] = (__bf16)(float)R[0]; row[4] = (__bf16)(float)R[3]; row[5] = (__bf16)(float)R[6];
-  row[6] = (__bf16)(float)R[1]; row[7] = (__bf16)(float)R[4]; row[8] = (__bf16)(float)R[7];
-  for (int c = 9; c < 64; ++c) row[c] = (__bf16)0.0f;
+  float* row = x0f + (size_t)w * kMlpInLd;
+  row[0] = (float)(pos[3 * s] * S.inv_b);
+  row[1] = (float)(pos[3 * s + 1] * S.inv_b);
+  row[2] = (float)(pos[3 * s + 2] * S.inv_b);
+  row[3] = (float)R[0]; row[4] = (float)R[3]; row[5] = (float)R[6];
+  row[6] = (float)R[1]; row[7] = (float)R[4]; row[8] = (float)R[7];
 }
 
 __global__ void k_surr_apply(SurrParams S, const double* pos, const int* list, const int* n_dev,
@@ -119,19 +119,31 @@ __global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const d
   if (w == 0) atomicAdd(&C->audit_n, (unsigned long long)n);
 }
 
+hipError_t surr_features(const SurrParams& S, const MlpBufs& m, const double* pos,
+                         const double* quat, const int* list, const int* n_dev, int n_slots,
+                         hipStream_t st) {
+  const int blocks = (n_slots + 255) / 256;
+  hipLaunchKernelGGL(k_surr_features, dim3(blocks), dim3(256), 0, st, S, pos, quat, list, n_dev,
+                     m.x0f);
+  return hipGetLastError();
+}
+
+hipError_t surr_apply(const SurrParams& S, const MlpBufs& m, const double* pos, const int* list,
+                      const int* n_dev, int n_slots, double* F, double* T, hipStream_t st) {
+  const int blocks = (n_slots + 255) / 256;
+  hipLaunchKernelGGL(k_surr_apply, dim3(blocks), dim3(256), 0, st, S, pos, list, n_dev, m.y, F, T);
+  return hipGetLastError();
+}
+
 hipError_t surr_step(const SurrParams& S, const MlpBufs& m, const double* pos, const double* quat,
                      const int* list, const int* n_dev, int n_slots, double* F, double* T,
                      hipStream_t st) {
-  const int blocks = (n_slots + 255) / 256;
-  hipLaunchKernelGGL(k_surr_features, dim3(blocks), dim3(256), 0, st, S, pos, quat, list, n_dev,
-                     m.x0);
-  hipError_t e = hipGetLastError();
+  hipError_t e = surr_features(S, m, pos, quat, list, n_dev, n_slots, st);
   if (e != hipSuccess) return e;
   const int Sp = (n_slots + 63) / 64 * 64;
   e = mlp_forward_packed(m, Sp, n_dev, false, m.y, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_surr_apply, dim3(blocks), dim3(256), 0, st, S, pos, list, n_dev, m.y, F, T);
-  return hipGetLastError();
+  return surr_apply(S, m, pos, list, n_dev, n_slots, F, T, st);
 }
 
 hipError_t surr_audit_pick(const int* list, const int* n_dev, int cap, const RunCounters* C,

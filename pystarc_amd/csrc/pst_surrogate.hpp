@@ -18,6 +18,13 @@ struct SurrParams {
   int audit_n;      // surrogate trajectories re-evaluated on the grid path per step
 };
 
+// The step's surrogate forces in three parts (features, then the network's
+// forward, then forces / torques applied to the surrogate list's slots).
+hipError_t surr_features(const SurrParams& S, const MlpBufs& m, const double* pos,
+                         const double* quat, const int* list, const int* n_dev, int n_slots,
+                         hipStream_t st);
+hipError_t surr_apply(const SurrParams& S, const MlpBufs& m, const double* pos, const int* list,
+                      const int* n_dev, int n_slots, double* F, double* T, hipStream_t st);
 hipError_t surr_step(const SurrParams& S, const MlpBufs& m, const double* pos, const double* quat,
                      const int* list, const int* n_dev, int n_slots, double* F, double* T,
                      hipStream_t st);

@@ -62,3 +62,23 @@ def test_group_run_equals_separate_runs(graph_steps):
         assert np.array_equal(a.sim_data["outcome"], b.sim_data["outcome"])
         assert np.array_equal(a.sim_data["n_steps"], b.sim_data["n_steps"])
     assert sum(r.n_reacted for r in grp) > 0
+
+
+def test_group_with_surrogates_equals_separate_runs():
+    """Each ligand with its own surrogate: the group step runs every network
+    in one grouped forward launch, and every ligand's trajectories are those
+    of its separate run with the same network."""
+    from pystarc_amd.surrogate import build_surrogate
+    ls, engines = _small(n_traj=400)
+    surr = [build_surrogate(e, s, n_train=100_000, epochs=2, batch=2048, shell=2.0,
+                            max_rows=2048, seed=k)
+            for k, (s, e) in enumerate(zip(ls, engines))]
+    kw = lambda k: dict(surrogate=surr[k], audit_tol=1e9)
+    sep = [_sim(s, e, **kw(k)).run() for k, (s, e) in enumerate(zip(ls, engines))]
+    grp = GPUBatchGroup([_sim(s, e, **kw(k)) for k, (s, e) in enumerate(zip(ls, engines))],
+                        graph_steps=8).run()
+    for a, b in zip(sep, grp):
+        assert a.sim_data["surrogate_audit"]["n"] > 0
+        assert (a.n_reacted, a.n_escaped, a.total_steps) == (b.n_reacted, b.n_escaped, b.total_steps)
+        assert np.array_equal(a.sim_data["outcome"], b.sim_data["outcome"])
+        assert np.array_equal(a.sim_data["n_steps"], b.sim_data["n_steps"])

@@ -111,3 +111,16 @@ def test_training_fits_a_smooth_map():
     hist = s.train(Xt, Yt, epochs=6, batch=2048, lr=2e-3)
     assert hist[-1] < 0.05 * float(np.var(Y))
     s.close()
+
+
+@pytest.mark.parametrize("B", [1, 63, 64, 1000, 4096])
+def test_fused_inference_equals_layer_gemms(mlp, B):
+    """The fused inference kernel (all layers per 64-row tile, activations in
+    LDS) runs the layer GEMMs' MFMA sequence and rounding: bit-identical
+    outputs to the train-mode forward's five layer launches."""
+    ctx, h = mlp
+    p = init(4)
+    x = np.random.default_rng(10 + B).normal(size=(B, 9))
+    y_fused, _ = run_fwd(ctx, h, p, x, train=False)
+    y_layers, _ = run_fwd(ctx, h, p, x, train=True)
+    assert np.array_equal(y_fused, y_layers)
