@@ -988,7 +988,7 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
 int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, double* out4) {
   if (!ctx) return 1;
   if (!ctx->pair.mem) FAIL("pst_gather_ceiling: needs the FP32 potential + Born pair grid");
-  if (mode != 0 && mode != 1) FAIL("pst_gather_ceiling: mode must be 0 or 1");
+  if (mode < 0 || mode > 16) FAIL("pst_gather_ceiling: mode must be 0..16");
   if (n_points <= 0 || n_points > (1ll << 30) || reps <= 0 || !out4)
     FAIL("pst_gather_ceiling: bad arguments");
   CK(set_dev(ctx));
@@ -2402,6 +2402,7 @@ int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out) {
   CK(al((void**)&m.y, R * 64 * 4));
   CK(al((void**)&m.gy, R * 64 * 2));
   CK(al((void**)&m.gw, (H * 64 * 2 + H * H * 3) * 4));
+  CK(al((void**)&m.gcol, (R + 255) / 256 * 256 * 4));
   CK(hipMemsetAsync(m.x0, 0, R * 64 * 2, ctx->stream));
   CK(hipMemsetAsync(m.x0f, 0, R * kMlpInLd * 4, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));

@@ -381,19 +381,36 @@ __global__ void k_mlp_pack_gy(const float* gy, int B, int Bp, __bf16* g) {
   g[t] = (__bf16)((r < B && c < kMlpOut) ? gy[r * kMlpOut + c] : 0.0f);
 }
 
-// Bias gradient: column sums of a BF16 [Bp][ld] gradient into grad[off + n].
-__global__ void k_colsum(const __bf16* g, int rows, int ld, int n_cols, float* out) {
-  const int n = blockIdx.x;
+// Bias gradient: column sums of a BF16 [Bp][ld] gradient into out[n], in two
+// passes with a fixed order (no atomics): block b sums rows 256b..256b+255 of
+// every column (thread n reads column n, so a wave reads whole row segments),
+// then one block adds the blocks' partial sums in order.
+constexpr int kColRows = 256;
+__global__ void k_colsum_part(const __bf16* g, int rows, int ld, float* part) {
+  const int n = threadIdx.x;
+  if (n >= ld) return;
+  const int r0 = blockIdx.x * kColRows, r1 = min(rows, r0 + kColRows);
   float s = 0.f;
-  for (int r = threadIdx.x; r < rows; r += blockDim.x) s += (float)g[(size_t)r * ld + n];
-  __shared__ float red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && n < n_cols) out[n] = red[0];
+  for (int r = r0; r < r1; ++r) s += (float)g[(size_t)r * ld + n];
+  part[(size_t)blockIdx.x * 256 + n] = s;
+}
+
+__global__ void k_colsum_final(const float* part, int n_part, int n_cols, float* out) {
+  const int n = threadIdx.x;
+  if (n >= n_cols) return;
+  float s = 0.f;
+  for (int b = 0; b < n_part; ++b) s += part[(size_t)b * 256 + n];
+  out[n] = s;
+}
+
+static hipError_t colsum(const __bf16* g, int rows, int ld, int n_cols, float* part, float* out,
+                         hipStream_t st) {
+  const int n_part = (rows + kColRows - 1) / kColRows;
+  hipLaunchKernelGGL(k_colsum_part, dim3(n_part), dim3(256), 0, st, g, rows, ld, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, st, part, n_part, n_cols, out);
+  return hipGetLastError();
 }
 
 // Padded weight gradients -> the flat FP32 [out][in] layout.
@@ -509,7 +526,7 @@ hipError_t mlp_backward(const MlpBufs& m, int B, const float* gy, float* gparams
     g.A = m.gy; g.lda = 64; g.B = m.h4; g.ldb = H; g.M = 64; g.N = H; g.K = Bp;
     g.Y = gw4; g.ldy = H;
     if ((e = gemm<LAY_KM, LAY_KM, EPI_WGRAD>(g, st, splits)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, st, m.gy, Bp, 64, kMlpOut, gparams + kOffB4);
+    if ((e = colsum(m.gy, Bp, 64, kMlpOut, m.gcol, gparams + kOffB4, st)) != hipSuccess) return e;
     GemmArgs d = {};
     d.A = m.gy; d.lda = 64; d.B = m.w4; d.ldb = H; d.M = Bp; d.N = H; d.K = 64;
     d.Z = m.z4; d.ldz = H; d.H = m.g4; d.ldh = H;
@@ -527,7 +544,7 @@ hipError_t mlp_backward(const MlpBufs& m, int B, const float* gy, float* gparams
     g.A = gz[l]; g.lda = H; g.B = hin[l]; g.ldb = nin; g.M = H; g.N = nin; g.K = Bp;
     g.Y = gws[l]; g.ldy = nin;
     if ((e = gemm<LAY_KM, LAY_KM, EPI_WGRAD>(g, st, splits)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_colsum, dim3(H), dim3(256), 0, st, gz[l], Bp, H, H, gparams + boff[l]);
+    if ((e = colsum(gz[l], Bp, H, H, m.gcol, gparams + boff[l], st)) != hipSuccess) return e;
     if (l > 0) {
       GemmArgs d = {};
       d.A = gz[l]; d.lda = H; d.B = ws[l]; d.ldb = H; d.M = Bp; d.N = H; d.K = H;

@@ -35,6 +35,7 @@ struct MlpBufs {
   __bf16* gy;                 // [rows][64]
   __bf16 *g1, *g2, *g3, *g4;  // [rows][256]
   float* gw;                  // padded weight gradients
+  float* gcol;                // [rows / 256][256] partial column sums (bias gradients)
 };
 
 // Operands of the fused inference forward of one network.

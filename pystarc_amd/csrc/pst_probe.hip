@@ -5,9 +5,10 @@
 // kernel does the force kernel's gathers with none of its arithmetic:
 //   mode 0: points uniform over the grid's valid box (every gather far from
 //           the last: the line traffic is served by the Infinity Cache / HBM);
-//   mode 1: the points of one workgroup inside a 12 A cube at a random place
-//           (the gathers of a ligand's atoms: neighbouring stencils share
-//           bricks and lines in L2).
+//   mode k >= 1: the points of one workgroup inside a cube of edge 12 / k A
+//           at a random place (mode 1: the gathers of a ligand's atoms,
+//           neighbouring stencils sharing bricks and lines in L2; higher k:
+//           tighter clusters, the locality a better ordering could reach).
 // Its rate is the ceiling a gather-bound force kernel can reach on the same
 // grids.
 #include <hip/hip_runtime.h>
@@ -37,7 +38,8 @@ __global__ void __launch_bounds__(256) k_gather_probe(GridDev g, long long n, in
       // the workgroup's cluster (one per 256 consecutive points)
       const long long c = p >> 8;
       for (int d = 0; d < 3; ++d) {
-        const double half = 6.0 < 0.5 * ext[d] ? 6.0 : 0.5 * ext[d];
+        const double h0 = 6.0 / mode;
+        const double half = h0 < 0.5 * ext[d] ? h0 : 0.5 * ext[d];
         const double cen = g.lo[d] + half + (ext[d] - 2.0 * half) *
                                                 hash_unit(seed ^ (0x5555ull + 3 * (unsigned long long)c + d));
         x[d] = cen + half * (2.0 * hash_unit(seed ^ (unsigned long long)(3 * p + d + 0x777)) - 1.0);

@@ -192,7 +192,8 @@ class Surrogate:
         t0 = time.time()
         for ep in range(epochs):
             perm = torch.randperm(n, device=self.device)
-            tot = 0.0
+            # summed on the device: no host synchronisation inside an epoch
+            tot = torch.zeros((), dtype=torch.float32, device=self.device)
             for i in range(0, n - batch + 1, batch):
                 idx = perm[i:i + batch]
                 xb, yb = X[idx].contiguous(), Y[idx].contiguous()
@@ -206,9 +207,9 @@ class Surrogate:
                     gparam["lr"] = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * k / steps))
                 opt.step()
                 self._load()
-                tot += float((diff * diff).mean())
+                tot += (diff * diff).mean()
                 k += 1
-            loss = tot / max(1, n // batch)
+            loss = float(tot) / max(1, n // batch)
             self.history.append(loss)
             if log:
                 log(f"surrogate epoch {ep}: mse {loss:.3e} ({time.time() - t0:.1f}s)")

@@ -65,3 +65,25 @@ def test_surrogate_k_on_and_gain(pp, trained):
     lo, hi = wilson_ci(plain.n_reacted, plain.n_escaped, 0.95)
     assert lo <= surr.reaction_probability <= hi
     assert surr.steps_per_sec > plain.steps_per_sec
+
+
+def test_surrogate_torque_audited_in_physical_mode():
+    """With the physical torque the surrogate applies its predicted torque;
+    the in-step audit then compares torques too (pst_prod_audit), and both
+    the validation and the audited torque errors are stated."""
+    from tests._systems import dx_grids, numpy_grids, small_protein_ligand
+    sysm = small_protein_ligand(n_traj=4000)
+    g = dx_grids(numpy_grids(sysm))
+    e64 = S.make_engine(sysm, g, torque_mode="physical")
+    s = build_surrogate(e64, sysm, n_train=400_000, epochs=10, batch=4096, max_rows=4096,
+                        shell=4.0)
+    assert s.use_torque and s.val_torque_error is not None
+    e32 = S.make_engine(sysm, g, precision="fp32", torque_mode="physical")
+    res = GPUBatchSimulator(sysm.mol1, sysm.mol2, sysm.mob, sysm.pathway_set, sysm.params, e32,
+                            surrogate=s, n_slots=4000).run()
+    a = res.sim_data["surrogate_audit"]
+    print(f"physical torque: validation force {s.val_error:.3e}, torque {s.val_torque_error:.3e}; "
+          f"audit force {a['rel_rms']:.3e}, torque {a['torque_rel_rms']:.3e}", flush=True)
+    assert a["n"] > 0 and a["torque_rel_rms"] is not None
+    assert a["fallback_step"] is None
+    assert a["torque_rel_rms"] < AUDIT_TOL and s.val_torque_error < AUDIT_TOL
