@@ -283,13 +283,16 @@ def cpu_baseline(sysm, grids, pos, quat, hydro, args, seconds=15.0, per_worker=3
 
 def gather_ceiling(eng, n_points=1 << 24, reps=10):
     """Gather ceiling of the receptor grids (pst_gather_ceiling): uniform
-    and clustered points, algorithmic GB/s of the 160-byte stencil."""
-    out = {}
-    for mode, name in ((0, "uniform"), (1, "clustered")):
+    and clustered points, algorithmic GB/s of the 160-byte stencil, and the
+    locality sweep (each workgroup's points inside a cube of edge 12/mode A)."""
+    def probe(mode):
         o4 = np.zeros(4)
         eng.ctx.check(eng.ctx.lib.pst_gather_ceiling(eng.ctx.h, mode, n_points, reps, ptr_np(o4)),
                       "pst_gather_ceiling")
-        out[name] = {"ms": float(o4[0]), "gbs": float(o4[1]), "points": int(o4[2])}
+        return {"ms": float(o4[0]), "gbs": float(o4[1]), "points": int(o4[2])}
+    out = {"uniform": probe(0), "clustered": probe(1)}
+    out["locality_sweep"] = [dict(cube_edge_A=12.0 / m, **(out["clustered"] if m == 1 else probe(m)))
+                             for m in (1, 2, 3, 4, 6, 12)]
     return out
 
 
