@@ -149,7 +149,9 @@ def run_window(sysm, eng, args, surrogate=None):
             "traj_steps": traj_steps, "value": traj_steps / wall if wall > 0 else 0.0,
             "pos": pos[live], "quat": q[live], "counters": c16.tolist(), "audit_rel_rms": audit,
             "n_surrogate_last_step": int(c16[11]),
-            "launch": tune["best"] if tune else None, "roof_force": roof_force}
+            "launch": tune["best"] if tune else None, "roof_force": roof_force,
+            "tune_fastest": sorted((r for r in tune["table"] if r["ms"] > 0),
+                                   key=lambda r: r["ms"])[:8] if tune else None}
 
 
 def mlp_roofline(surr, pos, quat, reps=20):
@@ -526,7 +528,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--slots", type=int, default=1 << 16)
+    ap.add_argument("--slots", type=int, default=1 << 18,
+                    help="trajectories in flight (throughput saturates from ~2^18 on the "
+                         "protein pair)")
     ap.add_argument("--trajectories", type=int, default=10_000_000)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--paths", default="fp64,fp32,surrogate")
@@ -650,14 +654,18 @@ def main():
         rm = best["roof_mlp"]
         roof = {"bound": "mfma", "achieved": rm["tflops"], "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": rm["tflops"] / BF16_PEAK_TFLOPS, "traffic": None,
-                "kernel": "k_gemm (MLP forward, 5 layers)", "ms_per_launch": rm["ms"],
+                "kernel": "k_mlp_infer (fused 5-layer MLP forward)", "ms_per_launch": rm["ms"],
                 "rows": rm["n"]}
     else:
         rf = best["roof_force"]
         lc = best.get("launch")
         tc = "double" if head == "fp64" else "float"
-        zp = "true" if lc and lc.get("mode") == "zpair" else "false"
-        kname = f"k_forces<{tc}, {lc['G']}, true, {zp}>" if lc else "k_forces"
+        zp = "true" if lc and lc.get("mode") in ("zpair", "split_zpair") else "false"
+        if lc and lc.get("mode", "").startswith("split"):
+            kname = (f"k_rev_sort + k_forces<{tc}, {lc['G']}, true, {zp}, 1> + "
+                     f"k_forces<{tc}, {lc['G']}, true, {zp}, 2>")
+        else:
+            kname = f"k_forces<{tc}, {lc['G']}, true, {zp}>" if lc else "k_forces"
         roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS,
                 "traffic": measured_traffic(args.workload, head),
@@ -701,7 +709,8 @@ def main():
                       "audit_rel_rms": v["audit_rel_rms"],
                       "force_ms": v["roof_force"]["ms"] if v.get("roof_force") else None,
                       "force_gbs": v["roof_force"]["achieved_gbs"] if v.get("roof_force") else None,
-                      "mlp": v.get("roof_mlp"), "launch": v.get("launch")}
+                      "mlp": v.get("roof_mlp"), "launch": v.get("launch"),
+                      "tune_fastest": v.get("tune_fastest")}
                   for k, v in results.items()},
     }
     print(json.dumps(out), flush=True)

@@ -82,12 +82,17 @@ int pst_set_force_group(pst_ctx* ctx, int lanes);
  * direct kernel (one trajectory per lane group, gathers from the grids), 1 =
  * brick-binned (gather requests sorted by 8^3-cell brick, each brick staged
  * once in LDS; same results bit for bit; needs the float2 receptor pair grid,
- * else the direct kernel runs).  Mode 1 sizes its buffers for max_traj
- * trajectories (~32 B x (ligand + receptor atoms) each).  No reference
- * counterpart (the reference has one kernel, gpu_batch_engine.py:65-123). */
+ * else the direct kernel runs), 2 = the direct kernel with z-pair fetches
+ * (FP32), 3 / 4 = split (production steps: the ligand-atom terms over the
+ * grid-path list, then the reverse Born over the list re-sorted by the
+ * ligand-frame direction of the receptor; same results bit for bit; 4 with
+ * z-pair fetches; elsewhere modes 0 / 2 run).  Mode 1 sizes its buffers for
+ * max_traj trajectories (~32 B x (ligand + receptor atoms) each).  No
+ * reference counterpart (the reference has one kernel,
+ * gpu_batch_engine.py:65-123). */
 int pst_set_gather_mode(pst_ctx* ctx, int mode, int max_traj);
-/* What the context's last force evaluation launched: mode 0 direct, 1
- * binned (-1: none yet), and its lanes per trajectory. */
+/* What the context's last force evaluation launched: its mode (0 to 4 as
+ * above, -1: none yet) and its lanes per trajectory. */
 int pst_last_force_eval(const pst_ctx* ctx, int* mode, int* G);
 
 /* ---- force operator ------------------------------------------------------
@@ -253,13 +258,14 @@ int pst_prod_counters(pst_sim* sim, double* c16);
  * events; the 4 fastest then run 4 more interleaved passes and the fastest of
  * that runoff becomes the launch of later steps.
  * table (max_rows,5) = {mode, G, cap, xcd, mean ms of the round-robin passes}
- * per candidate (mode 0 direct, 1 binned, 2 z-pair);
+ * per candidate (mode 0 direct, 1 binned, 2 z-pair, 3 split, 4 split with
+ * z-pair fetches);
  * best5 = {mode, G, cap, xcd, steps run}.
  * Both may be NULL. */
 int pst_prod_autotune(pst_sim* sim, int rounds, int steps_per_cand, double* table, int max_rows,
                       int* n_rows, int* best5);
 /* The same launch configuration set by hand (mode -1: the context's gather
- * mode, 0 direct, 1 binned; G = 0: heuristic). */
+ * mode, else 0 to 4 as pst_set_gather_mode; G = 0: heuristic). */
 int pst_prod_set_launch(pst_sim* sim, int mode, int G, int blocks_cap, int xcd);
 /* Times the force kernel inside the production step (no reference
  * counterpart): `steps` real steps with plain launches, each force launch

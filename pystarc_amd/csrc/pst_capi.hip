@@ -97,6 +97,9 @@ struct pst_sim {
   unsigned int* n_steps32 = nullptr;
   long long* traj = nullptr;
   int* plain_list = nullptr;
+  int* rev_list = nullptr;           // split evaluation: reverse-Born order,
+  double* F_rev = nullptr;           // and the reverse-Born corrections
+  double* T_rev = nullptr;
   int* mlp_list = nullptr;
   int* key = nullptr;
   int* bins = nullptr;
@@ -661,11 +664,30 @@ static bool bins_match(pst_ctx* ctx, const ForceSys& S) {
   return true;
 }
 
+// Whether the split evaluation (modes 3 / 4) applies: pose mode over a
+// device-counted list with per-trajectory outputs, a buffer for the reverse
+// order, the exact cull, G of a part kernel.
+static bool split_eligible(const ForceSys& S, const ForceWork& W, int G) {
+  return S.has_rev && W.rev_order && W.F_rev && W.list && W.n_dev && W.quat && !W.lig_pos &&
+         !W.out_compact && W.rev_skip_mode == 0 && (G == 16 || G == 32 || G == 64);
+}
+
 // One force evaluation: the brick-binned pipeline when asked for and
-// eligible (same results as the direct kernel with the same G), else k_forces.
+// eligible (same results as the direct kernel with the same G); the split
+// evaluation (the ligand-atom terms over the list and the reverse Born over
+// the list re-sorted by the ligand-frame direction of the receptor, in one
+// launch; the same results as the direct kernel with the same G); else
+// k_forces.  Modes 2 and 4 use z-pair fetches (FP32).
 static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const ForceWork& W,
                                     int precision, int G, int max_n, int cap, int mode) {
   ctx->last_G = G;
+  if ((mode == 3 || mode == 4) && split_eligible(S, W, G)) {
+    ctx->last_mode = mode;
+    return launch_forces_split(S, W, precision, G, max_n, cap, ctx->stream,
+                               mode == 4 && precision == 1);
+  }
+  if (mode == 3) mode = 0;
+  if (mode == 4) mode = 2;
   if (mode == 1 && G >= 8 && bins_match(ctx, S) && binned_eligible(S, W, ctx->bins, max_n)) {
     ctx->last_mode = 1;
     return launch_forces_binned(S, W, ctx->bins, precision, G, max_n, cap, ctx->stream);
@@ -720,7 +742,7 @@ int pst_last_force_eval(const pst_ctx* ctx, int* mode, int* G) {
 
 int pst_set_gather_mode(pst_ctx* ctx, int mode, int max_traj) {
   if (!ctx) return 1;
-  if (mode < 0 || mode > 2) FAIL("pst_set_gather_mode: mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 4) FAIL("pst_set_gather_mode: mode must be 0 to 4");
   if (mode == 1 && max_traj <= 0) FAIL("pst_set_gather_mode: max_traj must be positive");
   CK(set_dev(ctx));
   ctx->gather_mode = mode;
@@ -1105,7 +1127,8 @@ int pst_sim_destroy(pst_sim* s) {
                   s->n_steps64, s->flags, s->lists, s->counts, s->h_buf, s->n_steps32,
                   s->traj, s->plain_list, s->mlp_list, s->ctr, s->out_status, s->out_steps,
                   s->audit_F, s->audit_T, s->audit_list, s->audit_count, s->key, s->bins,
-                  s->census, s->sm_state, s->sm_mask, s->lmz_escaped};
+                  s->census, s->sm_state, s->sm_mask, s->lmz_escaped, s->rev_list,
+                  s->F_rev, s->T_rev};
   if (s->g_exec) hipGraphExecDestroy(s->g_exec);
   if (s->g_graph) hipGraphDestroy(s->g_graph);
   if (s->g_cap) hipStreamDestroy(s->g_cap);
@@ -1358,6 +1381,9 @@ int pst_prod_init(pst_sim* s, int record) {
     CK(hipMalloc(&s->n_steps32, S * 4));
     CK(hipMalloc(&s->traj, S * 8));
     CK(hipMalloc(&s->plain_list, S * 4));
+    CK(hipMalloc(&s->rev_list, S * 4));
+    CK(hipMalloc(&s->F_rev, S * 3 * 8));
+    CK(hipMalloc(&s->T_rev, S * 3 * 8));
     CK(hipMalloc(&s->mlp_list, S * 4));
     CK(hipMalloc(&s->ctr, sizeof(RunCounters)));
     CK(hipMalloc(&s->key, S * 4));
@@ -1418,6 +1444,9 @@ static ForceWork main_work(pst_sim* s) {
   ForceWork W;
   memset(&W, 0, sizeof(W));
   W.list = s->plain_list;
+  W.rev_order = s->rev_list;
+  W.F_rev = s->F_rev;
+  W.T_rev = s->T_rev;
   W.n_dev = &s->ctr->n_sr;
   W.xcd_chunks = s->tune_xcd;
   W.pos = s->pos;
@@ -1569,7 +1598,7 @@ static void graph_key(pst_sim* s, int gs, std::vector<unsigned char>& key) {
                         s->obs.s_min, s->obs.s_step, s->obs.s_time, s->obs.s_ret,
                         s->obs.s_prev_r, s->obs.s_nm, s->aux, ctx->stream,
                         ctx->bins.hist, ctx->bins.items, ctx->bins.sorted, ctx->bins.out,
-                        ctx->bins.out_rev, s->tev0, s->tev1};
+                        ctx->bins.out_rev, s->tev0, s->tev1, s->rev_list, s->F_rev, s->T_rev};
   put(ptrs, sizeof(ptrs));
   if (s->mlp) {
     put(&s->sp, sizeof(SurrParams));
@@ -1716,6 +1745,14 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
     for (int g : Gs)
       for (int c : {2048, 8192})
         for (int x = 0; x < 2; ++x) cands.push_back({2, g, c, x, 0.0, 0, 0.0, 0});
+  // split evaluation (reverse-Born pass in its own order), XCD-chunked
+  if (S.has_rev)
+    for (int g : Gs)
+      if (g >= 16)
+        for (int c : {2048, 8192}) {
+          cands.push_back({3, g, c, 1, 0.0, 0, 0.0, 0});
+          if (s->precision == 1) cands.push_back({4, g, c, 1, 0.0, 0, 0.0, 0});
+        }
   CK(hipEventCreate(&s->tev0));
   CK(hipEventCreate(&s->tev1));
   RunCounters h;
@@ -1847,7 +1884,7 @@ int pst_prod_time_forces(pst_sim* s, int steps, double* out6) {
 int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   if (!s) return 1;
   pst_ctx* ctx = s->ctx;
-  if (mode < -1 || mode > 2) FAIL("pst_prod_set_launch: mode must be -1, 0, 1 or 2");
+  if (mode < -1 || mode > 4) FAIL("pst_prod_set_launch: mode must be -1 to 4");
   if (!(G == 0 || G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64))
     FAIL("pst_prod_set_launch: G must be 0 or a power of two up to 64");
   if (blocks_cap < 1) FAIL("pst_prod_set_launch: blocks_cap must be positive");

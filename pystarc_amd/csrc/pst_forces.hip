@@ -89,6 +89,188 @@ __device__ inline bool atom_reverse(const ForceSys& S, double x, double y, doubl
 }
 
 
+// The ligand body staged in LDS by every workgroup of a pose-mode launch
+// (x, y, z, charge per atom) when it fits; returns whether it was staged.
+template <typename TC>
+__device__ __forceinline__ bool stage_ligand(const ForceSys& S, typename Vec4<TC>::type* s_lig) {
+  typedef typename Vec4<TC>::type V4;
+  if (S.M > kLdsAtomsMax) return false;
+  for (int i = threadIdx.x; i < S.M; i += blockDim.x) {
+    V4 v;
+    v.x = (TC)S.lig_body[3 * i]; v.y = (TC)S.lig_body[3 * i + 1];
+    v.z = (TC)S.lig_body[3 * i + 2]; v.w = (TC)S.lig_q[i];
+    s_lig[i] = v;
+  }
+  __syncthreads();
+  return true;
+}
+
+// One trajectory (list entry w) by the G lanes of a group.
+// PART 0 evaluates everything.  PART 1 only the terms on the ligand atoms
+// (receptor grids, Yukawa, energy), writing F, T and E as PART 0 does before
+// its reverse-Born correction.  PART 2 only the reverse Born, writing the
+// correction (R f_rec, and R tau for the physical torque) to F_rev / T_rev;
+// k_rev_apply then forms F - R f_rec and T - R tau.  With the same G the
+// parts perform PART 0's operations in the same order, so the results are
+// bit-identical to it.
+template <typename TC, int G, bool POSE, bool ZP, int PART>
+__device__ __forceinline__ void force_item(const ForceSys& S, const ForceWork& W, int w,
+                                           bool valid, int lane,
+                                           const typename Vec4<TC>::type* s_lig, bool lds) {
+  typedef typename Vec4<TC>::type V4;
+  const int M = S.M;
+  const double* lq = W.lig_q ? W.lig_q : S.lig_q;
+  const int t = valid ? (W.list ? W.list[w] : w) : 0;
+  double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double cx = 0, cy = 0, cz = 0;
+  bool have_pose = false;
+  if (valid) {
+    if (W.pos) { cx = W.pos[3 * t]; cy = W.pos[3 * t + 1]; cz = W.pos[3 * t + 2]; have_pose = true; }
+    if (W.quat) {
+      quat_to_rot(W.quat + 4 * (size_t)t, R);
+    } else if (W.rot) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) R[k] = W.rot[9 * (size_t)t + k];
+    } else {
+      have_pose = false;
+    }
+  }
+  TC fx = 0, fy = 0, fz = 0, e = 0;
+  TC tx = 0, ty = 0, tz = 0;
+  // lever point for the physical torque: centroid of charged atoms
+  double qx = 0, qy = 0, qz = 0;
+  if (PART != 2 && S.torque_mode == 1) {
+    if (POSE) {
+      qx = cx + R[0] * S.cq_body[0] + R[1] * S.cq_body[1] + R[2] * S.cq_body[2];
+      qy = cy + R[3] * S.cq_body[0] + R[4] * S.cq_body[1] + R[5] * S.cq_body[2];
+      qz = cz + R[6] * S.cq_body[0] + R[7] * S.cq_body[1] + R[8] * S.cq_body[2];
+    } else {
+      double sx = 0, sy = 0, sz = 0, cnt = 0;
+      if (valid) {
+        const double* lp = W.lig_pos + (size_t)t * M * 3;
+        for (int i = lane; i < M; i += G)
+          if (fabs(lq[i]) > 1e-9) { sx += lp[3 * i]; sy += lp[3 * i + 1]; sz += lp[3 * i + 2]; cnt += 1; }
+      }
+      sx = group_sum<G>(sx); sy = group_sum<G>(sy); sz = group_sum<G>(sz); cnt = group_sum<G>(cnt);
+      if (cnt > 0) { qx = sx / cnt; qy = sy / cnt; qz = sz / cnt; }
+    }
+  }
+  // no ligand atom can reach a receptor grid box: skip the per-atom box tests
+  const bool grids = !POSE || norm3(cx, cy, cz) - S.lig_rmax <= S.grid_reach;
+  if (PART != 2 && valid) {
+    for (int i = lane; i < M; i += G) {
+      double px, py, pz, q;
+      if (POSE) {
+        double bx, by, bz;
+        if (lds) {
+          const V4 v = s_lig[i];
+          bx = v.x; by = v.y; bz = v.z; q = S.lig_q[i];
+        } else {
+          bx = S.lig_body[3 * i]; by = S.lig_body[3 * i + 1]; bz = S.lig_body[3 * i + 2]; q = S.lig_q[i];
+        }
+        // lab = R b + c (gpu_batch_simulator.py:935-937)
+        px = R[0] * bx + R[1] * by + R[2] * bz + cx;
+        py = R[3] * bx + R[4] * by + R[5] * bz + cy;
+        pz = R[6] * bx + R[7] * by + R[8] * bz + cz;
+      } else {
+        const double* lp = W.lig_pos + ((size_t)t * M + i) * 3;
+        px = lp[0]; py = lp[1]; pz = lp[2]; q = lq[i];
+      }
+      TC ax = 0, ay = 0, az = 0, ae = 0;
+      atom_direct<TC, ZP>(S, px, py, pz, q, grids, ax, ay, az, ae);
+      fx += ax; fy += ay; fz += az; e += ae;
+      if (S.torque_mode == 1) {
+        const TC lx = (TC)(px - qx), ly = (TC)(py - qy), lz = (TC)(pz - qz);
+        tx += ly * az - lz * ay; ty += lz * ax - lx * az; tz += lx * ay - ly * ax;
+      }
+    }
+  }
+  // reverse Born: ligand Born grid at receptor atoms (:583-591, :607-657)
+  TC rx = 0, ry = 0, rz = 0, rtx = 0, rty = 0, rtz = 0;
+  if (PART != 1 && S.has_rev && have_pose && S.alpha > 1e-12) {
+    bool skip;
+    const double rc = norm3(cx, cy, cz);
+    if (W.rev_skip_mode == 1 && W.chunk_min_r) {
+      const int fbi = w / W.chunk_fb;
+      const int ci = fbi * W.chunks_per_fb + (w - fbi * W.chunk_fb) / W.chunk_cs;
+      skip = W.chunk_min_r[ci] - S.rec_max_r > S.rev_radius;
+    } else {
+      skip = rc - S.rec_max_r > S.rev_cull;
+    }
+    if (!skip) {
+      // runs of receptor atoms farther than rev_cull from the centre have no
+      // atom in the ligand grid box; lane `lane` still visits its atoms
+      // (j = lane mod G) in increasing order
+      for (int k = 0; k < S.n_rec_chunk; ++k) {
+        if (!rec_chunk_near(S, k, cx, cy, cz, S.rev_cull)) continue;
+        const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
+        for (int j = k * PST_REC_CHUNK + lane; j < j1; j += G) {
+          const double dx = S.rec_pos[3 * j] - cx, dy = S.rec_pos[3 * j + 1] - cy,
+                       dz = S.rec_pos[3 * j + 2] - cz;
+          // x = R^T (r - c) (:641-644)
+          const double x = R[0] * dx + R[3] * dy + R[6] * dz;
+          const double y = R[1] * dx + R[4] * dy + R[7] * dz;
+          const double z = R[2] * dx + R[5] * dy + R[8] * dz;
+          TC ax, ay, az;
+          if (atom_reverse<TC, ZP>(S, x, y, z, S.rec_q[j], ax, ay, az)) {
+            rx += ax; ry += ay; rz += az;
+            if (S.torque_mode == 1) {
+              const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
+                       lz = (TC)(z - S.cq_body[2]);
+              rtx += ly * az - lz * ay; rty += lz * ax - lx * az; rtz += lx * ay - ly * ax;
+            }
+          }
+        }
+      }
+    }
+  }
+  const size_t o = W.out_compact ? (size_t)w : (size_t)t;
+  double Fx = 0, Fy = 0, Fz = 0, Ee = 0, Tx = 0, Ty = 0, Tz = 0;
+  if (PART != 2) {
+    Fx = group_sum<G>((double)fx); Fy = group_sum<G>((double)fy); Fz = group_sum<G>((double)fz);
+    Ee = group_sum<G>((double)e);
+    if (S.torque_mode == 1) {
+      Tx = group_sum<G>((double)tx); Ty = group_sum<G>((double)ty); Tz = group_sum<G>((double)tz);
+    }
+  }
+  if (PART != 1) {
+    const double Rx = group_sum<G>((double)rx), Ry = group_sum<G>((double)ry),
+                 Rz = group_sum<G>((double)rz);
+    if (S.torque_mode == 1) {
+      double a = group_sum<G>((double)rtx), b = group_sum<G>((double)rty), c = group_sum<G>((double)rtz);
+      // reaction torque on the ligand: -R tau (tau in the ligand frame)
+      const double ux = R[0] * a + R[1] * b + R[2] * c, uy = R[3] * a + R[4] * b + R[5] * c,
+                   uz = R[6] * a + R[7] * b + R[8] * c;
+      if (PART == 2) {
+        Tx = ux; Ty = uy; Tz = uz;
+      } else {
+        Tx -= ux; Ty -= uy; Tz -= uz;
+      }
+    }
+    // Newton's third law: F_lig -= R f_rec (:653-656)
+    const double vx = R[0] * Rx + R[1] * Ry + R[2] * Rz, vy = R[3] * Rx + R[4] * Ry + R[5] * Rz,
+                 vz = R[6] * Rx + R[7] * Ry + R[8] * Rz;
+    if (PART == 2) {
+      Fx = vx; Fy = vy; Fz = vz;
+    } else {
+      Fx -= vx; Fy -= vy; Fz -= vz;
+    }
+  }
+  if (valid && lane == 0) {
+    if (PART == 2) {
+      // the corrections R f_rec and R tau, subtracted by k_rev_apply
+      W.F_rev[3 * o] = Fx; W.F_rev[3 * o + 1] = Fy; W.F_rev[3 * o + 2] = Fz;
+      if (W.T_rev && S.torque_mode == 1) {
+        W.T_rev[3 * o] = Tx; W.T_rev[3 * o + 1] = Ty; W.T_rev[3 * o + 2] = Tz;
+      }
+    } else {
+      W.F[3 * o] = Fx; W.F[3 * o + 1] = Fy; W.F[3 * o + 2] = Fz;
+      if (W.T) { W.T[3 * o] = Tx; W.T[3 * o + 1] = Ty; W.T[3 * o + 2] = Tz; }
+      if (W.E) W.E[o] = Ee;
+    }
+  }
+}
+
 // One system's force evaluation by workgroup `blk` of `nblk` (the workgroups
 // of k_forces, or the slice of a grouped launch that belongs to the system).
 // Always inlined: called as a function from k_forces_group it costs a stack
@@ -99,23 +281,12 @@ __device__ __forceinline__ void forces_body(const ForceSys& S, const ForceWork& 
   typedef typename Vec4<TC>::type V4;
   extern __shared__ char smem_raw[];
   V4* s_lig = reinterpret_cast<V4*>(smem_raw);
-  const bool lds = POSE && S.M <= kLdsAtomsMax;
-  if (lds) {
-    for (int i = threadIdx.x; i < S.M; i += blockDim.x) {
-      V4 v;
-      v.x = (TC)S.lig_body[3 * i]; v.y = (TC)S.lig_body[3 * i + 1];
-      v.z = (TC)S.lig_body[3 * i + 2]; v.w = (TC)S.lig_q[i];
-      s_lig[i] = v;
-    }
-    __syncthreads();
-  }
+  const bool lds = POSE && stage_ligand<TC>(S, s_lig);
   int n = W.n_dev ? *W.n_dev : W.n;
   if (W.n_cap > 0 && n > W.n_cap) n = W.n_cap;
   const int lane = threadIdx.x % G;
   const int grp = threadIdx.x / G;
   const int gpb = kForceBlock / G;
-  const int M = S.M;
-  const double* lq = W.lig_q ? W.lig_q : S.lig_q;
   // Work mapping.  Workgroups are dispatched to the 8 XCDs round-robin
   // (workgroup b runs on XCD b % 8).  With xcd_chunks the list, sorted by
   // direction, is cut into 8 contiguous chunks and XCD x walks chunk x, so
@@ -131,133 +302,7 @@ __device__ __forceinline__ void forces_body(const ForceSys& S, const ForceWork& 
   }
   for (int base = first; base < hi; base += stride) {
     const int w = base + grp;
-    const bool valid = w < hi;
-    const int t = valid ? (W.list ? W.list[w] : w) : 0;
-    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    double cx = 0, cy = 0, cz = 0;
-    bool have_pose = false;
-    if (valid) {
-      if (W.pos) { cx = W.pos[3 * t]; cy = W.pos[3 * t + 1]; cz = W.pos[3 * t + 2]; have_pose = true; }
-      if (W.quat) {
-        quat_to_rot(W.quat + 4 * (size_t)t, R);
-      } else if (W.rot) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) R[k] = W.rot[9 * (size_t)t + k];
-      } else {
-        have_pose = false;
-      }
-    }
-    TC fx = 0, fy = 0, fz = 0, e = 0;
-    TC tx = 0, ty = 0, tz = 0;
-    // lever point for the physical torque: centroid of charged atoms
-    double qx = 0, qy = 0, qz = 0;
-    if (S.torque_mode == 1) {
-      if (POSE) {
-        qx = cx + R[0] * S.cq_body[0] + R[1] * S.cq_body[1] + R[2] * S.cq_body[2];
-        qy = cy + R[3] * S.cq_body[0] + R[4] * S.cq_body[1] + R[5] * S.cq_body[2];
-        qz = cz + R[6] * S.cq_body[0] + R[7] * S.cq_body[1] + R[8] * S.cq_body[2];
-      } else {
-        double sx = 0, sy = 0, sz = 0, cnt = 0;
-        if (valid) {
-          const double* lp = W.lig_pos + (size_t)t * M * 3;
-          for (int i = lane; i < M; i += G)
-            if (fabs(lq[i]) > 1e-9) { sx += lp[3 * i]; sy += lp[3 * i + 1]; sz += lp[3 * i + 2]; cnt += 1; }
-        }
-        sx = group_sum<G>(sx); sy = group_sum<G>(sy); sz = group_sum<G>(sz); cnt = group_sum<G>(cnt);
-        if (cnt > 0) { qx = sx / cnt; qy = sy / cnt; qz = sz / cnt; }
-      }
-    }
-    // no ligand atom can reach a receptor grid box: skip the per-atom box tests
-    const bool grids = !POSE || norm3(cx, cy, cz) - S.lig_rmax <= S.grid_reach;
-    if (valid) {
-      for (int i = lane; i < M; i += G) {
-        double px, py, pz, q;
-        if (POSE) {
-          double bx, by, bz;
-          if (lds) {
-            const V4 v = s_lig[i];
-            bx = v.x; by = v.y; bz = v.z; q = S.lig_q[i];
-          } else {
-            bx = S.lig_body[3 * i]; by = S.lig_body[3 * i + 1]; bz = S.lig_body[3 * i + 2]; q = S.lig_q[i];
-          }
-          // lab = R b + c (gpu_batch_simulator.py:935-937)
-          px = R[0] * bx + R[1] * by + R[2] * bz + cx;
-          py = R[3] * bx + R[4] * by + R[5] * bz + cy;
-          pz = R[6] * bx + R[7] * by + R[8] * bz + cz;
-        } else {
-          const double* lp = W.lig_pos + ((size_t)t * M + i) * 3;
-          px = lp[0]; py = lp[1]; pz = lp[2]; q = lq[i];
-        }
-        TC ax = 0, ay = 0, az = 0, ae = 0;
-        atom_direct<TC, ZP>(S, px, py, pz, q, grids, ax, ay, az, ae);
-        fx += ax; fy += ay; fz += az; e += ae;
-        if (S.torque_mode == 1) {
-          const TC lx = (TC)(px - qx), ly = (TC)(py - qy), lz = (TC)(pz - qz);
-          tx += ly * az - lz * ay; ty += lz * ax - lx * az; tz += lx * ay - ly * ax;
-        }
-      }
-    }
-    // reverse Born: ligand Born grid at receptor atoms (:583-591, :607-657)
-    TC rx = 0, ry = 0, rz = 0, rtx = 0, rty = 0, rtz = 0;
-    if (S.has_rev && have_pose && S.alpha > 1e-12) {
-      bool skip;
-      const double rc = norm3(cx, cy, cz);
-      if (W.rev_skip_mode == 1 && W.chunk_min_r) {
-        const int fbi = w / W.chunk_fb;
-        const int ci = fbi * W.chunks_per_fb + (w - fbi * W.chunk_fb) / W.chunk_cs;
-        skip = W.chunk_min_r[ci] - S.rec_max_r > S.rev_radius;
-      } else {
-        skip = rc - S.rec_max_r > S.rev_cull;
-      }
-      if (!skip) {
-        // runs of receptor atoms farther than rev_cull from the centre have no
-        // atom in the ligand grid box; lane `lane` still visits its atoms
-        // (j = lane mod G) in increasing order
-        for (int k = 0; k < S.n_rec_chunk; ++k) {
-          if (!rec_chunk_near(S, k, cx, cy, cz, S.rev_cull)) continue;
-          const int j1 = min((k + 1) * PST_REC_CHUNK, S.n_rec);
-          for (int j = k * PST_REC_CHUNK + lane; j < j1; j += G) {
-            const double dx = S.rec_pos[3 * j] - cx, dy = S.rec_pos[3 * j + 1] - cy,
-                         dz = S.rec_pos[3 * j + 2] - cz;
-            // x = R^T (r - c) (:641-644)
-            const double x = R[0] * dx + R[3] * dy + R[6] * dz;
-            const double y = R[1] * dx + R[4] * dy + R[7] * dz;
-            const double z = R[2] * dx + R[5] * dy + R[8] * dz;
-            TC ax, ay, az;
-            if (atom_reverse<TC, ZP>(S, x, y, z, S.rec_q[j], ax, ay, az)) {
-              rx += ax; ry += ay; rz += az;
-              if (S.torque_mode == 1) {
-                const TC lx = (TC)(x - S.cq_body[0]), ly = (TC)(y - S.cq_body[1]),
-                         lz = (TC)(z - S.cq_body[2]);
-                rtx += ly * az - lz * ay; rty += lz * ax - lx * az; rtz += lx * ay - ly * ax;
-              }
-            }
-          }
-        }
-      }
-    }
-    double Fx = group_sum<G>((double)fx), Fy = group_sum<G>((double)fy), Fz = group_sum<G>((double)fz);
-    double Ee = group_sum<G>((double)e);
-    double Rx = group_sum<G>((double)rx), Ry = group_sum<G>((double)ry), Rz = group_sum<G>((double)rz);
-    double Tx = 0, Ty = 0, Tz = 0;
-    if (S.torque_mode == 1) {
-      Tx = group_sum<G>((double)tx); Ty = group_sum<G>((double)ty); Tz = group_sum<G>((double)tz);
-      double a = group_sum<G>((double)rtx), b = group_sum<G>((double)rty), c = group_sum<G>((double)rtz);
-      // reaction torque on the ligand: -R tau (tau in the ligand frame)
-      Tx -= R[0] * a + R[1] * b + R[2] * c;
-      Ty -= R[3] * a + R[4] * b + R[5] * c;
-      Tz -= R[6] * a + R[7] * b + R[8] * c;
-    }
-    if (valid && lane == 0) {
-      // Newton's third law: F_lig -= R f_rec (:653-656)
-      Fx -= R[0] * Rx + R[1] * Ry + R[2] * Rz;
-      Fy -= R[3] * Rx + R[4] * Ry + R[5] * Rz;
-      Fz -= R[6] * Rx + R[7] * Ry + R[8] * Rz;
-      const size_t o = W.out_compact ? (size_t)w : (size_t)t;
-      W.F[3 * o] = Fx; W.F[3 * o + 1] = Fy; W.F[3 * o + 2] = Fz;
-      if (W.T) { W.T[3 * o] = Tx; W.T[3 * o + 1] = Ty; W.T[3 * o + 2] = Tz; }
-      if (W.E) W.E[o] = Ee;
-    }
+    force_item<TC, G, POSE, ZP, 0>(S, W, w, w < hi, lane, s_lig, lds);
   }
 }
 
@@ -266,6 +311,137 @@ __device__ __forceinline__ void forces_body(const ForceSys& S, const ForceWork& 
 template <typename TC, int G, bool POSE, bool ZP>
 __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W) {
   forces_body<TC, G, POSE, ZP>(S, W, blockIdx.x, gridDim.x);
+}
+
+// Split evaluation in one launch (pose mode): 2n items over the n
+// trajectories of the list, item i < n the terms on the ligand atoms of
+// W.list[i] (force_item PART 1), item n + i the reverse Born of
+// W.rev_order[i] (PART 2, into F_rev / T_rev).  The two parts of one
+// trajectory are independent; k_rev_apply combines them afterwards.  With
+// xcd_chunks both lists are cut into 8 chunks and XCD x walks the items of
+// chunk x of each, ligand-atom items first: the trajectories in flight on an
+// XCD gather from one region of the receptor grids and, ordered by the
+// receptor's direction in the ligand frame, from one region of the ligand
+// Born grid, and the reverse items fill the tail of the ligand-atom items.
+// Either part alone holds fewer registers than the fused kernel (no
+// accumulators live across both loops).
+template <typename TC, int G, bool ZP>
+__global__ void __launch_bounds__(kForceBlock) k_forces_split(ForceSys S, ForceWork W) {
+  typedef typename Vec4<TC>::type V4;
+  extern __shared__ char smem_raw[];
+  V4* s_lig = reinterpret_cast<V4*>(smem_raw);
+  const bool lds = stage_ligand<TC>(S, s_lig);
+  int n = *W.n_dev;
+  if (W.n_cap > 0 && n > W.n_cap) n = W.n_cap;
+  const int blk = blockIdx.x, nblk = gridDim.x;
+  const int lane = threadIdx.x % G;
+  const int grp = threadIdx.x / G;
+  const int gpb = kForceBlock / G;
+  int lo = 0, m = n, first = blk * gpb, stride = nblk * gpb;
+  if (W.xcd_chunks) {
+    const int xcd = blk & 7;
+    lo = (int)(((long)n * xcd) >> 3);
+    m = (int)(((long)n * (xcd + 1)) >> 3) - lo;
+    first = (blk >> 3) * gpb;
+    stride = (nblk >> 3) * gpb;
+  }
+  ForceWork Wr = W;
+  Wr.list = W.rev_order;
+  for (int base = first; base < 2 * m; base += stride) {
+    const int i = base + grp;
+    if (i < m)
+      force_item<TC, G, true, ZP, 1>(S, W, lo + i, true, lane, s_lig, lds);
+    else if (i < 2 * m)
+      force_item<TC, G, true, ZP, 2>(S, Wr, lo + i - m, true, lane, s_lig, lds);
+  }
+}
+
+// F -= F_rev, T -= T_rev (physical torque) for the trajectories of the list:
+// the reverse-Born correction of the split evaluation, the last operation of
+// the fused kernel's sums.
+__global__ void __launch_bounds__(256) k_rev_apply(const int* __restrict__ list,
+                                                   const int* __restrict__ n_dev, int n_cap,
+                                                   double* __restrict__ F, double* __restrict__ T,
+                                                   const double* __restrict__ F_rev,
+                                                   const double* __restrict__ T_rev) {
+  int n = *n_dev;
+  if (n_cap > 0 && n > n_cap) n = n_cap;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+    const size_t t = (size_t)list[w];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[3 * t + d] -= F_rev[3 * t + d];
+    if (T_rev) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) T[3 * t + d] -= T_rev[3 * t + d];
+    }
+  }
+}
+
+// Order of the reverse-Born pass: the trajectories of `list` (count *n_dev)
+// re-sorted by the direction of the receptor centre seen from the ligand,
+// d = R^T (-c), so that the trajectories in flight on one XCD gather from
+// one region of the ligand Born grid.  Cube-map key: face = dominant axis
+// and its sign (6), then the two other components over the dominant one,
+// each in kRevSide cells, face-major and row-major (neighbouring keys are
+// neighbouring directions).  One workgroup: LDS histogram, scan, scatter
+// (the order inside a key is that of LDS atomics; the reverse pass's
+// results do not depend on it).
+constexpr int kRevSide = 6;
+constexpr int kRevKeys = 6 * kRevSide * kRevSide;   // 216
+
+__device__ inline int rev_key(const double* pos, const double* quat) {
+  double R[9];
+  quat_to_rot(quat, R);
+  const double cx = pos[0], cy = pos[1], cz = pos[2];
+  const double d[3] = {-(R[0] * cx + R[3] * cy + R[6] * cz), -(R[1] * cx + R[4] * cy + R[7] * cz),
+                       -(R[2] * cx + R[5] * cy + R[8] * cz)};
+  const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
+  const int dom = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  const double m = fmax(fabs(d[dom]), 1e-300);
+  const int u = dom == 0 ? 1 : 0, v = dom == 2 ? 1 : 2;
+  int iu = (int)((d[u] / m + 1.0) * (0.5 * kRevSide));
+  int iv = (int)((d[v] / m + 1.0) * (0.5 * kRevSide));
+  iu = iu < 0 ? 0 : (iu >= kRevSide ? kRevSide - 1 : iu);
+  iv = iv < 0 ? 0 : (iv >= kRevSide ? kRevSide - 1 : iv);
+  const int face = 2 * dom + (d[dom] > 0 ? 1 : 0);
+  return (face * kRevSide + iu) * kRevSide + iv;
+}
+
+__global__ void __launch_bounds__(1024) k_rev_sort(const int* __restrict__ list,
+                                                   const int* __restrict__ n_dev, int n_cap,
+                                                   const double* __restrict__ pos,
+                                                   const double* __restrict__ quat,
+                                                   int* __restrict__ out) {
+  __shared__ int hist[kRevKeys];
+  __shared__ int cur[kRevKeys];
+  int n = *n_dev;
+  if (n > n_cap) n = n_cap;
+  for (int k = threadIdx.x; k < kRevKeys; k += blockDim.x) hist[k] = 0;
+  __syncthreads();
+  for (int w = threadIdx.x; w < n; w += blockDim.x) {
+    const int t = list[w];
+    atomicAdd(&hist[rev_key(pos + 3 * (size_t)t, quat + 4 * (size_t)t)], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < kRevKeys; ++k) {
+      cur[k] = run;
+      run += hist[k];
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < n; w += blockDim.x) {
+    const int t = list[w];
+    const int p = atomicAdd(&cur[rev_key(pos + 3 * (size_t)t, quat + 4 * (size_t)t)], 1);
+    out[p] = t;
+  }
+}
+
+hipError_t launch_rev_sort(const int* list, const int* n_dev, int n_cap, const double* pos,
+                           const double* quat, int* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rev_sort, dim3(1), dim3(1024), 0, st, list, n_dev, n_cap, pos, quat, out);
+  return hipGetLastError();
 }
 
 // Several systems (one ligand each, sharing the receptor grids) in one
@@ -340,6 +516,24 @@ __global__ void k_chunk_min_r(const double* pos, const int* list, int n, int fb,
 }
 
 template <typename TC, int G>
+static hipError_t launch_split_g(const ForceSys& S, const ForceWork& W, int max_n, int blocks_cap,
+                                 hipStream_t st, bool zp) {
+  const int gpb = kForceBlock / G;
+  int blocks = (2 * max_n + gpb - 1) / gpb;
+  if (blocks > blocks_cap) blocks = blocks_cap;
+  if (blocks < 8) blocks = 8;
+  blocks = (blocks + 7) / 8 * 8;
+  const size_t lds = S.M <= kLdsAtomsMax ? (size_t)S.M * sizeof(typename Vec4<TC>::type) : 0;
+  if (zp && std::is_same<TC, float>::value)
+    hipLaunchKernelGGL((k_forces_split<TC, G, true>), dim3(blocks), dim3(kForceBlock), lds, st, S,
+                       W);
+  else
+    hipLaunchKernelGGL((k_forces_split<TC, G, false>), dim3(blocks), dim3(kForceBlock), lds, st,
+                       S, W);
+  return hipGetLastError();
+}
+
+template <typename TC, int G>
 static hipError_t launch_g(const ForceSys& S, const ForceWork& W, int max_n, int blocks_cap,
                            hipStream_t st, bool zp) {
   const int gpb = kForceBlock / G;
@@ -392,6 +586,37 @@ hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, i
     case 32: return launch_g<float, 32>(S, W, max_n, blocks_cap, st, zp);
     default: return launch_g<float, 64>(S, W, max_n, blocks_cap, st, zp);
   }
+}
+
+hipError_t launch_forces_split(const ForceSys& S, const ForceWork& W, int precision, int G,
+                               int max_n, int blocks_cap, hipStream_t st, bool zp) {
+  if (max_n <= 0) return hipSuccess;
+  if (!W.n_dev || !W.list || !W.rev_order || !W.F_rev || W.quat == nullptr ||
+      W.lig_pos != nullptr || W.out_compact || W.rev_skip_mode != 0)
+    return hipErrorInvalidValue;
+  const int n_cap = W.n_cap > 0 ? (W.n_cap < max_n ? W.n_cap : max_n) : max_n;
+  hipError_t e = launch_rev_sort(W.list, W.n_dev, n_cap, W.pos, W.quat, W.rev_order, st);
+  if (e != hipSuccess) return e;
+  switch (G) {
+    case 16:
+      e = precision == 0 ? launch_split_g<double, 16>(S, W, max_n, blocks_cap, st, zp)
+                         : launch_split_g<float, 16>(S, W, max_n, blocks_cap, st, zp);
+      break;
+    case 32:
+      e = precision == 0 ? launch_split_g<double, 32>(S, W, max_n, blocks_cap, st, zp)
+                         : launch_split_g<float, 32>(S, W, max_n, blocks_cap, st, zp);
+      break;
+    case 64:
+      e = precision == 0 ? launch_split_g<double, 64>(S, W, max_n, blocks_cap, st, zp)
+                         : launch_split_g<float, 64>(S, W, max_n, blocks_cap, st, zp);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  const double* T_rev = (W.T && S.torque_mode == 1) ? W.T_rev : nullptr;
+  hipLaunchKernelGGL(k_rev_apply, dim3((max_n + 255) / 256 < 1024 ? (max_n + 255) / 256 : 1024),
+                     dim3(256), 0, st, W.list, W.n_dev, n_cap, W.F, W.T, W.F_rev, T_rev);
+  return hipGetLastError();
 }
 
 // Atom census of one force launch over a trajectory list: trajectories,

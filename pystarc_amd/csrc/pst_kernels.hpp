@@ -7,6 +7,17 @@ namespace pst {
 
 hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, int G, int max_n,
                          int blocks_cap, hipStream_t st, bool zp = false);
+// Split evaluation over a device-counted pose-mode list (G in {16, 32, 64}):
+// the reverse order (k_rev_sort into W.rev_order), the ligand-atom and
+// reverse-Born items in one launch (k_forces_split, corrections into W.F_rev /
+// W.T_rev), then F -= F_rev, T -= T_rev (k_rev_apply).  Bit-identical to
+// launch_forces with the same G.
+hipError_t launch_forces_split(const ForceSys& S, const ForceWork& W, int precision, int G,
+                               int max_n, int blocks_cap, hipStream_t st, bool zp);
+// out[0 .. min(*n_dev, n_cap)) = list re-sorted by the ligand-frame direction
+// of the receptor (the reverse-Born pass's order).
+hipError_t launch_rev_sort(const int* list, const int* n_dev, int n_cap, const double* pos,
+                           const double* quat, int* out, hipStream_t st);
 hipError_t launch_chunk_min_r(const double* pos, const int* list, int n, int fb, int cs, int cpf,
                               double* out, int n_chunks, hipStream_t st);
 int pick_group(int M, int n_rec_active);

@@ -103,6 +103,11 @@ struct ForceWork {
   int out_compact;          // 1: write outputs at list position, 0: at trajectory index
   int n_cap;                // > 0: process at most n_cap list entries
   int xcd_chunks;           // 1: XCD x (= blockIdx % 8) takes the x-th eighth of the list
+  // split evaluation: room for the reverse pass's order (list capacity) and
+  // the reverse-Born corrections per trajectory (indexed like F / T)
+  int* rev_order;
+  double* F_rev;
+  double* T_rev;
 };
 
 // Step parameters (gpu_batch_simulator.py:480-781).

@@ -35,6 +35,8 @@ from .._lib import Context, ptr
 from ..constants import EPS0
 
 _KIND = {"elec": 0, "born": 1, "lig_born": 2}
+# force-evaluation modes of the library, by number (pst_set_gather_mode)
+GATHER_MODES = ("direct", "binned", "zpair", "split", "split_zpair")
 
 
 def _grid_parts(g):
@@ -146,21 +148,26 @@ class GPUBatchForceEngine:
         kernel runs.
         "zpair"  - the direct kernel fetching the two z-adjacent nodes of a
         stencil column with one wide load (FP32; identical results; the
-        FP64 engine runs the direct kernel)."""
-        if mode not in ("direct", "binned", "zpair"):
-            raise ValueError("mode must be 'direct', 'binned' or 'zpair'")
-        m = {"direct": 0, "binned": 1, "zpair": 2}[mode]
+        FP64 engine runs the direct kernel).
+        "split" / "split_zpair" - production steps only: the ligand-atom
+        terms over the grid-path list, then the reverse Born over the list
+        re-sorted by the ligand-frame direction of the receptor (identical
+        results); elsewhere "direct" / "zpair" run."""
+        if mode not in GATHER_MODES:
+            raise ValueError(f"mode must be one of {GATHER_MODES}")
+        m = GATHER_MODES.index(mode)
         self.ctx.check(self.ctx.lib.pst_set_gather_mode(self.ctx.h, m,
                                                         int(max_traj) if m == 1 else 0),
                        "pst_set_gather_mode")
         self.gather = mode
 
     def last_eval(self):
-        """(mode, G) of the last force evaluation: "direct" / "binned" / "zpair", lanes."""
+        """(mode, G) of the last force evaluation: a GATHER_MODES name (None before
+        the first), lanes."""
         m, g = C.c_int(), C.c_int()
         self.ctx.check(self.ctx.lib.pst_last_force_eval(self.ctx.h, C.byref(m), C.byref(g)),
                        "pst_last_force_eval")
-        return {-1: None, 0: "direct", 1: "binned", 2: "zpair"}[m.value], g.value
+        return (GATHER_MODES[m.value] if m.value >= 0 else None), g.value
 
     def set_ligand(self, body, charges):
         """Body-frame ligand coordinates (about the centroid) for the fused path."""

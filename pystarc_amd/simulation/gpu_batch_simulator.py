@@ -34,7 +34,8 @@ Keyword-only extensions (defaults keep the reference's physics):
                after which the production force launch (direct or
                brick-binned evaluation, lanes per trajectory, workgroup cap,
                XCD mapping) is tuned on live steps.
-  gather       None (the engine's), "direct", "binned" or "zpair": force evaluation
+  gather       None (the engine's) or a GATHER_MODES name ("direct", "binned", "zpair",
+               "split", "split_zpair"): force evaluation
                of the production step (identical results either way).
   hydro        "scalar" (default): the reference's scalar relative diffusion
                (D_t, or D_parallel(r) with hydrodynamic_interactions).
@@ -84,6 +85,7 @@ from typing import Dict
 import numpy as np
 
 from .._lib import MAX_PAIRS, StepParams, ptr
+from ..forces.gpu_batch_engine import GATHER_MODES
 from ..constants import CONV, EPS0, MU_REF, OVERLAP_THRESHOLD, QB_FACTOR
 from ..lib.numerical import romberg_integrate
 
@@ -285,7 +287,7 @@ def autotune_sim(lib, ctx, h, rounds=3, steps_per_cand=1):
     best = (C.c_int * 5)()
     ctx.check(lib.pst_prod_autotune(h, int(rounds), int(steps_per_cand), ptr(table), 96,
                                     C.byref(n), best), "pst_prod_autotune")
-    names = {0: "direct", 1: "binned", 2: "zpair"}
+    names = GATHER_MODES
     rows = [{"mode": names[int(r[0])], "G": int(r[1]), "cap": int(r[2]),
              "xcd": int(r[3]), "ms": float(r[4])} for r in table[:n.value]]
     return {"best": {"mode": names[int(best[0])], "G": best[1], "cap": best[2],
@@ -496,8 +498,8 @@ class GPUBatchSimulator:
             raise ValueError("rpy12 needs both hydrodynamic radii (mob.radius1, mob.radius2)")
         self.hydro = hydro
         self.autotune = int(autotune) if autotune is not True else 64
-        if gather not in (None, "direct", "binned", "zpair"):
-            raise ValueError("gather must be None, 'direct', 'binned' or 'zpair'")
+        if gather is not None and gather not in GATHER_MODES:
+            raise ValueError(f"gather must be None or one of {GATHER_MODES}")
         self.gather = gather
         self.mol1, self.mol2, self.mob = mol1, mol2, mob
         self.pathway_set, self.params, self.engine = pathway_set, params, batch_engine
@@ -697,7 +699,7 @@ class GPUBatchSimulator:
                       "pst_sim_create")
         if self.gather is not None:
             eng.ctx.check(eng.ctx.lib.pst_prod_set_launch(
-                h, {"direct": 0, "binned": 1, "zpair": 2}[self.gather], 0, 2048, 1),
+                h, GATHER_MODES.index(self.gather), 0, 2048, 1),
                 "pst_prod_set_launch")
         return h
 
@@ -795,8 +797,8 @@ class GPUBatchSimulator:
                 key = self.tune_key()
                 best = tune_cache_get(key)
                 if best is not None:
-                    ctx.check(lib.pst_prod_set_launch(h, {"direct": 0, "binned": 1, "zpair": 2}[
-                        best["mode"]], int(best["G"]), int(best["cap"]), int(best["xcd"])),
+                    ctx.check(lib.pst_prod_set_launch(h, GATHER_MODES.index(best["mode"]),
+                        int(best["G"]), int(best["cap"]), int(best["xcd"])),
                         "pst_prod_set_launch")
                     tuned = {"best": best, "table": [], "steps": 0, "cached": True}
                 else:
