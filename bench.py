@@ -26,7 +26,10 @@ window).  Paths timed on identical seeds:
   surrogate  MLP beyond contact + 8 A, FP32 grid path inside, per-step audit
 
 k_on: fp64 and surrogate each run --kon-trajectories complete trajectories;
-P_rxn with Wilson 95 % intervals go in the JSON line.
+P_rxn with Wilson 95 % intervals go in the JSON line, with whether the
+surrogate's P lies in the plain path's interval and the two-sample z of the
+difference (two independent estimates differ by more than the plain
+interval's half-width about one time in six).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--no-cpu]
 """
@@ -189,6 +192,15 @@ def full_run(sysm, eng, args, surrogate=None):
     return out
 
 
+def two_sample_z(a, b):
+    """z of P_a - P_b for two reaction probabilities from independent runs
+    (pooled binomial variance over reacted / (reacted + escaped))."""
+    na, nb = a["reacted"] + a["escaped"], b["reacted"] + b["escaped"]
+    p = (a["reacted"] + b["reacted"]) / max(na + nb, 1)
+    se = math.sqrt(max(p * (1.0 - p) * (1.0 / max(na, 1) + 1.0 / max(nb, 1)), 1e-300))
+    return (a["P"] - b["P"]) / se
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -332,16 +344,17 @@ def harvest_poses(sysm, eng, args, n_want, r_on, every=4, n_slots=None):
     return np.concatenate(got_p)[:n_want], np.concatenate(got_q)[:n_want]
 
 
-def measured_traffic(workload, path):
-    """HBM bytes per force launch of `path` from the committed rocprofv3 --pmc
-    summary of the workload (FETCH_SIZE and WRITE_SIZE, calibrated), or None."""
+def measured_traffic(workload, path, slots):
+    """HBM bytes per force evaluation of `path` from the committed rocprofv3
+    --pmc summary of the workload at the same slot count (FETCH_SIZE and
+    WRITE_SIZE, calibrated), or None."""
     f = os.path.join(ROOT, "profiles", "k_forces_traffic.json")
     try:
         with open(f) as fh:
             d = json.load(fh)
-        if d.get("workload_key", "protein_protein") != workload:
+        if d.get("workload_key", "protein_protein") != workload or d.get("slots") != slots:
             return None
-        return d["paths"][path]["hbm_bytes_per_launch"]
+        return d["paths"][path]["hbm_bytes_per_eval"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -499,6 +512,8 @@ def multi_ligand_bench(args):
             kon["surrogate_in_plain_ci95"] = [
                 bool(a["P_ci95"][0] <= b["P"] <= a["P_ci95"][1])
                 for a, b in zip(kon["fp64"]["ligands"], kon["surrogate"]["ligands"])]
+            kon["two_sample_z"] = [two_sample_z(b, a) for a, b in
+                                   zip(kon["fp64"]["ligands"], kon["surrogate"]["ligands"])]
     head = "surrogate" if "surrogate" in results else paths[0]
     best = results[head]
     plain = results.get("fp64")
@@ -534,7 +549,7 @@ def main():
     ap.add_argument("--trajectories", type=int, default=10_000_000)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--paths", default="fp64,fp32,surrogate")
-    ap.add_argument("--kon-trajectories", type=int, default=200_000)
+    ap.add_argument("--kon-trajectories", type=int, default=500_000)
     # the surrogate workload's 2e7 (pose, force) training pairs
     ap.add_argument("--train-poses", type=int, default=20_000_000)
     ap.add_argument("--train-epochs", type=int, default=40)
@@ -635,6 +650,7 @@ def main():
         if "fp64" in kon and "surrogate" in kon:
             lo, hi = kon["fp64"]["P_ci95"]
             kon["surrogate_in_plain_ci95"] = bool(lo <= kon["surrogate"]["P"] <= hi)
+            kon["two_sample_z"] = two_sample_z(kon["surrogate"], kon["fp64"])
     head = "surrogate" if "surrogate" in results else max(results, key=lambda k: results[k]["value"])
     best = results[head]
     cpu = None
@@ -662,13 +678,12 @@ def main():
         tc = "double" if head == "fp64" else "float"
         zp = "true" if lc and lc.get("mode") in ("zpair", "split_zpair") else "false"
         if lc and lc.get("mode", "").startswith("split"):
-            kname = (f"k_rev_sort + k_forces<{tc}, {lc['G']}, true, {zp}, 1> + "
-                     f"k_forces<{tc}, {lc['G']}, true, {zp}, 2>")
+            kname = f"k_rev_sort + k_forces_split<{tc}, {lc['G']}, {zp}> + k_rev_apply"
         else:
             kname = f"k_forces<{tc}, {lc['G']}, true, {zp}>" if lc else "k_forces"
         roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": rf["achieved_gbs"] / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.workload, head),
+                "traffic": measured_traffic(args.workload, head, args.slots),
                 "kernel": kname, "ms_per_launch": rf["ms"], "launches": rf["launches"],
                 "census": rf["census"], "algorithmic_bytes": rf["bytes"],
                 "gather_ceiling_gbs": ceiling["clustered"]["gbs"],

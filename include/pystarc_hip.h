@@ -146,8 +146,10 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
  * (measurement only; no reference counterpart): n_points 20-node stencil
  * gathers (160 algorithmic bytes each, the FP32 force path's gather) at
  * points uniform over the grid (mode 0) or clustered 256 to a cube of edge
- * 12 / mode A (mode 1..16), `reps` timed launches.  out4 = {ms per launch,
- * algorithmic GB/s, points, bytes per point}. */
+ * 12 / mode A (mode 1..16), `reps` timed launches; plus 32 x the fetch
+ * variant (0 the plain stencil, 1 the z-pair fetches of the force kernel's
+ * z-pair mode).  out4 = {ms per launch, algorithmic GB/s, points, bytes per
+ * point}. */
 int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, double* out4);
 
 /* ---- two-body RPY hydrodynamics ------------------------------------------

@@ -21,10 +21,17 @@ using namespace pst;
 namespace {
 
 struct DevGrid {
-  GridDev g;
+  GridDev g{};
   void* mem = nullptr;
   bool owned = true;   // false: device memory of another context's grid
 };
+
+static void free_grid(DevGrid& g) {
+  if (g.owned && g.mem) hipFree(g.mem);
+  g.mem = nullptr;
+  g.g.data = nullptr;
+  g.owned = true;
+}
 
 }  // namespace
 
@@ -316,8 +323,7 @@ int pst_ctx_create(pst_ctx** out, int device, void* stream, int use_stream) {
 }
 
 static void free_grids(std::vector<DevGrid>& v) {
-  for (auto& g : v)
-    if (g.mem && g.owned) hipFree(g.mem);
+  for (auto& g : v) free_grid(g);
   v.clear();
 }
 
@@ -328,7 +334,7 @@ int pst_ctx_destroy(pst_ctx* ctx) {
   free_grids(ctx->elec);
   free_grids(ctx->born);
   free_grids(ctx->lig_born);
-  if (ctx->pair.mem && ctx->pair.owned) hipFree(ctx->pair.mem);
+  free_grid(ctx->pair);
   if (ctx->d_body) hipFree(ctx->d_body);
   if (ctx->d_q) hipFree(ctx->d_q);
   if (ctx->d_rec) hipFree(ctx->d_rec);
@@ -348,11 +354,7 @@ const char* pst_last_error(const pst_ctx* ctx) { return ctx ? ctx->err.c_str() :
 // Keep an interleaved float2 copy of the receptor potential and Born grids
 // when there is exactly one of each, both FP32, with identical geometry.
 static int update_pair(pst_ctx* ctx) {
-  if (ctx->pair.mem) {
-    if (ctx->pair.owned) hipFree(ctx->pair.mem);
-    ctx->pair.mem = nullptr;
-    ctx->pair.owned = true;
-  }
+  free_grid(ctx->pair);
   if (ctx->elec.size() != 1 || ctx->born.size() != 1) return 0;
   const GridDev& a = ctx->elec[0].g;
   const GridDev& b = ctx->born[0].g;
@@ -426,7 +428,7 @@ int pst_share_receptor_grids(pst_ctx* dst, const pst_ctx* src) {
   CK(set_dev(ctx));
   for (const auto& g : src->elec) { DevGrid d = g; d.owned = false; dst->elec.push_back(d); }
   for (const auto& g : src->born) { DevGrid d = g; d.owned = false; dst->born.push_back(d); }
-  if (dst->pair.mem && dst->pair.owned) hipFree(dst->pair.mem);
+  free_grid(dst->pair);
   dst->pair = src->pair;
   dst->pair.owned = false;
   return 0;
@@ -681,10 +683,12 @@ static bool split_eligible(const ForceSys& S, const ForceWork& W, int G) {
 static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const ForceWork& W,
                                     int precision, int G, int max_n, int cap, int mode) {
   ctx->last_G = G;
+  // z-pair fetches are an FP32 launch variant
+  if (precision != 1 && mode == 2) mode = 0;
+  if (precision != 1 && mode == 4) mode = 3;
   if ((mode == 3 || mode == 4) && split_eligible(S, W, G)) {
     ctx->last_mode = mode;
-    return launch_forces_split(S, W, precision, G, max_n, cap, ctx->stream,
-                               mode == 4 && precision == 1);
+    return launch_forces_split(S, W, precision, G, max_n, cap, ctx->stream, mode == 4);
   }
   if (mode == 3) mode = 0;
   if (mode == 4) mode = 2;
@@ -1010,7 +1014,9 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
 int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, double* out4) {
   if (!ctx) return 1;
   if (!ctx->pair.mem) FAIL("pst_gather_ceiling: needs the FP32 potential + Born pair grid");
-  if (mode < 0 || mode > 16) FAIL("pst_gather_ceiling: mode must be 0..16");
+  const int variant = mode >> 5;
+  mode &= 31;
+  if (mode > 16 || variant > 1) FAIL("pst_gather_ceiling: mode must be 0..16 plus 32 x variant 0..1");
   if (n_points <= 0 || n_points > (1ll << 30) || reps <= 0 || !out4)
     FAIL("pst_gather_ceiling: bad arguments");
   CK(set_dev(ctx));
@@ -1021,11 +1027,12 @@ int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, dou
   if (e == hipSuccess) e = hipEventCreate(&ev.b);
   const long long nb = (n_points + 255) / 256;
   const int blocks = (int)(nb < 65536 ? nb : 65536);
-  if (e == hipSuccess) e = launch_gather_probe(ctx->pair.g, n_points, mode, 12345, out, blocks,
-                                               ctx->stream);
+  if (e == hipSuccess) e = launch_gather_probe(ctx->pair.g, n_points, mode, variant, 12345, out,
+                                               blocks, ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ev.a, ctx->stream);
   for (int r = 0; r < reps && e == hipSuccess; ++r)
-    e = launch_gather_probe(ctx->pair.g, n_points, mode, 777 + r, out, blocks, ctx->stream);
+    e = launch_gather_probe(ctx->pair.g, n_points, mode, variant, 777 + r, out, blocks,
+                            ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ev.b, ctx->stream);
   if (e == hipSuccess) e = hipEventSynchronize(ev.b);
   float ms = 0;
@@ -1355,7 +1362,7 @@ static ProdBufs prod_bufs(pst_sim* s) {
   ProdBufs B;
   B.pos = s->pos; B.quat = s->quat; B.status = s->status; B.n_steps = s->n_steps32;
   B.traj = s->traj; B.plain_list = s->plain_list; B.mlp_list = s->mlp_list;
-  B.key = s->key; B.bins = s->bins;
+  B.key = s->key; B.bins = s->bins; B.rev_list = s->rev_list;
   B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
   B.ctr = s->ctr; B.out_status = s->out_status; B.out_steps = s->out_steps;
   B.obs = s->obs;
@@ -1387,12 +1394,12 @@ int pst_prod_init(pst_sim* s, int record) {
     CK(hipMalloc(&s->mlp_list, S * 4));
     CK(hipMalloc(&s->ctr, sizeof(RunCounters)));
     CK(hipMalloc(&s->key, S * 4));
-    CK(hipMalloc(&s->bins, 512 * 4));
+    CK(hipMalloc(&s->bins, 1024 * 4));
     CK(hipMalloc(&s->sm_state, S * 4));
     CK(hipMalloc(&s->sm_mask, S * 4));
     CK(hipMalloc(&s->lmz_escaped, S * 4));
   }
-  CK(hipMemsetAsync(s->bins, 0, 512 * 4, ctx->stream));
+  CK(hipMemsetAsync(s->bins, 0, 1024 * 4, ctx->stream));
   CK(hipMemsetAsync(s->lmz_escaped, 0, S * 4, ctx->stream));
   if (s->out_status) { hipFree(s->out_status); s->out_status = nullptr; }
   if (s->out_steps) { hipFree(s->out_steps); s->out_steps = nullptr; }
@@ -1935,6 +1942,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     fill_sys(sims[l]->ctx, S[l]);
     P[l] = sims[l]->P;
     B[l] = prod_bufs(sims[l]);
+    B[l].rev_list = nullptr;   // the grouped force launch has no split evaluation
     sstart[l + 1] = sstart[l] + (B[l].n_slots + sb - 1) / sb;
     W[l] = main_work(sims[l]);
     W[l].xcd_chunks = 1;
@@ -2210,7 +2218,7 @@ static std::vector<Region> state_regions(pst_sim* s) {
       {s->n_steps32, S * 4},     {s->traj, S * 8},            {s->F, S * 3 * 8},
       {s->T, S * 3 * 8},         {s->dt, S * 8},              {s->x0, S * PST_MAX_PAIRS * 8},
       {s->x1, S * PST_MAX_PAIRS * 8}, {s->sm_state, S * 4},   {s->sm_mask, S * 4},
-      {s->lmz_escaped, S * 4},   {s->key, S * 4},             {s->bins, 512 * 4},
+      {s->lmz_escaped, S * 4},   {s->key, S * 4},             {s->bins, 1024 * 4},
       {s->ctr, sizeof(RunCounters)}};
   if (s->out_status) {
     r.push_back({s->out_status, N});

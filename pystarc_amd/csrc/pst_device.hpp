@@ -124,6 +124,27 @@ __device__ inline void apply_rotation(double* q, double ox, double oy, double oz
 
 __device__ inline double norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
 
+// Cube-map cell of a direction d (the order of the split evaluation's
+// reverse-Born pass: d = R^T (-c), the receptor centre seen from the ligand):
+// face = dominant axis and its sign (6), then the two other components over
+// the dominant one, each in kDirSide cells; face-major, row-major, so
+// neighbouring cells are neighbouring directions.
+constexpr int kDirSide = 6;
+constexpr int kDirCells = 6 * kDirSide * kDirSide;   // 216
+__device__ inline int dir_cell(double dx, double dy, double dz) {
+  const double d[3] = {dx, dy, dz};
+  const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+  const int dom = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  const double m = fmax(fabs(d[dom]), 1e-300);
+  const int u = dom == 0 ? 1 : 0, v = dom == 2 ? 1 : 2;
+  int iu = (int)((d[u] / m + 1.0) * (0.5 * kDirSide));
+  int iv = (int)((d[v] / m + 1.0) * (0.5 * kDirSide));
+  iu = iu < 0 ? 0 : (iu >= kDirSide ? kDirSide - 1 : iu);
+  iv = iv < 0 ? 0 : (iv >= kDirSide ? kDirSide - 1 : iv);
+  const int face = 2 * dom + (d[dom] > 0 ? 1 : 0);
+  return (face * kDirSide + iu) * kDirSide + iv;
+}
+
 // ---------------------------------------------------------------------------
 // Trilinear gather, batch_force_kernel (gpu_batch_engine.py:95-110): cell
 // index floor((p - o) / sp); value 0 when the cell is outside [0, n-2].
@@ -164,8 +185,8 @@ template <> __device__ inline Pair<double> gld<Pair<double>, float2>(const float
 // 64 B of FP32 or 128 B of float2).  The 20 nodes of one atom's stencil then
 // sit in ~3-5 cache lines instead of the ~8-9 of a C-order array, where
 // every (i, j) column of the stencil is a line of its own.
-// ZP: fetch the z-adjacent node pairs of a stencil with one wide load where
-// both nodes sit in one brick (the direct kernel's "z-pair" launch variant).
+// ZP: fetch the z nodes of every stencil column as aligned pairs (the direct
+// kernel's "z-pair" launch variant, zcol3 below).
 template <bool ZP>
 struct GlobalAccT {
   static constexpr bool kZPairs = ZP;
@@ -185,9 +206,8 @@ struct TileAcc {
   __device__ long oz(int k) const { return (long)(k - tk); }
   __device__ long at(int i, int j, int k) const { return ox(i) + oy(j) + oz(k); }
 };
-// Nodes (k0, k0 + 1) of one (i, j) column.  In a bricked grid they are
-// adjacent unless k0 ends a brick; with z-pair fetches they then come in one
-// 8- or 16-byte load (the values are those of two single loads).
+// Two z-adjacent nodes of one (i, j) column in one 8- or 16-byte load (the
+// values are those of two single loads).
 typedef float pst_f2a4 __attribute__((ext_vector_type(2), aligned(4)));
 typedef float pst_f4a8 __attribute__((ext_vector_type(4), aligned(8)));
 typedef double pst_d2a8 __attribute__((ext_vector_type(2), aligned(8)));
@@ -218,16 +238,21 @@ template <> __device__ inline void gld_pair2<Pair<double>, float2>(const float2*
   const pst_f4a8 v = *(const pst_f4a8*)(d + o);
   lo = {(double)v.x, (double)v.y}; hi = {(double)v.z, (double)v.w};
 }
-template <typename V, typename TG, typename A>
-__device__ inline void gld_z(const TG* d, const A& a, long xy, int k0, long Z0, long Z1, V& lo, V& hi) {
-  if constexpr (A::kZPairs) {
-    if ((k0 & 3) != 3) {
-      gld_pair2<V, TG>(d, xy + Z0, lo, hi);
-      return;
-    }
-  }
-  lo = gld<V>(d, xy + Z0);
-  hi = gld<V>(d, xy + Z1);
+// Nodes kmin, kmin + 1, kmin + 2 of one column of a bricked grid from its
+// aligned pairs (e, e + 1) at offset oa and (e + 2, e + 3) at ob, e = kmin
+// rounded down to even (odd: kmin = e + 1).  An even node starts a pair
+// inside its brick's z-row, so neither load crosses a brick and every lane
+// takes the same two loads, wherever its stencil sits; a wave whose pairs
+// were fetched only where they fit in a brick would issue both the pair and
+// the single loads whenever its lanes disagree.
+template <typename V, typename TG>
+__device__ inline void zcol3(const TG* d, long oa, long ob, bool odd, V& m0, V& m1, V& m2) {
+  V q0, q1, q2, q3;
+  gld_pair2<V, TG>(d, oa, q0, q1);
+  gld_pair2<V, TG>(d, ob, q2, q3);
+  m0 = odd ? q1 : q0;
+  m1 = odd ? q2 : q1;
+  m2 = odd ? q3 : q2;
 }
 
 template <bool ZP = false>
@@ -235,6 +260,7 @@ __device__ inline GlobalAccT<ZP> global_acc(const GridDev& g) {
   const long nbz = (g.nz + 3) >> 2, nby = (g.ny + 3) >> 2;
   return {nby * nbz * 64, nbz * 64};
 }
+
 
 template <typename TG, typename TC, typename V, typename A>
 __device__ inline V grid_interp_a(const GridDev& g, const TG* __restrict__ d, const A& a, double x,
@@ -318,19 +344,46 @@ __device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __res
   // (index -1 when a = -1, +2 when a = 0)
   const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Xe = a.ox(ai < 0 ? i0 - 1 : i0 + 2);
   const long Y0 = a.oy(j0), Y1 = a.oy(j0 + 1), Ye = a.oy(aj < 0 ? j0 - 1 : j0 + 2);
-  const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1), Ze = a.oz(ak < 0 ? k0 - 1 : k0 + 2);
-  // centre cell and the x- and y-extra planes, z-pairs (k0, k0 + 1)
+  // centre cell (c), the x- and y-extra planes (x, y) at z nodes (k0, k0 + 1),
+  // and the z-extra plane (z) at node k0 - 1 (ak = -1) or k0 + 2 (ak = 0)
   V c000, c001, c010, c011, c100, c101, c110, c111, x00, x01, x10, x11, y00, y01, y10, y11;
-  gld_z<V, TG, A>(d, a, X0 + Y0, k0, Z0, Z1, c000, c001);
-  gld_z<V, TG, A>(d, a, X0 + Y1, k0, Z0, Z1, c010, c011);
-  gld_z<V, TG, A>(d, a, X1 + Y0, k0, Z0, Z1, c100, c101);
-  gld_z<V, TG, A>(d, a, X1 + Y1, k0, Z0, Z1, c110, c111);
-  gld_z<V, TG, A>(d, a, Xe + Y0, k0, Z0, Z1, x00, x01);
-  gld_z<V, TG, A>(d, a, Xe + Y1, k0, Z0, Z1, x10, x11);
-  gld_z<V, TG, A>(d, a, X0 + Ye, k0, Z0, Z1, y00, y01);
-  gld_z<V, TG, A>(d, a, X1 + Ye, k0, Z0, Z1, y10, y11);
-  const V z00 = gld<V>(d, X0 + Y0 + Ze), z01 = gld<V>(d, X0 + Y1 + Ze),
-          z10 = gld<V>(d, X1 + Y0 + Ze), z11 = gld<V>(d, X1 + Y1 + Ze);
+  V z00, z01, z10, z11;
+  if constexpr (A::kZPairs) {
+    // every column's nodes kmin .. kmin + 2 (kmin = k0 + ak) in two aligned
+    // pair loads: 16 loads per stencil, the same for every lane
+    const int kmin = k0 + ak, e = kmin & ~1;
+    const bool odd = (kmin & 1) != 0, lowx = ak < 0;
+    const long Za = a.oz(e), Zb = a.oz(e + 2);
+    V u;
+    // (m0, m1, m2) = nodes kmin .. kmin + 2: the z-extra node is m0 (ak = -1) or m2
+    auto centre = [&](long xy, V& lo, V& hi, V& ze) {
+      V m0, m1, m2;
+      zcol3<V, TG>(d, xy + Za, xy + Zb, odd, m0, m1, m2);
+      lo = lowx ? m1 : m0;
+      hi = lowx ? m2 : m1;
+      ze = lowx ? m0 : m2;
+    };
+    centre(X0 + Y0, c000, c001, z00);
+    centre(X0 + Y1, c010, c011, z01);
+    centre(X1 + Y0, c100, c101, z10);
+    centre(X1 + Y1, c110, c111, z11);
+    centre(Xe + Y0, x00, x01, u);
+    centre(Xe + Y1, x10, x11, u);
+    centre(X0 + Ye, y00, y01, u);
+    centre(X1 + Ye, y10, y11, u);
+  } else {
+    const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1), Ze = a.oz(ak < 0 ? k0 - 1 : k0 + 2);
+    c000 = gld<V>(d, X0 + Y0 + Z0); c001 = gld<V>(d, X0 + Y0 + Z1);
+    c010 = gld<V>(d, X0 + Y1 + Z0); c011 = gld<V>(d, X0 + Y1 + Z1);
+    c100 = gld<V>(d, X1 + Y0 + Z0); c101 = gld<V>(d, X1 + Y0 + Z1);
+    c110 = gld<V>(d, X1 + Y1 + Z0); c111 = gld<V>(d, X1 + Y1 + Z1);
+    x00 = gld<V>(d, Xe + Y0 + Z0); x01 = gld<V>(d, Xe + Y0 + Z1);
+    x10 = gld<V>(d, Xe + Y1 + Z0); x11 = gld<V>(d, Xe + Y1 + Z1);
+    y00 = gld<V>(d, X0 + Ye + Z0); y01 = gld<V>(d, X0 + Ye + Z1);
+    y10 = gld<V>(d, X1 + Ye + Z0); y11 = gld<V>(d, X1 + Ye + Z1);
+    z00 = gld<V>(d, X0 + Y0 + Ze); z01 = gld<V>(d, X0 + Y1 + Ze);
+    z10 = gld<V>(d, X1 + Y0 + Ze); z11 = gld<V>(d, X1 + Y1 + Ze);
+  }
   const TC one = (TC)1;
   const TC gxw = one - fx, gyw = one - fy, gzw = one - fz;
   phi = c000 * gxw * gyw * gzw + c100 * fx * gyw * gzw + c010 * gxw * fy * gzw +
@@ -411,6 +464,7 @@ __device__ inline void grid_phi_grad_pair(const GridDev& g, double px, double py
   else
     grid_phi_grad_stencil<float2, float, Pair<float>, ZP>(g, px, py, pz, phi, gx, gy, gz);
 }
+
 
 // The same two gathers through an accessor (a grid tile staged in LDS):
 // the float2 potential + Born pair, and a single FP32 field.

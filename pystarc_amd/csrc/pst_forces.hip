@@ -316,7 +316,9 @@ __global__ void __launch_bounds__(kForceBlock) k_forces(ForceSys S, ForceWork W)
 // Split evaluation in one launch (pose mode): 2n items over the n
 // trajectories of the list, item i < n the terms on the ligand atoms of
 // W.list[i] (force_item PART 1), item n + i the reverse Born of
-// W.rev_order[i] (PART 2, into F_rev / T_rev).  The two parts of one
+// W.rev_order[i] (PART 2, into F_rev / T_rev).  W.rev_order holds the same
+// trajectories ordered by dir_cell of the receptor's direction in the ligand
+// frame (the production step's k_bins_scatter writes it).  The two parts of one
 // trajectory are independent; k_rev_apply combines them afterwards.  With
 // xcd_chunks both lists are cut into 8 chunks and XCD x walks the items of
 // chunk x of each, ligand-atom items first: the trajectories in flight on an
@@ -377,72 +379,6 @@ __global__ void __launch_bounds__(256) k_rev_apply(const int* __restrict__ list,
   }
 }
 
-// Order of the reverse-Born pass: the trajectories of `list` (count *n_dev)
-// re-sorted by the direction of the receptor centre seen from the ligand,
-// d = R^T (-c), so that the trajectories in flight on one XCD gather from
-// one region of the ligand Born grid.  Cube-map key: face = dominant axis
-// and its sign (6), then the two other components over the dominant one,
-// each in kRevSide cells, face-major and row-major (neighbouring keys are
-// neighbouring directions).  One workgroup: LDS histogram, scan, scatter
-// (the order inside a key is that of LDS atomics; the reverse pass's
-// results do not depend on it).
-constexpr int kRevSide = 6;
-constexpr int kRevKeys = 6 * kRevSide * kRevSide;   // 216
-
-__device__ inline int rev_key(const double* pos, const double* quat) {
-  double R[9];
-  quat_to_rot(quat, R);
-  const double cx = pos[0], cy = pos[1], cz = pos[2];
-  const double d[3] = {-(R[0] * cx + R[3] * cy + R[6] * cz), -(R[1] * cx + R[4] * cy + R[7] * cz),
-                       -(R[2] * cx + R[5] * cy + R[8] * cz)};
-  const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
-  const int dom = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
-  const double m = fmax(fabs(d[dom]), 1e-300);
-  const int u = dom == 0 ? 1 : 0, v = dom == 2 ? 1 : 2;
-  int iu = (int)((d[u] / m + 1.0) * (0.5 * kRevSide));
-  int iv = (int)((d[v] / m + 1.0) * (0.5 * kRevSide));
-  iu = iu < 0 ? 0 : (iu >= kRevSide ? kRevSide - 1 : iu);
-  iv = iv < 0 ? 0 : (iv >= kRevSide ? kRevSide - 1 : iv);
-  const int face = 2 * dom + (d[dom] > 0 ? 1 : 0);
-  return (face * kRevSide + iu) * kRevSide + iv;
-}
-
-__global__ void __launch_bounds__(1024) k_rev_sort(const int* __restrict__ list,
-                                                   const int* __restrict__ n_dev, int n_cap,
-                                                   const double* __restrict__ pos,
-                                                   const double* __restrict__ quat,
-                                                   int* __restrict__ out) {
-  __shared__ int hist[kRevKeys];
-  __shared__ int cur[kRevKeys];
-  int n = *n_dev;
-  if (n > n_cap) n = n_cap;
-  for (int k = threadIdx.x; k < kRevKeys; k += blockDim.x) hist[k] = 0;
-  __syncthreads();
-  for (int w = threadIdx.x; w < n; w += blockDim.x) {
-    const int t = list[w];
-    atomicAdd(&hist[rev_key(pos + 3 * (size_t)t, quat + 4 * (size_t)t)], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int k = 0; k < kRevKeys; ++k) {
-      cur[k] = run;
-      run += hist[k];
-    }
-  }
-  __syncthreads();
-  for (int w = threadIdx.x; w < n; w += blockDim.x) {
-    const int t = list[w];
-    const int p = atomicAdd(&cur[rev_key(pos + 3 * (size_t)t, quat + 4 * (size_t)t)], 1);
-    out[p] = t;
-  }
-}
-
-hipError_t launch_rev_sort(const int* list, const int* n_dev, int n_cap, const double* pos,
-                           const double* quat, int* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_rev_sort, dim3(1), dim3(1024), 0, st, list, n_dev, n_cap, pos, quat, out);
-  return hipGetLastError();
-}
 
 // Several systems (one ligand each, sharing the receptor grids) in one
 // launch: workgroup b belongs to system l with start[l] <= b < start[l + 1]
@@ -595,8 +531,7 @@ hipError_t launch_forces_split(const ForceSys& S, const ForceWork& W, int precis
       W.lig_pos != nullptr || W.out_compact || W.rev_skip_mode != 0)
     return hipErrorInvalidValue;
   const int n_cap = W.n_cap > 0 ? (W.n_cap < max_n ? W.n_cap : max_n) : max_n;
-  hipError_t e = launch_rev_sort(W.list, W.n_dev, n_cap, W.pos, W.quat, W.rev_order, st);
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
   switch (G) {
     case 16:
       e = precision == 0 ? launch_split_g<double, 16>(S, W, max_n, blocks_cap, st, zp)

@@ -7,17 +7,13 @@ namespace pst {
 
 hipError_t launch_forces(const ForceSys& S, const ForceWork& W, int precision, int G, int max_n,
                          int blocks_cap, hipStream_t st, bool zp = false);
-// Split evaluation over a device-counted pose-mode list (G in {16, 32, 64}):
-// the reverse order (k_rev_sort into W.rev_order), the ligand-atom and
-// reverse-Born items in one launch (k_forces_split, corrections into W.F_rev /
-// W.T_rev), then F -= F_rev, T -= T_rev (k_rev_apply).  Bit-identical to
-// launch_forces with the same G.
+// Split evaluation over a device-counted pose-mode list (G in {16, 32, 64};
+// W.rev_order = the same trajectories ordered for the reverse-Born pass):
+// the ligand-atom and reverse-Born items in one launch (k_forces_split,
+// corrections into W.F_rev / W.T_rev), then F -= F_rev, T -= T_rev
+// (k_rev_apply).  Bit-identical to launch_forces with the same G.
 hipError_t launch_forces_split(const ForceSys& S, const ForceWork& W, int precision, int G,
                                int max_n, int blocks_cap, hipStream_t st, bool zp);
-// out[0 .. min(*n_dev, n_cap)) = list re-sorted by the ligand-frame direction
-// of the receptor (the reverse-Born pass's order).
-hipError_t launch_rev_sort(const int* list, const int* n_dev, int n_cap, const double* pos,
-                           const double* quat, int* out, hipStream_t st);
 hipError_t launch_chunk_min_r(const double* pos, const int* list, int n, int fb, int cs, int cpf,
                               double* out, int n_chunks, hipStream_t st);
 int pick_group(int M, int n_rec_active);
@@ -32,8 +28,8 @@ hipError_t launch_census(const ForceSys& S, const ForceWork& W, unsigned long lo
 // Empty marker kernel bracketing a timing window in a kernel trace.
 hipError_t launch_window_mark(hipStream_t st);
 // Gather ceiling: n points, the FP32 pair-grid stencil at each (pst_probe.hip).
-hipError_t launch_gather_probe(const GridDev& g, long long n, int mode, unsigned long long seed,
-                               float* out, int blocks, hipStream_t st);
+hipError_t launch_gather_probe(const GridDev& g, long long n, int mode, int variant,
+                               unsigned long long seed, float* out, int blocks, hipStream_t st);
 
 // Brick-binned force evaluation: request counts / order per brick of the
 // receptor float2 pair grid and of the ligand Born grid, and per-request results.
@@ -108,8 +104,12 @@ struct ObsBufs {
 struct ProdBufs {
   double* pos; double* quat; int* status; unsigned int* n_steps; long long* traj;
   int* plain_list; int* mlp_list;
-  int* key;                    // per-slot direction key of the grid-path list (-1: none)
-  int* bins;                   // [256] key histogram, [256..511] scatter cursors
+  int* rev_list;               // the grid-path list ordered for the reverse-Born pass
+  // per-slot step role: -1 not stepped, -2 surrogate list, else the grid-path
+  // direction key (bits 0-7) and the reverse-pass cell (bits 8-15)
+  int* key;
+  int* bins;                   // [256] key histogram, [256..511] scatter cursors,
+                               // [512..767] / [768..1023] the same for the reverse cells
   double* F; double* T; double* dt; double* x0; double* x1;
   RunCounters* ctr;
   signed char* out_status;     // per trajectory id (nullable)

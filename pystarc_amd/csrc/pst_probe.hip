@@ -10,7 +10,8 @@
 //           neighbouring stencils sharing bricks and lines in L2; higher k:
 //           tighter clusters, the locality a better ordering could reach).
 // Its rate is the ceiling a gather-bound force kernel can reach on the same
-// grids.
+// grids.  Fetch variants: 0 the plain stencil (20 single-node loads); 1 the
+// z-pair fetches of the force kernel's z-pair mode (16 aligned pair loads).
 #include <hip/hip_runtime.h>
 #include "pst_device.hpp"
 #include "pst_kernels.hpp"
@@ -25,6 +26,7 @@ __device__ inline double hash_unit(unsigned long long x) {
   return (double)(x >> 11) * (1.0 / 9007199254740992.0);
 }
 
+template <int VAR>
 __global__ void __launch_bounds__(256) k_gather_probe(GridDev g, long long n, int mode,
                                                       unsigned long long seed, float* out) {
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -46,14 +48,17 @@ __global__ void __launch_bounds__(256) k_gather_probe(GridDev g, long long n, in
       }
     }
     Pair<float> phi, gx, gy, gz;
-    grid_phi_grad_pair<float>(g, x[0], x[1], x[2], phi, gx, gy, gz);
+    grid_phi_grad_pair<float, VAR == 1>(g, x[0], x[1], x[2], phi, gx, gy, gz);
     out[p] = phi.a + phi.b + gx.a + gx.b + gy.a + gy.b + gz.a + gz.b;
   }
 }
 
-hipError_t launch_gather_probe(const GridDev& g, long long n, int mode, unsigned long long seed,
-                               float* out, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, st, g, n, mode, seed, out);
+hipError_t launch_gather_probe(const GridDev& g, long long n, int mode, int variant,
+                               unsigned long long seed, float* out, int blocks, hipStream_t st) {
+  if (variant == 1)
+    hipLaunchKernelGGL(k_gather_probe<1>, dim3(blocks), dim3(256), 0, st, g, n, mode, seed, out);
+  else
+    hipLaunchKernelGGL(k_gather_probe<0>, dim3(blocks), dim3(256), 0, st, g, n, mode, seed, out);
   return hipGetLastError();
 }
 

@@ -638,12 +638,16 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
   // Grid-path trajectories get a direction key for the locality sort: lab
   // octant of the centroid (bits 5-7, the receptor-grid region it gathers
   // from) and the octant + dominant axis of the receptor direction in the
-  // ligand frame (bits 0-4, the ligand-grid region of its reverse Born).
+  // ligand frame (bits 0-4, the ligand-grid region of its reverse Born); with
+  // a reverse list, the cube-map cell of that direction (bits 8-15) orders the
+  // split evaluation's reverse pass.  Surrogate slots get -2, the rest -1.
   __shared__ int hist[256];
+  __shared__ int hist2[256];
   hist[threadIdx.x] = 0;
+  hist2[threadIdx.x] = 0;
   __syncthreads();
   if (s < B.n_slots) {
-    int k = -1;
+    int k = live && far ? -2 : -1;
     if (live && !far) {
       const double* pos = B.pos + 3 * (size_t)s;
       double R[9];
@@ -659,11 +663,17 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
       const int lig = ((dx > 0) | ((dy > 0) << 1) | ((dz > 0) << 2)) * 3 + dom;
       k = lab * 32 + lig;
       atomicAdd(&hist[k], 1);
+      if (B.rev_list) {
+        const int c = dir_cell(dx, dy, dz);
+        atomicAdd(&hist2[c], 1);
+        k |= c << 8;
+      }
     }
     B.key[s] = k;
   }
   __syncthreads();
   if (hist[threadIdx.x]) atomicAdd(&B.bins[threadIdx.x], hist[threadIdx.x]);
+  if (hist2[threadIdx.x]) atomicAdd(&B.bins[512 + threadIdx.x], hist2[threadIdx.x]);
 }
 
 // Outer excursions of the step (LMZ mode), before k_prod_pre: a running
@@ -724,12 +734,12 @@ __global__ void __launch_bounds__(kStepBlock) k_prod_pre_group(const StepParams*
   prod_pre_body(Pg[l], Bg[l], par, blockIdx.x - start[l]);
 }
 
-// Exclusive scan of the key histogram into scatter cursors; sets the length
-// of the grid-path list and clears the histogram for the next step.
-__device__ inline void bins_scan_body(const ProdBufs& B) {
+// Exclusive scan of 256 histogram bins (h) into scatter cursors (h + 256);
+// clears the histogram for the next step and returns the total.
+__device__ inline int scan256(int* h) {
   __shared__ int sc[256];
   const int t = threadIdx.x;
-  const int v = B.bins[t];
+  const int v = h[t];
   sc[t] = v;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
@@ -738,9 +748,19 @@ __device__ inline void bins_scan_body(const ProdBufs& B) {
     sc[t] += add;
     __syncthreads();
   }
-  B.bins[256 + t] = sc[t] - v;
-  B.bins[t] = 0;
-  if (t == 255) B.ctr->n_sr = sc[255];
+  h[256 + t] = sc[t] - v;
+  h[t] = 0;
+  const int total = sc[255];
+  __syncthreads();
+  return total;
+}
+
+// Scatter cursors of the grid-path list (and of its reverse-pass order);
+// sets the length of the grid-path list.
+__device__ inline void bins_scan_body(const ProdBufs& B) {
+  const int n = scan256(B.bins);
+  if (B.rev_list) scan256(B.bins + 512);
+  if (threadIdx.x == 255) B.ctr->n_sr = n;
 }
 
 __global__ void __launch_bounds__(256) k_bins_scan(ProdBufs B) { bins_scan_body(B); }
@@ -755,8 +775,9 @@ __device__ inline void bins_scatter_body(const ProdBufs& B, int blk) {
   if (s >= B.n_slots) return;
   const int k = B.key[s];
   if (k < 0) return;
-  const int p = atomicAdd(&B.bins[256 + k], 1);
+  const int p = atomicAdd(&B.bins[256 + (k & 255)], 1);
   B.plain_list[p] = s;
+  if (B.rev_list) B.rev_list[atomicAdd(&B.bins[768 + (k >> 8)], 1)] = s;
 }
 
 __global__ void k_bins_scatter(ProdBufs B) { bins_scatter_body(B, blockIdx.x); }
@@ -795,12 +816,12 @@ __device__ inline void prod_post_body(const StepParams& P, const ForceSys& S, co
     for (int i = threadIdx.x; i < kObsHist; i += blockDim.x) h[i] = 0;
     __syncthreads();
   }
-  const int w = blk * blockDim.x + threadIdx.x;
-  const int n_plain = B.ctr->n_sr, n_mlp = B.ctr->n_mlp;
-  const bool active = w < n_plain + n_mlp;
+  // the slots of this step's grid-path and surrogate lists, in slot order
+  // (coalesced per-slot reads; the lists are in direction order)
+  const int s = blk * blockDim.x + threadIdx.x;
+  const bool active = s < B.n_slots && B.key[s] != -1;
   bool any = false;
   if (active) {
-    const int s = w < n_plain ? B.plain_list[w] : B.mlp_list[w - n_plain];
     double* pos = B.pos + 3 * (size_t)s;
     double* q = B.quat + 4 * (size_t)s;
     const long long t = B.traj[s];
@@ -912,12 +933,11 @@ hipError_t prod_flush_observables(const StepParams& P, const ProdBufs& B, hipStr
 
 // Bridge decision, n_steps += 1, finish + refill of bridged trajectories.
 __device__ inline void prod_bridge_body(const StepParams& P, const ProdBufs& B, int par, int blk) {
-  const int w = blk * blockDim.x + threadIdx.x;
+  // the slots of this step's lists, in slot order (as k_prod_post)
+  const int s = blk * blockDim.x + threadIdx.x;
   RunCounters* C = B.ctr;
-  const int n_plain = C->n_sr, n_mlp = C->n_mlp;
-  if (w == 0) atomicAdd(&C->total_steps, (unsigned long long)(n_plain + n_mlp));
-  if (w >= n_plain + n_mlp) return;
-  const int s = w < n_plain ? B.plain_list[w] : B.mlp_list[w - n_plain];
+  if (s == 0) atomicAdd(&C->total_steps, (unsigned long long)(C->n_sr + C->n_mlp));
+  if (s >= B.n_slots || B.key[s] == -1) return;
   const long long t = B.traj[s];
   const unsigned int ns = B.n_steps[s];
   bool react = false;

@@ -24,12 +24,14 @@ def test_more_than_one_gpu_exits_with_a_message():
 def test_traffic_summary_covers_every_path():
     sys.path.insert(0, ROOT)
     import bench
+    slots = json.load(open(TRAFFIC))["slots"]
     for path in ("surrogate", "fp32", "fp64"):
-        v = bench.measured_traffic("protein_protein", path)
+        v = bench.measured_traffic("protein_protein", path, slots)
         assert isinstance(v, int) and v > 0
-    assert bench.measured_traffic("protein_protein", "no_such_path") is None
-    # a summary measured on another workload is not reported for this one
-    assert bench.measured_traffic("protein_ligand", "surrogate") is None
+    assert bench.measured_traffic("protein_protein", "no_such_path", slots) is None
+    # a summary measured on another workload or slot count is not reported
+    assert bench.measured_traffic("protein_ligand", "surrogate", slots) is None
+    assert bench.measured_traffic("protein_protein", "surrogate", slots // 4) is None
 
 
 @pytest.mark.parametrize("path", ["surrogate", "fp32", "fp64"])
@@ -38,19 +40,19 @@ def test_traffic_bytes_follow_from_the_counters(path):
     e = d["paths"][path]
     # the copy calibration: FETCH_SIZE reads half of a 262,144 KiB copy's
     # reads, WRITE_SIZE all of its writes
-    expect = (e["fetch_kib_per_dispatch"] * 2 + e["write_kib_per_dispatch"]) * 1024
-    assert abs(e["hbm_bytes_per_launch"] - expect) <= 1024
+    expect = (e["fetch_kib_per_eval"] * 2 + e["write_kib_per_eval"]) * 1024
+    assert abs(e["hbm_bytes_per_eval"] - expect) <= 1024
     assert e["launches"] == 20
     assert all("rocprofv3 --pmc" in c and "bench.py --pmc" in c for c in e["commands"])
 
 
 def test_surrogate_traffic_exceeds_its_algorithmic_bytes():
-    # algorithmic bytes of one surrogate-window launch: ~4,100 trajectories x
-    # 104 B plus 80 B per atom-in-grid (5.6 M + 5.6 M + 6.8 M); line traffic
-    # cannot be below it
+    # algorithmic bytes of one surrogate-window evaluation at 262,144 slots:
+    # ~16,300 trajectories x 104 B plus 80 B per atom-in-grid (20.5 M + 20.5 M
+    # + 24.7 M in the counter runs); line traffic cannot be below it
     e = json.load(open(TRAFFIC))["paths"]["surrogate"]
-    alg = 4100 * 104 + (5.6e6 + 5.6e6 + 6.8e6) * 80
-    assert e["hbm_bytes_per_launch"] > alg
+    alg = 16300 * 104 + (20.5e6 + 20.5e6 + 24.7e6) * 80
+    assert e["hbm_bytes_per_eval"] > alg
 
 
 def test_cpu_model_and_workload_text():
@@ -62,3 +64,14 @@ def test_cpu_model_and_workload_text():
     assert "3220/18" in bench.workload_text(a)
     a = argparse.Namespace(workload="protein_protein", hydro="rpy12")
     assert "1700/1400" in bench.workload_text(a) and "RPY" in bench.workload_text(a)
+
+
+def test_two_sample_z():
+    sys.path.insert(0, ROOT)
+    import bench
+    a = {"reacted": 20, "escaped": 80, "P": 0.2}
+    b = {"reacted": 10, "escaped": 90, "P": 0.1}
+    # pooled p = 0.15, se = sqrt(0.15 * 0.85 * (1/100 + 1/100)) = 0.050498
+    assert abs(bench.two_sample_z(a, b) - 0.1 / 0.0504975) < 1e-4
+    assert abs(bench.two_sample_z(b, a) + bench.two_sample_z(a, b)) < 1e-12
+    assert bench.two_sample_z(a, a) == 0.0
