@@ -4,7 +4,6 @@ pystarc/forces/electrostatic/grid_force.py:90-145 reads them: ``counts``,
 ``data`` (nx, ny, nz) in C order."""
 from __future__ import annotations
 
-import re
 from pathlib import Path
 
 import numpy as np
@@ -22,43 +21,39 @@ class DXGrid:
 
     @classmethod
     def from_file(cls, path):
+        """Header lines anywhere before the data (``object 1 ... counts``,
+        ``origin``, three ``delta`` rows); values only inside the array object
+        (after the ``object 3`` line, up to the next ``object`` or
+        ``attribute`` line), as grid_force.py:110-143 reads them.  A value
+        count other than nx ny nz is an error, as the reference's reshape
+        makes it."""
         counts = origin = None
         deltas = []
         values = []
-        n_items = None
+        in_data = False
         with open(Path(path), "r") as fh:
             for line in fh:
                 s = line.strip()
                 if not s or s.startswith("#"):
                     continue
-                if s.startswith("object"):
-                    m = re.search(r"gridpositions\s+counts\s+(\d+)\s+(\d+)\s+(\d+)", s)
-                    if m:
-                        counts = tuple(int(v) for v in m.groups())
-                    m = re.search(r"items\s+(\d+)", s)
-                    if m:
-                        n_items = int(m.group(1))
-                    continue
-                if s.startswith("origin"):
-                    origin = [float(v) for v in s.split()[1:4]]
-                    continue
-                if s.startswith("delta"):
-                    deltas.append([float(v) for v in s.split()[1:4]])
-                    continue
-                if s.startswith("attribute") or s.startswith("component"):
-                    continue
-                try:
+                if in_data:
+                    if s.startswith("object") or s.startswith("attribute"):
+                        break
                     values.extend(float(v) for v in s.split())
-                except ValueError:
-                    continue
+                elif s.startswith("object 1"):
+                    counts = tuple(int(v) for v in s.split()[-3:])
+                elif s.startswith("origin"):
+                    origin = [float(v) for v in s.split()[1:4]]
+                elif s.startswith("delta"):
+                    deltas.append([float(v) for v in s.split()[1:4]])
+                elif s.startswith("object 3"):
+                    in_data = True
         if counts is None or origin is None or len(deltas) != 3:
             raise ValueError(f"{path}: not an OpenDX gridpositions file")
         n = counts[0] * counts[1] * counts[2]
-        if n_items is not None and n_items != n:
-            raise ValueError(f"{path}: items {n_items} != counts product {n}")
-        if len(values) < n:
+        if len(values) != n:
             raise ValueError(f"{path}: {len(values)} values for {n} grid points")
-        data = np.asarray(values[:n], dtype=np.float64).reshape(counts)
+        data = np.asarray(values, dtype=np.float64).reshape(counts)
         return cls(origin, np.array(deltas), data)
 
     def to_file(self, path, fmt="%.6e"):

@@ -28,8 +28,9 @@ Keyword-only extensions (defaults keep the reference's physics):
   surrogate    a trained pystarc_amd.surrogate.Surrogate: trajectories beyond
                its contact shell take the MLP force (philox mode only).
   audit_n      surrogate trajectories re-evaluated on the grid path per step.
-  audit_tol    relative RMS force error above which the run falls back to
-               the grid path for every trajectory.
+  audit_tol    relative RMS force (or torque) error of a window of at least
+               AUDIT_MIN_POSES audited poses above which the run falls back
+               to the grid path for every trajectory.
   autotune     False (default) or the number of warm-up steps (True: 64)
                after which the production force launch (direct or
                brick-binned evaluation, lanes per trajectory, workgroup cap,
@@ -94,9 +95,16 @@ log = logging.getLogger("pystarc_amd")
 # the reference checks for stalled runs every 10,000 loop steps
 # (gpu_batch_simulator.py:1349-1356)
 STALL_EVERY = 10_000
-# stated relative RMS force error of the surrogate: above it (audit per chunk
-# of steps) the grid path takes over every trajectory
+# stated relative RMS force error of the surrogate: above it (audit per
+# window of at least AUDIT_MIN_POSES audited poses) the grid path takes over
+# every trajectory
 AUDIT_TOL = 0.25
+# fewest audited poses behind one fallback decision.  A chunk of steps in the
+# run's tail, when few trajectories are left beyond the surrogate radius,
+# audits only a handful of poses, whose relative RMS error scatters far
+# around the surrogate's real error; such chunks are pooled until the window
+# holds this many.
+AUDIT_MIN_POSES = 2048
 
 
 def _z_quantile(p):
@@ -307,6 +315,28 @@ def stall_stop(n_total, done, done_before):
     every STALL_EVERY steps: stop when more than 95 % of the trajectories are
     done and fewer than max(2, 0.1 % of them) finished since the last check."""
     return done > int(n_total * 0.95) and done - done_before < max(2, int(n_total * 0.001))
+
+
+class AuditWindow:
+    """Pools the surrogate audit's sums (squared force error, squared force,
+    audited poses, squared torque error, squared torque) over chunks of steps
+    until at least `min_poses` poses are in, then yields the window's relative
+    RMS force and torque errors for one fallback decision and starts a new
+    window."""
+
+    def __init__(self, min_poses=AUDIT_MIN_POSES, sums=None):
+        self.min_poses = int(min_poses)
+        self.sums = np.zeros(5) if sums is None else np.asarray(sums, dtype=np.float64).copy()
+
+    def add(self, err2, ref2, n, terr2=0.0, tref2=0.0):
+        """Adds one chunk; returns (force error, torque error) when the window
+        is complete (torque error 0 without audited torques), else None."""
+        self.sums += (err2, ref2, n, terr2, tref2)
+        e2, r2, k, t2, tr2 = self.sums
+        if k < self.min_poses or r2 <= 0:
+            return None
+        self.sums[:] = 0.0
+        return math.sqrt(e2 / r2), (math.sqrt(t2 / tr2) if tr2 > 0 else 0.0)
 
 
 def is_stall_check(step, every=STALL_EVERY):
@@ -758,6 +788,7 @@ class GPUBatchSimulator:
                 surr_on = True
             audit = {"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None,
                      "terr2": 0.0, "tref2": 0.0, "worst_torque": 0.0}
+            window = AuditWindow()
             c16 = np.zeros(16)
             steps = stall_done = last_ckpt = 0
             conv_done = 0
@@ -777,6 +808,8 @@ class GPUBatchSimulator:
                                  worst=float(a[3]), terr2=float(a[4]), tref2=float(a[5]),
                                  worst_torque=float(a[6]),
                                  fallback_step=None if a[7] < 0 else int(a[7]))
+                    if len(a) >= 13:
+                        window = AuditWindow(sums=a[8:13])
                     if surr_on and audit["fallback_step"] is not None:
                         ctx.check(lib.pst_sim_set_surrogate(h, None, None), "pst_sim_set_surrogate")
                         surr_on = False
@@ -827,16 +860,16 @@ class GPUBatchSimulator:
                 if surr_on and c16[13] > 0:
                     a6 = np.zeros(6)
                     ctx.check(lib.pst_prod_audit(h, ptr(a6)), "pst_prod_audit")
-                    err = math.sqrt(a6[0] / a6[1])
-                    terr = math.sqrt(a6[3] / a6[4]) if a6[4] > 0 else 0.0
                     audit["err2"] += a6[0]
                     audit["ref2"] += a6[1]
                     audit["n"] += int(a6[2])
                     audit["terr2"] += a6[3]
                     audit["tref2"] += a6[4]
+                    ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
+                    verdict = window.add(a6[0], a6[1], a6[2], a6[3], a6[4])
+                    err, terr = verdict if verdict is not None else (0.0, 0.0)
                     audit["worst"] = max(audit["worst"], err)
                     audit["worst_torque"] = max(audit["worst_torque"], terr)
-                    ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
                     if err > self.audit_tol or terr > self.audit_tol:
                         # the grid path takes over every trajectory from here on
                         ctx.check(lib.pst_sim_set_surrogate(h, None, None), "pst_sim_set_surrogate")
@@ -859,7 +892,7 @@ class GPUBatchSimulator:
                 if ckpt_every > 0 and done - last_ckpt >= ckpt_every:
                     last_ckpt = done
                     ckpt.save(lib, ctx, h, self._ckpt_host(steps, stall_done, last_ckpt, conv_done,
-                                                           audit, paths, energies, conv))
+                                                           audit, paths, energies, conv, window))
                     log.info("checkpoint saved (%d/%d done) -> %s", done, N, ckpt.path)
                 if c16[9]:
                     stop_reason = "done_99.5"
@@ -884,7 +917,7 @@ class GPUBatchSimulator:
             if ckpt_every > 0 and interrupted:
                 ctx.check(lib.pst_prod_counters(h, ptr(c16)), "pst_prod_counters")
                 ckpt.save(lib, ctx, h, self._ckpt_host(steps, stall_done, last_ckpt, conv_done,
-                                                       audit, paths, energies, conv))
+                                                       audit, paths, energies, conv, window))
             elif not interrupted and ckpt_every > 0:
                 ckpt.remove()
             n_r, n_e = int(c16[0]), int(c16[1])
@@ -943,10 +976,14 @@ class GPUBatchSimulator:
         return dict(zip(_CONV_KEYS, [float(milestone), float(done), pct, P_live, k_live,
                                      float(n_r), float(n_e), float(steps), elapsed]))
 
-    def _ckpt_host(self, steps, stall_done, last_ckpt, conv_done, audit, paths, energies, conv):
+    def _ckpt_host(self, steps, stall_done, last_ckpt, conv_done, audit, paths, energies, conv,
+                   window=None):
         a = [audit["err2"], audit["ref2"], audit["n"], audit["worst"], audit["terr2"],
              audit["tref2"], audit["worst_torque"],
              -1 if audit["fallback_step"] is None else audit["fallback_step"]]
+        # the open audit window, so a resumed run decides its fallback where
+        # the uninterrupted run does
+        a += list(window.sums) if window is not None else [0.0] * 5
         host = {"loop": np.array([steps, stall_done, last_ckpt, conv_done], dtype=np.int64),
                 "audit": np.array(a, dtype=np.float64)}
         if paths:
@@ -1172,7 +1209,8 @@ class GPUBatchGroup:
     ``run()`` returns one GPUBatchResult per ligand.  Philox mode; every
     simulator must use the same force precision.  Each ligand may carry its
     own surrogate: its audit is read after every chunk of steps and, when the
-    chunk's relative RMS force error exceeds that simulator's audit_tol, the
+    relative RMS force error of a window of at least AUDIT_MIN_POSES audited
+    poses exceeds that simulator's audit_tol, the
     grid path takes over that ligand's trajectories from then on (the other
     ligands keep theirs), as in a single run."""
 
@@ -1258,6 +1296,7 @@ class GPUBatchGroup:
             surr_on = [s.surrogate is not None for s in sims]
             audits = [{"err2": 0.0, "ref2": 0.0, "n": 0, "worst": 0.0, "fallback_step": None}
                       for _ in sims]
+            windows = [AuditWindow() for _ in sims]
             stall_done = [0] * L
             stalled = [False] * L
             t0 = time.time()
@@ -1279,12 +1318,14 @@ class GPUBatchGroup:
                     s = sims[l]
                     if not surr_on[l] or c16[l, 13] <= 0:
                         continue
-                    a, err = audits[l], math.sqrt(c16[l, 12] / c16[l, 13])
+                    a = audits[l]
                     a["err2"] += c16[l, 12]
                     a["ref2"] += c16[l, 13]
                     a["n"] += int(c16[l, 14])
-                    a["worst"] = max(a["worst"], err)
                     ctx.check(lib.pst_sim_reset_audit(handles[l]), "pst_sim_reset_audit")
+                    verdict = windows[l].add(c16[l, 12], c16[l, 13], c16[l, 14])
+                    err = verdict[0] if verdict is not None else 0.0
+                    a["worst"] = max(a["worst"], err)
                     if err > s.audit_tol:
                         ctx.check(lib.pst_sim_set_surrogate(handles[l], None, None),
                                   "pst_sim_set_surrogate")

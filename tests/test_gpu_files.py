@@ -89,3 +89,40 @@ def test_two_spheres_example_reproduces_the_published_p_rxn():
           f"k_b={res.k_b:.3f}, {res.n_completed} completed", flush=True)
     assert res.n_completed >= int(0.995 * 100_000)
     assert lo <= 0.4501 <= hi
+
+
+def test_engine_from_dx_files_equals_in_memory_grids(tmp_path):
+    """Grids written as OpenDX files and read back (the reference's input
+    path for APBS grids, grid_force.py:110-143) give the engine the same
+    forces, torques and energies bit for bit as the in-memory grids."""
+    from oracle.physics import quats_to_rotmats, shoemake
+    from pystarc_amd.forces.electrostatic.grid_force import DXGrid
+    from tests._systems import small_protein_ligand
+    sysm = small_protein_ligand(n_traj=64)
+    mem = dx_grids(numpy_grids(sysm))
+    files = {}
+    for kind, lst in mem.items():
+        files[kind] = []
+        for i, g in enumerate(lst):
+            p = tmp_path / f"{kind}{i}.dx"
+            g.to_file(p, fmt="%.17g")
+            files[kind].append(DXGrid.from_file(p))
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(64, 3))
+    c = d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(6, 20, (64, 1))
+    R = quats_to_rotmats(shoemake(rng.random((64, 3))))
+    body = sysm.mol2.positions_array() - sysm.mol2.centroid()
+    lab = np.einsum("nij,aj->nai", R, body) + c[:, None, :]
+    # files hold FP64 values (exactly the FP32 grid values here); the engine
+    # stores both sets in the in-memory grids' type
+    gdt = mem["elec"][0].data.dtype
+    assert all(g.data.dtype == np.float64 for lst in files.values() for g in lst)
+    out = []
+    for grids in (mem, files):
+        eng = S.make_engine(sysm, grids, torque_mode="physical", grid_dtype=gdt)
+        F_, T_, E_ = eng(lab, sysm.mol2.charges_array(), R_matrices=R, centroids=c)
+        torch.cuda.synchronize()
+        out.append([x.cpu().numpy() for x in (F_, T_, E_)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    assert np.abs(out[0][0]).max() > 0

@@ -121,3 +121,30 @@ def test_checkpoint_host_entries_roundtrip():
     back = G._split_rows(host["paths"], host["paths_n"])
     assert len(back) == 2 and np.array_equal(back[0], paths[0]) and np.array_equal(back[1], paths[1])
     assert dict(zip(G._CONV_KEYS, host["conv"][0].tolist())) == conv[0]
+
+
+def test_audit_window_pools_small_chunks_before_deciding():
+    w = G.AuditWindow(min_poses=100)
+    # tail chunks: 10 poses each with a large error, no decision until 100 poses
+    for _ in range(9):
+        assert w.add(err2=4.0, ref2=10.0, n=10) is None
+    err, terr = w.add(err2=4.0, ref2=10.0, n=10)
+    # pooled error sqrt(40 / 100), not any single chunk's
+    assert err == pytest.approx(np.sqrt(0.4)) and terr == 0.0
+    # the window starts over after a decision
+    assert w.add(err2=1.0, ref2=100.0, n=50) is None
+    err, terr = w.add(err2=1.0, ref2=100.0, n=50, terr2=9.0, tref2=100.0)
+    assert err == pytest.approx(np.sqrt(2.0 / 200.0)) and terr == pytest.approx(0.3)
+
+
+def test_audit_window_one_large_chunk_decides_at_once():
+    # a steady-state chunk (640 steps x 64 audited poses) is a window by itself
+    err, _ = G.AuditWindow().add(err2=1.0, ref2=100.0, n=640 * 64)
+    assert err == pytest.approx(0.1)
+
+
+def test_audit_window_resumes_from_its_sums():
+    a = G.AuditWindow(min_poses=100)
+    a.add(2.0, 10.0, 60)
+    b = G.AuditWindow(min_poses=100, sums=a.sums)
+    assert a.add(2.0, 10.0, 60) == b.add(2.0, 10.0, 60)
