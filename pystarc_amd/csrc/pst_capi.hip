@@ -1358,11 +1358,19 @@ int pst_replay_forces(pst_sim* s, double* F_host, int n_sr) {
 }
 
 // ---- production ------------------------------------------------------------
+// Whether the sim's force evaluation is the split one (modes 3 / 4): its
+// pre-phase then sorts the ligand-atom list and the reverse-Born order by
+// keys of their own.
+static bool sim_split(const pst_sim* s) {
+  const int mode = s->tune_mode >= 0 ? s->tune_mode : s->ctx->gather_mode;
+  return mode == 3 || mode == 4;
+}
+
 static ProdBufs prod_bufs(pst_sim* s) {
   ProdBufs B;
   B.pos = s->pos; B.quat = s->quat; B.status = s->status; B.n_steps = s->n_steps32;
   B.traj = s->traj; B.plain_list = s->plain_list; B.mlp_list = s->mlp_list;
-  B.key = s->key; B.bins = s->bins; B.rev_list = s->rev_list;
+  B.key = s->key; B.bins = s->bins; B.rev_list = sim_split(s) ? s->rev_list : nullptr;
   B.F = s->F; B.T = s->T; B.dt = s->dt; B.x0 = s->x0; B.x1 = s->x1;
   B.ctr = s->ctr; B.out_status = s->out_status; B.out_steps = s->out_steps;
   B.obs = s->obs;
@@ -1973,7 +1981,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     const int Sp = (sl->S + 63) / 64 * 64;
     surr_l.push_back(l);
     MI.push_back(mlp_infer_args(sl->mlp->m, Sp, &sl->ctr->n_mlp, sl->mlp->m.y));
-    mstart.push_back(mstart.back() + Sp / 64);
+    mstart.push_back(mstart.back() + (Sp + kInferRows - 1) / kInferRows);
   }
   const int n_surr = (int)surr_l.size();
   ForceSys* dS = nullptr;
@@ -2415,7 +2423,7 @@ int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out) {
   p->ctx = ctx;
   *out = p;
   MlpBufs& m = p->m;
-  const size_t R = (size_t)(max_rows + 63) / 64 * 64;
+  const size_t R = (size_t)(max_rows + kInferRows - 1) / kInferRows * kInferRows;
   m.max_rows = (int)R;
   const size_t H = kMlpWidth;
   auto al = [&](void** dst, size_t bytes) -> hipError_t {

@@ -30,9 +30,13 @@ namespace pst {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU z sigmoid(z) with the hardware reciprocal (1 ulp) in place of an
+// IEEE division: the activation epilogues, not the MFMAs, bound the fused
+// inference, and the division's correction sequence doubled their cost.
+__device__ inline float sigmoid_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu_f(float z) { return z * sigmoid_f(z); }
 __device__ inline float dsilu_f(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid_f(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -182,121 +186,144 @@ static hipError_t gemm(const GemmArgs& g, hipStream_t st, int splits = 1) {
 
 // ---- fused inference forward ---------------------------------------------------------
 // All five layers of the inference forward in one kernel: a 256-thread
-// workgroup owns 64 rows and keeps their activations in LDS as BF16
-// [64][256 + 8] (one buffer: a layer's outputs wait in registers until every
-// wave has read its inputs).  Wave w computes output columns 64w..64w+63 of a
-// hidden layer as a 4 x 4 block of 16x16x32 MFMA tiles per 32-wide k step;
-// A fragments come from LDS, B fragments (weights, [out][in] with k
-// contiguous, shared by every workgroup) straight from L2 / L1 with the next
-// k step's fragments loaded while the current step's MFMAs run.  Layer 0
-// runs one k step (the 9 inputs padded to the MFMA's K of 32, not 64), the
-// output layer one n tile (13 outputs padded to 16, one m tile per wave).
-// The MFMA sequence (k order, fragments, rounding of every activation to
-// BF16 after bias + SiLU in FP32) is that of the layer GEMMs, so outputs are
-// the same bit for bit; rows past *m_dev skip their 64-row tile.
+// workgroup owns kInferRows (128) rows and keeps their activations in LDS as
+// BF16 [128][256 + 8] (one buffer: a layer's outputs wait in registers until
+// every wave has read its inputs).  Wave w computes neurons 64w..64w+63 of a
+// hidden layer for all 128 rows as an 8 x 4 block of 16x16x32 MFMA tiles per
+// 32-wide k step, with the weights as the A operand and the activations as
+// the B operand (the transposed product W A^T): a lane's four results are
+// then four consecutive neurons of one row, stored as one 8-byte LDS write
+// instead of four 2-byte ones.  B fragments come from LDS; A fragments
+// (weights, [out][in] with k contiguous, shared by every workgroup) straight
+// from L2 / L1, issued two k steps ahead of the MFMAs that consume them,
+// across layer boundaries too.  128 rows per workgroup read each weight once
+// per 128 rows; two workgroups per CU (256 VGPRs) let one's SiLU epilogue
+// run beside the other's MFMAs.  Layer 0 runs on the FP32 units (thread t:
+// neurons 4 (t % 64) .. + 3 of 32 rows), the output layer as one 16-neuron
+// tile (13 outputs), two row tiles per wave.  Each output is the same sum of
+// the same products in the same k order as in the layer GEMMs, and every
+// activation is rounded to BF16 after bias + SiLU in FP32 by the same
+// functions, so outputs are the same bit for bit; rows past *m_dev skip
+// their tile.  Row buffers are allocated in multiples of kInferRows, so a
+// tile never leaves them.
 constexpr int kActLd = kMlpWidth + 8;
+constexpr int kInferMT = kInferRows / 16;   // 16-row tiles per workgroup
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline void act_store(__bf16 (*act)[kActLd], const f32x4 (&acc)[4][4],
-                                 const float (&bias)[4], int wn, int lane) {
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        act[mt * 16 + (lane >> 4) * 4 + i][wn + nt * 16 + (lane & 15)] =
-            (__bf16)silu_f(acc[mt][nt][i] + bias[nt]);
-}
-
-// Tile `tile` (rows 64 tile .. 64 tile + 63) of one network's inference.
+// Tile `tile` (rows kInferRows tile .. + kInferRows - 1) of one network's inference.
 __device__ __forceinline__ void mlp_infer_tile(const MlpInfer& p, int tile) {
-  __shared__ __attribute__((aligned(16))) __bf16 act[64][kActLd];
-  const int tile_m = tile * 64;
+  __shared__ __attribute__((aligned(16))) __bf16 act[kInferRows][kActLd];
+  __shared__ float xs[kInferRows][kMlpIn];
+  const int tile_m = tile * kInferRows;
   const int n_rows = p.m_dev ? *p.m_dev : p.rows;
   if (tile_m >= n_rows) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r = lane & 15, kq = (lane >> 4) * 8;
+  const int r = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
   const int wn = wave * 64;
-  f32x4 acc[4][4];
-  float bias[4];
-  bf16x8 a[4], b[4], bn[4];
-  (void)n_rows;
-  // layer 0 on the FP32 units: lane l of wave w computes column 64w + l for
-  // the tile's 64 rows (features staged once in LDS, read as broadcasts)
-  __shared__ float xs[64][kMlpIn];
-  for (int i = threadIdx.x; i < 64 * kMlpIn; i += 256)
-    xs[i / kMlpIn][i % kMlpIn] = p.x0f[(size_t)(tile_m + i / kMlpIn) * kMlpInLd + i % kMlpIn];
-  {
-    const int n = wn + lane;
-    float w[kMlpIn];
-#pragma unroll
-    for (int k = 0; k < kMlpIn; ++k) w[k] = p.w0f[n * kMlpIn + k];
-    const float bn0 = p.b0[n];
-    __syncthreads();
-#pragma unroll 4
-    for (int row = 0; row < 64; ++row) act[row][n] = (__bf16)silu_f(layer0_z(xs[row], w, bn0));
-  }
-  __syncthreads();
-  // hidden layers 1..3: [64][256] x [256][256]^T
   const __bf16* ws[3] = {p.w1, p.w2, p.w3};
   const float* bs[3] = {p.b1, p.b2, p.b3};
-#pragma unroll 1
-  for (int l = 0; l < 3; ++l) {
-    const __bf16* W = ws[l];
+  // weight fragments of the 24 (layer, k step) pairs of the hidden layers; at
+  // most three are live at once (the one in use and two in flight)
+  bf16x8 wf[24][4];
+  auto load_w = [&](int step) {
+    const __bf16* W = ws[step >> 3];
+    const int k0 = (step & 7) * 32;
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      b[nt] = *(const bf16x8*)&W[(size_t)(wn + nt * 16 + r) * kMlpWidth + kq];
-      bias[nt] = bs[l][wn + nt * 16 + r];
+    for (int nt = 0; nt < 4; ++nt)
+      wf[step][nt] = *(const bf16x8*)&W[(size_t)(wn + nt * 16 + r) * kMlpWidth + k0 + kq];
+  };
+  load_w(0);
+  load_w(1);
+  // layer 0 on the FP32 units (features staged once in LDS, read as broadcasts)
+  for (int i = threadIdx.x; i < kInferRows * kMlpIn; i += 256)
+    xs[i / kMlpIn][i % kMlpIn] = p.x0f[(size_t)(tile_m + i / kMlpIn) * kMlpInLd + i % kMlpIn];
+  {
+    const int n0 = 4 * lane, rb = wave * (kInferRows / 4);
+    float w[4][kMlpIn], bn0[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int k = 0; k < kMlpIn; ++k) w[j][k] = p.w0f[(n0 + j) * kMlpIn + k];
+      bn0[j] = p.b0[n0 + j];
     }
+    __syncthreads();
+#pragma unroll 2
+    for (int row = rb; row < rb + kInferRows / 4; ++row) {
+      bf16x4 v;
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+      for (int j = 0; j < 4; ++j) v[j] = (__bf16)silu_f(layer0_z(xs[row], w[j], bn0[j]));
+      *(bf16x4*)&act[row][n0] = v;
+    }
+  }
+  __syncthreads();
+  // hidden layers 1..3: [256][256] x [128][256]^T
+  f32x4 acc[kInferMT][4];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    float bias[4][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) bias[nt][i] = bs[l][wn + nt * 16 + q4 + i];
+#pragma unroll
+    for (int mt = 0; mt < kInferMT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < kMlpWidth / 32; ++ks) {
-      const int k0 = ks * 32;
+      const int step = l * 8 + ks;
+      if (step + 2 < 24) load_w(step + 2);
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) a[mt] = *(const bf16x8*)&act[mt * 16 + r][k0 + kq];
-      if (ks + 1 < kMlpWidth / 32) {
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-          bn[nt] = *(const bf16x8*)&W[(size_t)(wn + nt * 16 + r) * kMlpWidth + k0 + 32 + kq];
-      }
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt)
+      for (int mt = 0; mt < kInferMT; ++mt) {
+        const bf16x8 a = *(const bf16x8*)&act[mt * 16 + r][ks * 32 + kq];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
-      if (ks + 1 < kMlpWidth / 32) {
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) b[nt] = bn[nt];
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[step][nt], a, acc[mt][nt], 0, 0, 0);
       }
     }
     __syncthreads();   // every wave has read this layer's inputs
-    act_store(act, acc, bias, wn, lane);
+#pragma unroll
+    for (int mt = 0; mt < kInferMT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        bf16x4 v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = (__bf16)silu_f(acc[mt][nt][i] + bias[nt][i]);
+        *(bf16x4*)&act[mt * 16 + r][wn + nt * 16 + q4] = v;
+      }
     __syncthreads();
   }
-  // output layer: one 16-column n tile (13 outputs), wave w takes rows 16w..16w+15
-  f32x4 o = (f32x4){0.f, 0.f, 0.f, 0.f};
+  // output layer: one 16-neuron tile (13 outputs); wave w takes row tiles
+  // 2w and 2w + 1
+  constexpr int kOutMT = kInferMT / 4;
+  f32x4 o[kOutMT];
+#pragma unroll
+  for (int j = 0; j < kOutMT; ++j) o[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < kMlpWidth / 32; ++ks) {
-    const bf16x8 av = *(const bf16x8*)&act[wave * 16 + r][ks * 32 + kq];
-    const bf16x8 bv = *(const bf16x8*)&p.w4[(size_t)r * kMlpWidth + ks * 32 + kq];
-    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, o, 0, 0, 0);
-  }
-  const float b4 = p.b4[lane & 15];
+    const bf16x8 wv = *(const bf16x8*)&p.w4[(size_t)r * kMlpWidth + ks * 32 + kq];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int row = tile_m + wave * 16 + (lane >> 4) * 4 + i;
-    p.y[(size_t)row * 64 + (lane & 15)] = o[i] + b4;
+    for (int j = 0; j < kOutMT; ++j) {
+      const bf16x8 av = *(const bf16x8*)&act[(wave * kOutMT + j) * 16 + r][ks * 32 + kq];
+      o[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, av, o[j], 0, 0, 0);
+    }
+  }
+  float b4[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b4[i] = p.b4[q4 + i];
+#pragma unroll
+  for (int j = 0; j < kOutMT; ++j) {
+    const int row = tile_m + (wave * kOutMT + j) * 16 + r;
+    const float4 v = {o[j][0] + b4[0], o[j][1] + b4[1], o[j][2] + b4[2], o[j][3] + b4[3]};
+    *(float4*)&p.y[(size_t)row * 64 + q4] = v;
   }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpInfer p) { mlp_infer_tile(p, blockIdx.x); }
+__global__ void __launch_bounds__(256, 2) k_mlp_infer(MlpInfer p) { mlp_infer_tile(p, blockIdx.x); }
 
 // Several networks (one per ligand) in one launch: workgroup b runs tile
 // b - start[l] of network l for start[l] <= b < start[l + 1].
-__global__ void __launch_bounds__(256) k_mlp_infer_group(const MlpInfer* __restrict__ P,
+__global__ void __launch_bounds__(256, 2) k_mlp_infer_group(const MlpInfer* __restrict__ P,
                                                          const int* __restrict__ start, int L) {
   const int b = blockIdx.x;
   int l = 0;
@@ -448,7 +475,7 @@ hipError_t mlp_cast(MlpBufs& m, const float* params, hipStream_t st) {
 hipError_t mlp_forward_packed(const MlpBufs& m, int Bp, const int* m_dev, bool train, float* y,
                               hipStream_t st) {
   if (!train) {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(Bp / 64), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_mlp_infer, dim3((Bp + kInferRows - 1) / kInferRows), dim3(256), 0, st,
                        mlp_infer_args(m, Bp, m_dev, y));
     return hipGetLastError();
   }

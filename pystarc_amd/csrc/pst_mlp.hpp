@@ -21,9 +21,12 @@ constexpr int kOffB4 = kOffW4 + kMlpOut * kMlpWidth;
 constexpr int kMlpParams = kOffB4 + kMlpOut;  // 203,277
 
 constexpr int kMlpInLd = 16;   // FP32 feature row stride (9 used)
+// rows of one fused-inference tile; every row buffer (features, outputs) is
+// allocated in a multiple of it
+constexpr int kInferRows = 128;
 
 struct MlpBufs {
-  int max_rows;  // padded to 64
+  int max_rows;  // padded to kInferRows
   __bf16 *w0, *w1, *w2, *w3, *w4;
   float* w0f;                 // [256][9] FP32 first-layer weights
   float* x0f;                 // [rows][16] FP32 features (9 used)
@@ -50,7 +53,7 @@ struct MlpInfer {
 };
 MlpInfer mlp_infer_args(const MlpBufs& m, int Bp, const int* m_dev, float* y);
 // L networks' inference in one launch: device descriptors P[L], workgroup
-// partition start[L + 1] (64-row tiles), blocks = start[L].
+// partition start[L + 1] (kInferRows-row tiles), blocks = start[L].
 hipError_t launch_mlp_infer_group(const MlpInfer* P, const int* start, int L, int blocks,
                                   hipStream_t st);
 

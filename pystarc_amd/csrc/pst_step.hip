@@ -635,12 +635,16 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
   }
   RunCounters* Cw = B.ctr;
   wave_append(B.mlp_list, &Cw->n_mlp, s, live && far);
-  // Grid-path trajectories get a direction key for the locality sort: lab
-  // octant of the centroid (bits 5-7, the receptor-grid region it gathers
-  // from) and the octant + dominant axis of the receptor direction in the
-  // ligand frame (bits 0-4, the ligand-grid region of its reverse Born); with
-  // a reverse list, the cube-map cell of that direction (bits 8-15) orders the
-  // split evaluation's reverse pass.  Surrogate slots get -2, the rest -1.
+  // Grid-path trajectories get a direction key for the locality sort.  For
+  // the fused force kernel (no reverse list): lab octant of the centroid
+  // (bits 5-7, the receptor-grid region it gathers from) and the octant +
+  // dominant axis of the receptor direction in the ligand frame (bits 0-4,
+  // the ligand-grid region of its reverse Born).  For the split evaluation
+  // the two passes get keys of their own: the cube-map cell of the centroid
+  // direction in the lab frame (bits 0-7) orders the ligand-atom pass, whose
+  // gathers depend only on where the ligand sits in the receptor grids, and
+  // the cube-map cell of the receptor direction in the ligand frame (bits
+  // 8-15) the reverse pass.  Surrogate slots get -2, the rest -1.
   __shared__ int hist[256];
   __shared__ int hist2[256];
   hist[threadIdx.x] = 0;
@@ -657,17 +661,19 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
       const double dx = -(R[0] * cx + R[3] * cy + R[6] * cz);
       const double dy = -(R[1] * cx + R[4] * cy + R[7] * cz);
       const double dz = -(R[2] * cx + R[5] * cy + R[8] * cz);
-      const int lab = (cx > 0) | ((cy > 0) << 1) | ((cz > 0) << 2);
-      const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
-      const int dom = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
-      const int lig = ((dx > 0) | ((dy > 0) << 1) | ((dz > 0) << 2)) * 3 + dom;
-      k = lab * 32 + lig;
-      atomicAdd(&hist[k], 1);
       if (B.rev_list) {
+        k = dir_cell(cx, cy, cz);
         const int c = dir_cell(dx, dy, dz);
         atomicAdd(&hist2[c], 1);
         k |= c << 8;
+      } else {
+        const int lab = (cx > 0) | ((cy > 0) << 1) | ((cz > 0) << 2);
+        const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+        const int dom = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+        const int lig = ((dx > 0) | ((dy > 0) << 1) | ((dz > 0) << 2)) * 3 + dom;
+        k = lab * 32 + lig;
       }
+      atomicAdd(&hist[k & 255], 1);
     }
     B.key[s] = k;
   }

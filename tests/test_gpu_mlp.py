@@ -113,11 +113,13 @@ def test_training_fits_a_smooth_map():
     s.close()
 
 
-@pytest.mark.parametrize("B", [1, 63, 64, 1000, 4096])
+@pytest.mark.parametrize("B", [1, 63, 64, 127, 128, 129, 1000, 4000, 4096])
 def test_fused_inference_equals_layer_gemms(mlp, B):
-    """The fused inference kernel (all layers per 64-row tile, activations in
-    LDS) runs the layer GEMMs' MFMA sequence and rounding: bit-identical
-    outputs to the train-mode forward's five layer launches."""
+    """The fused inference kernel (all layers per 128-row tile, activations in
+    LDS, the transposed product W A^T) forms every output from the same
+    products in the layer GEMMs' k order with the same roundings:
+    bit-identical outputs to the train-mode forward's five layer launches,
+    also for batches that end inside a tile."""
     ctx, h = mlp
     p = init(4)
     x = np.random.default_rng(10 + B).normal(size=(B, 9))
