@@ -2455,7 +2455,7 @@ int pst_mlp_create(pst_ctx* ctx, int max_rows, pst_mlp** out) {
   CK(al((void**)&m.y, R * 64 * 4));
   CK(al((void**)&m.gy, R * 64 * 2));
   CK(al((void**)&m.gw, (H * 64 * 2 + H * H * 3) * 4));
-  CK(al((void**)&m.gcol, (R + 255) / 256 * 256 * 4));
+  CK(al((void**)&m.gcol, (R + kColRows - 1) / kColRows * 256 * 4));
   CK(hipMemsetAsync(m.x0, 0, R * 64 * 2, ctx->stream));
   CK(hipMemsetAsync(m.x0f, 0, R * kMlpInLd * 4, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));

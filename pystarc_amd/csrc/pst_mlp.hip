@@ -409,15 +409,17 @@ __global__ void k_mlp_pack_gy(const float* gy, int B, int Bp, __bf16* g) {
 }
 
 // Bias gradient: column sums of a BF16 [Bp][ld] gradient into out[n], in two
-// passes with a fixed order (no atomics): block b sums rows 256b..256b+255 of
-// every column (thread n reads column n, so a wave reads whole row segments),
-// then one block adds the blocks' partial sums in order.
-constexpr int kColRows = 256;
+// passes with a fixed order (no atomics): block b sums rows
+// kColRows b .. kColRows b + kColRows - 1 of every column (thread n reads
+// column n, so a wave reads whole row segments; a training batch of 16,384
+// rows gives 256 blocks, one per CU), then one block adds the blocks'
+// partial sums in order.
 __global__ void k_colsum_part(const __bf16* g, int rows, int ld, float* part) {
   const int n = threadIdx.x;
   if (n >= ld) return;
   const int r0 = blockIdx.x * kColRows, r1 = min(rows, r0 + kColRows);
   float s = 0.f;
+#pragma unroll 16
   for (int r = r0; r < r1; ++r) s += (float)g[(size_t)r * ld + n];
   part[(size_t)blockIdx.x * 256 + n] = s;
 }
@@ -426,6 +428,7 @@ __global__ void k_colsum_final(const float* part, int n_part, int n_cols, float*
   const int n = threadIdx.x;
   if (n >= n_cols) return;
   float s = 0.f;
+#pragma unroll 16
   for (int b = 0; b < n_part; ++b) s += part[(size_t)b * 256 + n];
   out[n] = s;
 }

@@ -24,6 +24,9 @@ constexpr int kMlpInLd = 16;   // FP32 feature row stride (9 used)
 // rows of one fused-inference tile; every row buffer (features, outputs) is
 // allocated in a multiple of it
 constexpr int kInferRows = 128;
+// rows per partial column sum of the bias gradients; the partial-sum buffer
+// holds 256 floats per kColRows rows
+constexpr int kColRows = 64;
 
 struct MlpBufs {
   int max_rows;  // padded to kInferRows
@@ -38,7 +41,7 @@ struct MlpBufs {
   __bf16* gy;                 // [rows][64]
   __bf16 *g1, *g2, *g3, *g4;  // [rows][256]
   float* gw;                  // padded weight gradients
-  float* gcol;                // [rows / 256][256] partial column sums (bias gradients)
+  float* gcol;                // [rows / kColRows][256] partial column sums (bias gradients)
 };
 
 // Operands of the fused inference forward of one network.

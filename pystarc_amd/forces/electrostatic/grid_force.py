@@ -61,9 +61,11 @@ class DXGrid:
         flat = np.asarray(self.data, dtype=np.float64).ravel()
         with open(Path(path), "w") as fh:
             fh.write(f"object 1 class gridpositions counts {nx} {ny} {nz}\n")
-            fh.write("origin " + " ".join(f"{v:.6f}" for v in self.origin) + "\n")
+            # geometry as shortest round-trip decimals: a grid read back sits
+            # exactly where it was written
+            fh.write("origin " + " ".join(repr(float(v)) for v in self.origin) + "\n")
             for row in self.delta:
-                fh.write("delta " + " ".join(f"{v:.6f}" for v in row) + "\n")
+                fh.write("delta " + " ".join(repr(float(v)) for v in row) + "\n")
             fh.write(f"object 2 class gridconnections counts {nx} {ny} {nz}\n")
             fh.write(f"object 3 class array type double rank 0 items {flat.size} data follows\n")
             full = flat.size - flat.size % 3

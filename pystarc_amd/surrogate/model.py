@@ -185,7 +185,8 @@ class Surrogate:
         weights W) are device float32 tensors."""
         torch = self.torch
         n = X.shape[0]
-        opt = torch.optim.Adam([self.params], lr=lr)
+        # one fused kernel per step (the foreach form launches several per step)
+        opt = torch.optim.Adam([self.params], lr=lr, fused=self.params.is_cuda)
         steps = max(1, epochs * (n // batch))
         wts = torch.as_tensor(self.loss_weights(), dtype=torch.float32, device=self.device)
         k = 0

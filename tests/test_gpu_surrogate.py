@@ -31,8 +31,11 @@ def pp():
 def trained(pp):
     sysm, grids = pp
     eng = S.make_engine(sysm, grids)
-    s = build_surrogate(eng, sysm, n_train=2_000_000, epochs=12, batch=4096, max_rows=32768,
-                        log=print)
+    # a surrogate whose in-step force error (~6-8 %) is what the bench's
+    # training reaches; a loosely fitted one (~17 %) shifts k_on by ~2 standard
+    # errors of the 30,000-trajectory estimate
+    s = build_surrogate(eng, sysm, n_train=4_000_000, epochs=24, batch=8192, max_rows=32768,
+                        log=print, lr=3e-3)
     return eng, s
 
 
