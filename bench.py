@@ -158,8 +158,11 @@ def run_window(sysm, eng, args, surrogate=None):
 
 
 def mlp_roofline(surr, pos, quat, reps=20):
-    """Average duration of the MLP forward on n poses (events on torch's
-    stream, which is the library's stream) and its FLOP rate."""
+    """Average duration of the fused inference kernel on the features of n
+    live surrogate poses (HIP events around the kernel's launches on the
+    library's stream, pst_mlp_time_infer) and its FLOP rate; `call_ms` is
+    the library's whole forward call (feature packing, kernel, unpacking)
+    timed the same way from torch's stream."""
     import torch
     n = min(len(pos), surr.max_rows)
     X = torch.as_tensor(surr.features(pos[:n], quat[:n]), device=surr.device)
@@ -170,8 +173,12 @@ def mlp_roofline(surr, pos, quat, reps=20):
         surr.forward(X)
     b.record()
     torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / reps
-    return {"ms": ms, "n": n, "tflops": MLP_FLOP_PER_TRAJ * n / (ms * 1e-3) / 1e12}
+    call_ms = a.elapsed_time(b) / reps
+    ms = C.c_double()
+    surr.ctx.check(surr.lib.pst_mlp_time_infer(surr.h, n, reps, C.byref(ms)), "pst_mlp_time_infer")
+    ms = ms.value
+    return {"ms": ms, "n": n, "tflops": MLP_FLOP_PER_TRAJ * n / (ms * 1e-3) / 1e12,
+            "call_ms": call_ms}
 
 
 def full_run(sysm, eng, args, surrogate=None):
@@ -355,6 +362,22 @@ def measured_traffic(workload, path, slots):
         if d.get("workload_key", "protein_protein") != workload or d.get("slots") != slots:
             return None
         return d["paths"][path]["hbm_bytes_per_eval"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def measured_limiter(workload, path, slots):
+    """The committed cache-counter summary of the force kernel for `path`
+    (L1 tag lookups per stencil, L1 / L2 hit rates, address / data unit busy
+    fractions), or None."""
+    f = os.path.join(ROOT, "profiles", "k_forces_traffic.json")
+    try:
+        with open(f) as fh:
+            d = json.load(fh)
+        cc = d.get("cache_counters") or {}
+        if d.get("workload_key") != workload or d.get("slots") != slots or cc.get("path") != path:
+            return None
+        return dict(cc["derived"], source="profiles/k_forces_traffic.json")
     except (OSError, KeyError, ValueError):
         return None
 
@@ -552,7 +575,7 @@ def main():
     ap.add_argument("--kon-trajectories", type=int, default=500_000)
     # the surrogate workload's 2e7 (pose, force) training pairs
     ap.add_argument("--train-poses", type=int, default=20_000_000)
-    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--train-epochs", type=int, default=64)
     ap.add_argument("--train-batch", type=int, default=16384)
     ap.add_argument("--train-lr", type=float, default=4e-3)
     ap.add_argument("--audit-n", type=int, default=64)
@@ -687,7 +710,8 @@ def main():
                 "kernel": kname, "ms_per_launch": rf["ms"], "launches": rf["launches"],
                 "census": rf["census"], "algorithmic_bytes": rf["bytes"],
                 "gather_ceiling_gbs": ceiling["clustered"]["gbs"],
-                "frac_of_gather_ceiling": rf["achieved_gbs"] / ceiling["clustered"]["gbs"]}
+                "frac_of_gather_ceiling": rf["achieved_gbs"] / ceiling["clustered"]["gbs"],
+                "limiter": measured_limiter(args.workload, head, args.slots)}
     plain = results.get("fp64")
     out = {
         "metric": METRIC,

@@ -317,6 +317,11 @@ int pst_mlp_load(pst_mlp* mlp, const float* params);
 /* x (B,9) f32 -> y (B,13) f32, device pointers; train != 0 keeps the
  * activations for pst_mlp_backward. */
 int pst_mlp_forward(pst_mlp* mlp, int B, const float* x, float* y, int train);
+/* Average duration (HIP events on the context stream) of the fused
+ * inference kernel alone over `reps` launches on the first B rows of the
+ * features left by the last pst_mlp_forward of at least B rows; the outputs
+ * stay in the library's buffer.  Measurement only; no reference counterpart. */
+int pst_mlp_time_infer(pst_mlp* mlp, int B, int reps, double* ms_per_launch);
 /* dL/dy (B,13) f32 -> dL/dparams (flat f32), for the last train forward. */
 int pst_mlp_backward(pst_mlp* mlp, int B, const float* gy, float* gparams);
 /* Attaches (mlp != NULL) or detaches the surrogate branch of the production

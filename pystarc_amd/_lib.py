@@ -123,6 +123,7 @@ _SIGS = {
     "pst_mlp_destroy": ([_vp], C.c_int),
     "pst_mlp_load": ([_vp, _vp], C.c_int),
     "pst_mlp_forward": ([_vp, C.c_int, _vp, _vp, C.c_int], C.c_int),
+    "pst_mlp_time_infer": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "pst_mlp_backward": ([_vp, C.c_int, _vp, _vp], C.c_int),
     "pst_sim_set_surrogate": ([_vp, _vp, _vp], C.c_int),
     "pst_sim_reset_audit": ([_vp], C.c_int),

@@ -2489,6 +2489,34 @@ int pst_mlp_forward(pst_mlp* p, int B, const float* x, float* y, int train) {
   return 0;
 }
 
+int pst_mlp_time_infer(pst_mlp* p, int B, int reps, double* ms_per_launch) {
+  if (!p) return 1;
+  pst_ctx* ctx = p->ctx;
+  if (B <= 0 || B > p->m.max_rows || reps <= 0 || !ms_per_launch)
+    FAIL("pst_mlp_time_infer: bad arguments");
+  CK(set_dev(ctx));
+  const int Bp = (B + 63) / 64 * 64;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e == hipSuccess) e = mlp_forward_packed(p->m, Bp, nullptr, false, p->m.y, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+  for (int i = 0; i < reps && e == hipSuccess; ++i)
+    e = mlp_forward_packed(p->m, Bp, nullptr, false, p->m.y, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (e != hipSuccess) {
+    ctx->err = std::string("pst_mlp_time_infer: ") + hipGetErrorString(e);
+    return 2;
+  }
+  *ms_per_launch = ms / reps;
+  return 0;
+}
+
 int pst_mlp_backward(pst_mlp* p, int B, const float* gy, float* gparams) {
   if (!p) return 1;
   pst_ctx* ctx = p->ctx;

@@ -46,13 +46,29 @@ def test_traffic_bytes_follow_from_the_counters(path):
     assert all("rocprofv3 --pmc" in c and "bench.py --pmc" in c for c in e["commands"])
 
 
-def test_surrogate_traffic_exceeds_its_algorithmic_bytes():
-    # algorithmic bytes of one surrogate-window evaluation at 262,144 slots:
-    # ~16,300 trajectories x 104 B plus 80 B per atom-in-grid (20.5 M + 20.5 M
-    # + 24.7 M in the counter runs); line traffic cannot be below it
-    e = json.load(open(TRAFFIC))["paths"]["surrogate"]
-    alg = 16300 * 104 + (20.5e6 + 20.5e6 + 24.7e6) * 80
-    assert e["hbm_bytes_per_eval"] > alg
+def test_cache_counter_summary_follows_from_its_counters():
+    # the derived figures of the cache-counter summary (and the limiter the
+    # bench line carries) follow from the per-dispatch counters
+    sys.path.insert(0, ROOT)
+    import bench
+    d = json.load(open(TRAFFIC))
+    cc = d["cache_counters"]
+    c, dv = cc["per_dispatch"], cc["derived"]
+    stencils = c["census"][1] + c["census"][3]
+    assert dv["l1_tag_lookups_per_stencil"] == pytest.approx(
+        c["TCP_TOTAL_CACHE_ACCESSES_sum"] / stencils, abs=0.05)
+    assert dv["l1_hit_rate"] == pytest.approx(
+        1 - c["TCP_TCC_READ_REQ_sum"] / c["TCP_TOTAL_CACHE_ACCESSES_sum"], abs=0.002)
+    assert dv["l2_hit_rate"] == pytest.approx(
+        c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"]), abs=0.002)
+    cycles = c["GRBM_GUI_ACTIVE"] / 8            # per XCD
+    assert dv["ta_busy_frac"] == pytest.approx(c["TA_TA_BUSY_sum"] / 256 / cycles, abs=0.02)
+    assert dv["td_busy_frac"] == pytest.approx(c["TD_TD_BUSY_sum"] / 256 / cycles, abs=0.02)
+    assert dv["l2_requests_per_clock_per_xcd"] == pytest.approx(
+        c["TCP_TCC_READ_REQ_sum"] / 8 / cycles, abs=0.1)
+    lim = bench.measured_limiter("protein_protein", cc["path"], d["slots"])
+    assert lim["l1_hit_rate"] == dv["l1_hit_rate"]
+    assert bench.measured_limiter("protein_protein", "fp64", d["slots"]) is None
 
 
 def test_cpu_model_and_workload_text():

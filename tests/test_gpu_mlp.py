@@ -126,3 +126,20 @@ def test_fused_inference_equals_layer_gemms(mlp, B):
     y_fused, _ = run_fwd(ctx, h, p, x, train=False)
     y_layers, _ = run_fwd(ctx, h, p, x, train=True)
     assert np.array_equal(y_fused, y_layers)
+
+
+def test_time_infer_measures_the_kernel_and_rejects_bad_sizes(mlp):
+    """pst_mlp_time_infer times the fused kernel alone on the features the
+    last forward packed; its launches recompute the same outputs."""
+    ctx, h = mlp
+    p = init(5)
+    x = np.random.default_rng(5).normal(size=(1000, 9))
+    y, _ = run_fwd(ctx, h, p, x, train=False)
+    ms = C.c_double()
+    ctx.check(ctx.lib.pst_mlp_time_infer(h, 1000, 3, C.byref(ms)), "pst_mlp_time_infer")
+    assert 0.0 < ms.value < 100.0
+    y2, _ = run_fwd(ctx, h, p, x, train=False)
+    assert np.array_equal(y, y2)
+    assert ctx.lib.pst_mlp_time_infer(h, 0, 3, C.byref(ms)) != 0
+    assert ctx.lib.pst_mlp_time_infer(h, 1 << 20, 3, C.byref(ms)) != 0
+    assert ctx.lib.pst_mlp_time_infer(h, 10, 0, C.byref(ms)) != 0
