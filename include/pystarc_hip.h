@@ -148,7 +148,7 @@ int pst_time_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* q
  * points uniform over the grid (mode 0) or clustered 256 to a cube of edge
  * 12 / mode A (mode 1..16), `reps` timed launches; plus 32 x the fetch
  * variant (0 the plain stencil, 1 the z-pair fetches of the force kernel's
- * z-pair mode).  out4 = {ms per launch, algorithmic GB/s, points, bytes per
+ * z-pair mode, 2 the same fetches issued by teams of 8 lanes).  out4 = {ms per launch, algorithmic GB/s, points, bytes per
  * point}. */
 int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, double* out4);
 

@@ -1016,7 +1016,7 @@ int pst_gather_ceiling(pst_ctx* ctx, int mode, long long n_points, int reps, dou
   if (!ctx->pair.mem) FAIL("pst_gather_ceiling: needs the FP32 potential + Born pair grid");
   const int variant = mode >> 5;
   mode &= 31;
-  if (mode > 16 || variant > 1) FAIL("pst_gather_ceiling: mode must be 0..16 plus 32 x variant 0..1");
+  if (mode > 16 || variant > 2) FAIL("pst_gather_ceiling: mode must be 0..16 plus 32 x variant 0..2");
   if (n_points <= 0 || n_points > (1ll << 30) || reps <= 0 || !out4)
     FAIL("pst_gather_ceiling: bad arguments");
   CK(set_dev(ctx));

@@ -327,63 +327,128 @@ __device__ inline void grid_phi_grad(const GridDev& g, double px, double py, dou
 // are weighted, which keeps FP32 free of the cancellation of two nearly
 // equal interpolated values, and every node is loaded once.  Callers
 // guarantee the whole stencil is inside the grid (the 3-spacing valid box).
-template <typename TG, typename TC, typename V, typename A>
-__device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __restrict__ d,
-                                               const A& a, double px, double py, double pz,
-                                               V& phi, V& gx, V& gy, V& gz) {
+//
+// The evaluation comes in three parts so that the loads can be issued by
+// other lanes than the one that evaluates (stencil_team_load): the geometry
+// of the point (stencil_geo), the 20 node values (per lane: stencil_load),
+// and the arithmetic on them (stencil_eval).
+
+// Centre cell (i0, j0, k0) and fractions f; the half-shifted cell's fractions
+// e and whether its lower plane is the centre cell's lower plane (a = 0) or
+// the one below it (a = -1), per axis.
+template <typename TC>
+struct StencilGeo {
+  int i0, j0, k0;
+  int ai, aj, ak;
+  TC fx, fy, fz;
+  TC ex, ey, ez;
+};
+
+template <typename TC>
+__device__ inline StencilGeo<TC> stencil_geo(const GridDev& g, double px, double py, double pz) {
+  StencilGeo<TC> s;
   const double ix = (px - g.o[0]) * g.inv[0];
   const double iy = (py - g.o[1]) * g.inv[1];
   const double iz = (pz - g.o[2]) * g.inv[2];
   const double fi = floor(ix), fj = floor(iy), fk = floor(iz);
-  const int i0 = (int)fi, j0 = (int)fj, k0 = (int)fk;
-  const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
+  s.i0 = (int)fi; s.j0 = (int)fj; s.k0 = (int)fk;
+  s.fx = (TC)(ix - fi); s.fy = (TC)(iy - fj); s.fz = (TC)(iz - fk);
   const double mi = floor(ix - 0.5), mj = floor(iy - 0.5), mk = floor(iz - 0.5);
-  const TC ex = (TC)(ix - 0.5 - mi), ey = (TC)(iy - 0.5 - mj), ez = (TC)(iz - 0.5 - mk);
-  const int ai = (int)mi - i0, aj = (int)mj - j0, ak = (int)mk - k0;  // -1 or 0
+  s.ex = (TC)(ix - 0.5 - mi); s.ey = (TC)(iy - 0.5 - mj); s.ez = (TC)(iz - 0.5 - mk);
+  s.ai = (int)mi - s.i0; s.aj = (int)mj - s.j0; s.ak = (int)mk - s.k0;  // -1 or 0
+  return s;
+}
+
+// The 20 nodes: centre cell (c), the x- and y-extra planes (x, y) at z nodes
+// (k0, k0 + 1), and the z-extra plane (z) at node k0 - 1 (ak = -1) or
+// k0 + 2 (ak = 0).
+template <typename V>
+struct StencilNodes {
+  V c000, c001, c010, c011, c100, c101, c110, c111;
+  V x00, x01, x10, x11, y00, y01, y10, y11;
+  V z00, z01, z10, z11;
+};
+
+// The stencil as 8 (i, j) columns of 3 z nodes kmin .. kmin + 2
+// (kmin = k0 + ak): column c = 0..3 the centre cell's (i0 + (c >> 1),
+// j0 + (c & 1)), 4..5 the x-extra plane at j0 + (c & 1), 6..7 the y-extra
+// plane at i0 + (c & 1).  Offsets of column c's (i, j) through accessor a.
+template <typename A, typename TC>
+__device__ inline long stencil_col_xy(const A& a, const StencilGeo<TC>& s, int c) {
+  const int xe = s.ai < 0 ? s.i0 - 1 : s.i0 + 2, ye = s.aj < 0 ? s.j0 - 1 : s.j0 + 2;
+  const int i = c < 4 ? s.i0 + (c >> 1) : (c < 6 ? xe : s.i0 + (c & 1));
+  const int j = c < 4 ? s.j0 + (c & 1) : (c < 6 ? s.j0 + (c & 1) : ye);
+  return a.ox(i) + a.oy(j);
+}
+
+// Column c's nodes kmin .. kmin + 2 (m0, m1, m2) -> the stencil's nodes.
+template <typename V, typename TC>
+__device__ inline void stencil_put_col(const StencilGeo<TC>& s, int c, const V& m0, const V& m1,
+                                       const V& m2, StencilNodes<V>& n) {
+  // the z-extra node is m0 (ak = -1) or m2; the centre cell's nodes the others
+  const bool low = s.ak < 0;
+  const V lo = low ? m1 : m0, hi = low ? m2 : m1, ze = low ? m0 : m2;
+  switch (c) {
+    case 0: n.c000 = lo; n.c001 = hi; n.z00 = ze; break;
+    case 1: n.c010 = lo; n.c011 = hi; n.z01 = ze; break;
+    case 2: n.c100 = lo; n.c101 = hi; n.z10 = ze; break;
+    case 3: n.c110 = lo; n.c111 = hi; n.z11 = ze; break;
+    case 4: n.x00 = lo; n.x01 = hi; break;
+    case 5: n.x10 = lo; n.x11 = hi; break;
+    case 6: n.y00 = lo; n.y01 = hi; break;
+    default: n.y10 = lo; n.y11 = hi; break;
+  }
+}
+
+// Per-lane loads of the 20 nodes through accessor a (z-pair fetches when the
+// accessor asks for them).
+template <typename TG, typename V, typename A, typename TC>
+__device__ inline void stencil_load(const TG* __restrict__ d, const A& a, const StencilGeo<TC>& s,
+                                    StencilNodes<V>& n) {
+  const int i0 = s.i0, j0 = s.j0, k0 = s.k0, ai = s.ai, aj = s.aj, ak = s.ak;
   // per-axis offsets of the centre cell's two planes and of the extra plane
   // (index -1 when a = -1, +2 when a = 0)
   const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Xe = a.ox(ai < 0 ? i0 - 1 : i0 + 2);
   const long Y0 = a.oy(j0), Y1 = a.oy(j0 + 1), Ye = a.oy(aj < 0 ? j0 - 1 : j0 + 2);
-  // centre cell (c), the x- and y-extra planes (x, y) at z nodes (k0, k0 + 1),
-  // and the z-extra plane (z) at node k0 - 1 (ak = -1) or k0 + 2 (ak = 0)
-  V c000, c001, c010, c011, c100, c101, c110, c111, x00, x01, x10, x11, y00, y01, y10, y11;
-  V z00, z01, z10, z11;
   if constexpr (A::kZPairs) {
     // every column's nodes kmin .. kmin + 2 (kmin = k0 + ak) in two aligned
     // pair loads: 16 loads per stencil, the same for every lane
     const int kmin = k0 + ak, e = kmin & ~1;
-    const bool odd = (kmin & 1) != 0, lowx = ak < 0;
+    const bool odd = (kmin & 1) != 0;
     const long Za = a.oz(e), Zb = a.oz(e + 2);
-    V u;
-    // (m0, m1, m2) = nodes kmin .. kmin + 2: the z-extra node is m0 (ak = -1) or m2
-    auto centre = [&](long xy, V& lo, V& hi, V& ze) {
+    const long xy[8] = {X0 + Y0, X0 + Y1, X1 + Y0, X1 + Y1, Xe + Y0, Xe + Y1, X0 + Ye, X1 + Ye};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
       V m0, m1, m2;
-      zcol3<V, TG>(d, xy + Za, xy + Zb, odd, m0, m1, m2);
-      lo = lowx ? m1 : m0;
-      hi = lowx ? m2 : m1;
-      ze = lowx ? m0 : m2;
-    };
-    centre(X0 + Y0, c000, c001, z00);
-    centre(X0 + Y1, c010, c011, z01);
-    centre(X1 + Y0, c100, c101, z10);
-    centre(X1 + Y1, c110, c111, z11);
-    centre(Xe + Y0, x00, x01, u);
-    centre(Xe + Y1, x10, x11, u);
-    centre(X0 + Ye, y00, y01, u);
-    centre(X1 + Ye, y10, y11, u);
+      zcol3<V, TG>(d, xy[c] + Za, xy[c] + Zb, odd, m0, m1, m2);
+      stencil_put_col(s, c, m0, m1, m2, n);
+    }
   } else {
     const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1), Ze = a.oz(ak < 0 ? k0 - 1 : k0 + 2);
-    c000 = gld<V>(d, X0 + Y0 + Z0); c001 = gld<V>(d, X0 + Y0 + Z1);
-    c010 = gld<V>(d, X0 + Y1 + Z0); c011 = gld<V>(d, X0 + Y1 + Z1);
-    c100 = gld<V>(d, X1 + Y0 + Z0); c101 = gld<V>(d, X1 + Y0 + Z1);
-    c110 = gld<V>(d, X1 + Y1 + Z0); c111 = gld<V>(d, X1 + Y1 + Z1);
-    x00 = gld<V>(d, Xe + Y0 + Z0); x01 = gld<V>(d, Xe + Y0 + Z1);
-    x10 = gld<V>(d, Xe + Y1 + Z0); x11 = gld<V>(d, Xe + Y1 + Z1);
-    y00 = gld<V>(d, X0 + Ye + Z0); y01 = gld<V>(d, X0 + Ye + Z1);
-    y10 = gld<V>(d, X1 + Ye + Z0); y11 = gld<V>(d, X1 + Ye + Z1);
-    z00 = gld<V>(d, X0 + Y0 + Ze); z01 = gld<V>(d, X0 + Y1 + Ze);
-    z10 = gld<V>(d, X1 + Y0 + Ze); z11 = gld<V>(d, X1 + Y1 + Ze);
+    n.c000 = gld<V>(d, X0 + Y0 + Z0); n.c001 = gld<V>(d, X0 + Y0 + Z1);
+    n.c010 = gld<V>(d, X0 + Y1 + Z0); n.c011 = gld<V>(d, X0 + Y1 + Z1);
+    n.c100 = gld<V>(d, X1 + Y0 + Z0); n.c101 = gld<V>(d, X1 + Y0 + Z1);
+    n.c110 = gld<V>(d, X1 + Y1 + Z0); n.c111 = gld<V>(d, X1 + Y1 + Z1);
+    n.x00 = gld<V>(d, Xe + Y0 + Z0); n.x01 = gld<V>(d, Xe + Y0 + Z1);
+    n.x10 = gld<V>(d, Xe + Y1 + Z0); n.x11 = gld<V>(d, Xe + Y1 + Z1);
+    n.y00 = gld<V>(d, X0 + Ye + Z0); n.y01 = gld<V>(d, X0 + Ye + Z1);
+    n.y10 = gld<V>(d, X1 + Ye + Z0); n.y11 = gld<V>(d, X1 + Ye + Z1);
+    n.z00 = gld<V>(d, X0 + Y0 + Ze); n.z01 = gld<V>(d, X0 + Y1 + Ze);
+    n.z10 = gld<V>(d, X1 + Y0 + Ze); n.z11 = gld<V>(d, X1 + Y1 + Ze);
   }
+}
+
+// phi and the gradient from the 20 nodes.
+template <typename TC, typename V>
+__device__ inline void stencil_eval(const GridDev& g, const StencilGeo<TC>& s,
+                                    const StencilNodes<V>& n, V& phi, V& gx, V& gy, V& gz) {
+  const TC fx = s.fx, fy = s.fy, fz = s.fz, ex = s.ex, ey = s.ey, ez = s.ez;
+  const int ai = s.ai, aj = s.aj, ak = s.ak;
+  const V c000 = n.c000, c001 = n.c001, c010 = n.c010, c011 = n.c011;
+  const V c100 = n.c100, c101 = n.c101, c110 = n.c110, c111 = n.c111;
+  const V x00 = n.x00, x01 = n.x01, x10 = n.x10, x11 = n.x11;
+  const V y00 = n.y00, y01 = n.y01, y10 = n.y10, y11 = n.y11;
+  const V z00 = n.z00, z01 = n.z01, z10 = n.z10, z11 = n.z11;
   const TC one = (TC)1;
   const TC gxw = one - fx, gyw = one - fy, gzw = one - fz;
   phi = c000 * gxw * gyw * gzw + c100 * fx * gyw * gzw + c010 * gxw * fy * gzw +
@@ -428,6 +493,77 @@ __device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __res
     const V d11 = e1 * (n1_11 - n0_11) + ez * (n2_11 - n1_11);
     gz = (d00 * gxw * gyw + d10 * fx * gyw + d01 * gxw * fy + d11 * fx * fy) * (TC)g.inv[2];
   }
+}
+
+// Cooperative stencil loads of one wave (64 lanes, one point each, the
+// z-pair accessor).  A lane's 16 loads touch ~6 cache lines, but the 64 lanes
+// of a load instruction touch ~53 (every lane a different point), and the
+// L1 looks up one line per clock: that lookup rate, not HBM or L2, bounds a
+// gather over scattered points.  Here the lanes form 8 teams of 8; in round
+// r (0..7) team t loads the stencil of lane 8t + r, lane `rank` of the team
+// its column `rank` (two aligned z-pair loads, as stencil_load does), so one
+// instruction touches the few lines of 8 points.  Each lane stores its
+// column's 3 nodes in the owner's record in LDS (`buf`, kTeamWords values of
+// V per wave: node k of column c of lane L at k * 520 + c * 65 + L, which
+// keeps both the stores of a round and the owner's reads free of bank
+// conflicts), and every lane then reads its own 20 nodes back.  The nodes
+// are those stencil_load reads, so stencil_eval gives the same bits.  All 64
+// lanes must call it together (lanes without a stencil pass need = false).
+constexpr int kTeamWords = 3 * 520;
+
+template <typename TG, typename V, typename A, typename TC>
+__device__ inline void stencil_team_load(const TG* __restrict__ d, const A& a,
+                                         const StencilGeo<TC>& s, bool need, V* buf,
+                                         StencilNodes<V>& n) {
+  static_assert(A::kZPairs, "team loads use the z-pair accessor");
+  const int lane = __lane_id();
+  const int team = lane >> 3, rank = lane & 7;
+  const int w0 = (s.i0 & 0xFFFF) | (s.j0 << 16);
+  const int w1 = (s.k0 & 0xFFFF) | ((s.ai < 0) << 16) | ((s.aj < 0) << 17) | ((s.ak < 0) << 18) |
+                 ((need ? 1 : 0) << 19);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int owner = (team << 3) + r;
+    const int o0 = __shfl(w0, owner), o1 = __shfl(w1, owner);
+    if (o1 & (1 << 19)) {
+      StencilGeo<TC> t;
+      t.i0 = o0 & 0xFFFF; t.j0 = o0 >> 16; t.k0 = o1 & 0xFFFF;
+      t.ai = (o1 >> 16) & 1 ? -1 : 0; t.aj = (o1 >> 17) & 1 ? -1 : 0; t.ak = (o1 >> 18) & 1 ? -1 : 0;
+      const long xy = stencil_col_xy(a, t, rank);
+      const int kmin = t.k0 + t.ak, e = kmin & ~1;
+      V m0, m1, m2;
+      zcol3<V, TG>(d, xy + a.oz(e), xy + a.oz(e + 2), (kmin & 1) != 0, m0, m1, m2);
+      V* rec = buf + rank * 65 + owner;
+      rec[0] = m0;
+      rec[520] = m1;
+      rec[1040] = m2;
+    }
+  }
+  // the stores of the other lanes of this wave are visible after this point
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (need) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const V* rec = buf + c * 65 + lane;
+      stencil_put_col(s, c, rec[0], rec[520], rec[1040], n);
+    }
+  }
+  // the records may be overwritten by the next call only after every lane read its own
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename TG, typename TC, typename V, typename A>
+__device__ inline void grid_phi_grad_stencil_a(const GridDev& g, const TG* __restrict__ d,
+                                               const A& a, double px, double py, double pz,
+                                               V& phi, V& gx, V& gy, V& gz) {
+  const StencilGeo<TC> s = stencil_geo<TC>(g, px, py, pz);
+  StencilNodes<V> n;
+  stencil_load<TG, V>(d, a, s, n);
+  stencil_eval<TC, V>(g, s, n, phi, gx, gy, gz);
 }
 
 template <typename TG, typename TC, typename V, bool ZP = false>
