@@ -410,7 +410,35 @@ __device__ inline void stencil_load(const TG* __restrict__ d, const A& a, const 
   // (index -1 when a = -1, +2 when a = 0)
   const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Xe = a.ox(ai < 0 ? i0 - 1 : i0 + 2);
   const long Y0 = a.oy(j0), Y1 = a.oy(j0 + 1), Ye = a.oy(aj < 0 ? j0 - 1 : j0 + 2);
-  if constexpr (A::kZPairs) {
+  if constexpr (A::kZPairs && std::is_same<TG, float>::value && std::is_same<V, float>::value) {
+    // a single FP32 field: every column's nodes kmin .. kmin + 2 from the
+    // 16-byte z-row (4 nodes of one brick) that holds kmin, and the next
+    // brick's row only where the three cross it (kmin mod 4 >= 2, half the
+    // points): 8-16 loads per stencil, and the lanes that need no second row
+    // cost the L1 no lookup in the other 8
+    const int kmin = k0 + ak, kq = kmin & 3, rb = kmin - kq;
+    const bool two = kq >= 2;
+    const long Zr0 = a.oz(rb), Zr1 = a.oz(rb + 4);
+    const long xy[8] = {X0 + Y0, X0 + Y1, X1 + Y0, X1 + Y1, Xe + Y0, Xe + Y1, X0 + Ye, X1 + Ye};
+    float4 r0[8];
+    float2 r1[8];   // of the next row only its first two nodes can be needed
+#pragma unroll
+    for (int c = 0; c < 8; ++c) r0[c] = *(const float4*)(d + xy[c] + Zr0);
+    if (two) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) r1[c] = *(const float2*)(d + xy[c] + Zr1);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const float4 u = r0[c];
+      const float2 w = two ? r1[c] : make_float2(u.z, u.w);
+      // nodes kq, kq + 1, kq + 2 of the 8 nodes (u, w)
+      const float m0 = kq == 0 ? u.x : kq == 1 ? u.y : kq == 2 ? u.z : u.w;
+      const float m1 = kq == 0 ? u.y : kq == 1 ? u.z : kq == 2 ? u.w : w.x;
+      const float m2 = kq == 0 ? u.z : kq == 1 ? u.w : kq == 2 ? w.x : w.y;
+      stencil_put_col(s, c, m0, m1, m2, n);
+    }
+  } else if constexpr (A::kZPairs) {
     // every column's nodes kmin .. kmin + 2 (kmin = k0 + ak) in two aligned
     // pair loads: 16 loads per stencil, the same for every lane
     const int kmin = k0 + ak, e = kmin & ~1;
