@@ -679,13 +679,10 @@ static bool split_eligible(const ForceSys& S, const ForceWork& W, int G) {
 // evaluation (the ligand-atom terms over the list and the reverse Born over
 // the list re-sorted by the ligand-frame direction of the receptor, in one
 // launch; the same results as the direct kernel with the same G); else
-// k_forces.  Modes 2 and 4 use z-pair fetches (FP32).
+// k_forces.  Modes 2 and 4 use z-pair fetches.
 static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const ForceWork& W,
                                     int precision, int G, int max_n, int cap, int mode) {
   ctx->last_G = G;
-  // z-pair fetches are an FP32 launch variant
-  if (precision != 1 && mode == 2) mode = 0;
-  if (precision != 1 && mode == 4) mode = 3;
   if ((mode == 3 || mode == 4) && split_eligible(S, W, G)) {
     ctx->last_mode = mode;
     return launch_forces_split(S, W, precision, G, max_n, cap, ctx->stream, mode == 4);
@@ -696,7 +693,7 @@ static hipError_t launch_force_eval(pst_ctx* ctx, const ForceSys& S, const Force
     ctx->last_mode = 1;
     return launch_forces_binned(S, W, ctx->bins, precision, G, max_n, cap, ctx->stream);
   }
-  if (mode == 2 && precision == 1) {
+  if (mode == 2) {
     ctx->last_mode = 2;
     return launch_forces(S, W, precision, G, max_n, cap, ctx->stream, true);
   }
@@ -1755,18 +1752,17 @@ int pst_prod_autotune(pst_sim* s, int rounds, int steps_per_cand, double* table,
     for (int g : Gs)
       if (g >= 8)
         for (int c : {1024, 4096}) cands.push_back({1, g, c, 1, 0.0, 0, 0.0, 0});
-  // FP32: the direct kernel with z-pair fetches
-  if (s->precision == 1)
-    for (int g : Gs)
-      for (int c : {2048, 8192})
-        for (int x = 0; x < 2; ++x) cands.push_back({2, g, c, x, 0.0, 0, 0.0, 0});
+  // the direct kernel with z-pair fetches
+  for (int g : Gs)
+    for (int c : {2048, 8192})
+      for (int x = 0; x < 2; ++x) cands.push_back({2, g, c, x, 0.0, 0, 0.0, 0});
   // split evaluation (reverse-Born pass in its own order), XCD-chunked
   if (S.has_rev)
     for (int g : Gs)
       if (g >= 16)
         for (int c : {2048, 8192}) {
           cands.push_back({3, g, c, 1, 0.0, 0, 0.0, 0});
-          if (s->precision == 1) cands.push_back({4, g, c, 1, 0.0, 0, 0.0, 0});
+          cands.push_back({4, g, c, 1, 0.0, 0, 0.0, 0});
         }
   CK(hipEventCreate(&s->tev0));
   CK(hipEventCreate(&s->tev1));

@@ -147,7 +147,9 @@ __device__ inline int dir_cell(double dx, double dy, double dz) {
 
 // ---------------------------------------------------------------------------
 // Trilinear gather, batch_force_kernel (gpu_batch_engine.py:95-110): cell
-// index floor((p - o) / sp); value 0 when the cell is outside [0, n-2].
+// index floor((p - o) / sp).  The reference's value 0 for a cell outside
+// [0, n-2] never applies here: a gather is evaluated only strictly inside
+// the 3-spacing valid box, where the whole stencil is inside the grid.
 // Index arithmetic is FP64; weights and sums are in TC (float or double).
 // The value type V is TC for one grid, or Pair<TC> when the receptor
 // potential and Born grids share a geometry and are stored interleaved
@@ -262,65 +264,15 @@ __device__ inline GlobalAccT<ZP> global_acc(const GridDev& g) {
 }
 
 
-template <typename TG, typename TC, typename V, typename A>
-__device__ inline V grid_interp_a(const GridDev& g, const TG* __restrict__ d, const A& a, double x,
-                                  double y, double z) {
-  const double ix = (x - g.o[0]) * g.inv[0];
-  const double iy = (y - g.o[1]) * g.inv[1];
-  const double iz = (z - g.o[2]) * g.inv[2];
-  const double fi = floor(ix), fj = floor(iy), fk = floor(iz);
-  const int i0 = (int)fi, j0 = (int)fj, k0 = (int)fk;
-  if (i0 < 0 || i0 >= g.nx - 1 || j0 < 0 || j0 >= g.ny - 1 || k0 < 0 || k0 >= g.nz - 1)
-    return V{};
-  const TC fx = (TC)(ix - fi), fy = (TC)(iy - fj), fz = (TC)(iz - fk);
-  const long X0 = a.ox(i0), X1 = a.ox(i0 + 1), Y0 = a.oy(j0), Y1 = a.oy(j0 + 1);
-  const long Z0 = a.oz(k0), Z1 = a.oz(k0 + 1);
-  const V c000 = gld<V>(d, X0 + Y0 + Z0), c001 = gld<V>(d, X0 + Y0 + Z1);
-  const V c010 = gld<V>(d, X0 + Y1 + Z0), c011 = gld<V>(d, X0 + Y1 + Z1);
-  const V c100 = gld<V>(d, X1 + Y0 + Z0), c101 = gld<V>(d, X1 + Y0 + Z1);
-  const V c110 = gld<V>(d, X1 + Y1 + Z0), c111 = gld<V>(d, X1 + Y1 + Z1);
-  const TC gx = (TC)1 - fx, gy = (TC)1 - fy, gz = (TC)1 - fz;
-  return c000 * gx * gy * gz + c100 * fx * gy * gz + c010 * gx * fy * gz + c001 * gx * gy * fz +
-         c110 * fx * fy * gz + c101 * fx * gy * fz + c011 * gx * fy * fz + c111 * fx * fy * fz;
-}
-
-template <typename TG, typename TC, typename V>
-__device__ inline V grid_interp(const GridDev& g, double x, double y, double z) {
-  return grid_interp_a<TG, TC, V>(g, (const TG*)g.data, global_acc(g), x, y, z);
-}
-
 // Strictly inside the valid box (gpu_batch_engine.py:317-319).
 __device__ inline bool grid_inside(const GridDev& g, double x, double y, double z) {
   return x > g.lo[0] && y > g.lo[1] && z > g.lo[2] && x < g.hi[0] && y < g.hi[1] && z < g.hi[2];
 }
 
-// phi and grad phi by central differences at +-sp/2 (gpu_batch_engine.py:111-115),
-// seven interpolations exactly as the reference evaluates them.
-template <typename TG, typename TC, typename V, typename A>
-__device__ inline void grid_phi_grad_a(const GridDev& g, const TG* __restrict__ d, const A& a,
-                                       double px, double py, double pz, V& phi, V& gx, V& gy,
-                                       V& gz) {
-  const double h0 = g.sp[0] * 0.5, h1 = g.sp[1] * 0.5, h2 = g.sp[2] * 0.5;
-  gx = (grid_interp_a<TG, TC, V>(g, d, a, px + h0, py, pz) -
-        grid_interp_a<TG, TC, V>(g, d, a, px - h0, py, pz)) /
-       (TC)(2 * h0);
-  gy = (grid_interp_a<TG, TC, V>(g, d, a, px, py + h1, pz) -
-        grid_interp_a<TG, TC, V>(g, d, a, px, py - h1, pz)) /
-       (TC)(2 * h1);
-  gz = (grid_interp_a<TG, TC, V>(g, d, a, px, py, pz + h2) -
-        grid_interp_a<TG, TC, V>(g, d, a, px, py, pz - h2)) /
-       (TC)(2 * h2);
-  phi = grid_interp_a<TG, TC, V>(g, d, a, px, py, pz);
-}
-
-template <typename TG, typename TC, typename V>
-__device__ inline void grid_phi_grad(const GridDev& g, double px, double py, double pz, V& phi,
-                                     V& gx, V& gy, V& gz) {
-  grid_phi_grad_a<TG, TC, V>(g, (const TG*)g.data, global_acc(g), px, py, pz, phi, gx, gy, gz);
-}
-
-// Same phi and central-difference gradient, evaluated from the 20 distinct
-// stencil nodes: the probes at p -+ sp/2 along an axis sit in neighbouring
+// phi and grad phi by central differences at +-sp/2 of trilinear
+// interpolations (batch_force_kernel, gpu_batch_engine.py:95-115: cell index
+// floor((p - o) / sp), seven interpolations of 8 nodes each), evaluated from
+// the 20 distinct stencil nodes: the probes at p -+ sp/2 along an axis sit in neighbouring
 // cells with one common fraction f, so
 //   phi(p + h) - phi(p - h) = sum_yz w_yz [(1-f)(v[m+1]-v[m]) + f(v[m+2]-v[m+1])],
 // m = floor(i - 1/2).  Neighbouring grid values are subtracted before they
@@ -601,32 +553,26 @@ __device__ inline void grid_phi_grad_stencil(const GridDev& g, double px, double
                                      gy, gz);
 }
 
-// FP64 follows the reference's seven interpolations literally (parity with
-// gpu_batch_engine.py to rounding); FP32 uses the node-difference stencil.
+// Both precisions evaluate the node-difference stencil (20 nodes); the
+// reference's seven trilinear interpolations (gpu_batch_engine.py:95-115)
+// give the same phi and gradient algebraically wherever the whole stencil is
+// inside the grid, which every caller guarantees (the 3-spacing valid box),
+// so FP64 results equal theirs to rounding (~1e-16 relative) with 20 node
+// loads instead of 56.
 template <typename TC, bool ZP = false>
 __device__ inline void grid_phi_grad_any(const GridDev& g, double px, double py, double pz,
                                          TC& phi, TC& gx, TC& gy, TC& gz) {
-  if constexpr (std::is_same<TC, double>::value) {
-    if (g.dtype == 1)
-      grid_phi_grad<double, double, double>(g, px, py, pz, phi, gx, gy, gz);
-    else
-      grid_phi_grad<float, double, double>(g, px, py, pz, phi, gx, gy, gz);
-  } else {
-    if (g.dtype == 1)
-      grid_phi_grad_stencil<double, float, float, ZP>(g, px, py, pz, phi, gx, gy, gz);
-    else
-      grid_phi_grad_stencil<float, float, float, ZP>(g, px, py, pz, phi, gx, gy, gz);
-  }
+  if (g.dtype == 1)
+    grid_phi_grad_stencil<double, TC, TC, ZP>(g, px, py, pz, phi, gx, gy, gz);
+  else
+    grid_phi_grad_stencil<float, TC, TC, ZP>(g, px, py, pz, phi, gx, gy, gz);
 }
 
 // Interleaved potential + Born grid (float2 per node): both fields at once.
 template <typename TC, bool ZP = false>
 __device__ inline void grid_phi_grad_pair(const GridDev& g, double px, double py, double pz,
                                           Pair<TC>& phi, Pair<TC>& gx, Pair<TC>& gy, Pair<TC>& gz) {
-  if constexpr (std::is_same<TC, double>::value)
-    grid_phi_grad<float2, double, Pair<double>>(g, px, py, pz, phi, gx, gy, gz);
-  else
-    grid_phi_grad_stencil<float2, float, Pair<float>, ZP>(g, px, py, pz, phi, gx, gy, gz);
+  grid_phi_grad_stencil<float2, TC, Pair<TC>, ZP>(g, px, py, pz, phi, gx, gy, gz);
 }
 
 
@@ -636,19 +582,13 @@ template <typename TC, typename A>
 __device__ inline void grid_phi_grad_pair_a(const GridDev& g, const float2* d, const A& a,
                                             double px, double py, double pz, Pair<TC>& phi,
                                             Pair<TC>& gx, Pair<TC>& gy, Pair<TC>& gz) {
-  if constexpr (std::is_same<TC, double>::value)
-    grid_phi_grad_a<float2, double, Pair<double>>(g, d, a, px, py, pz, phi, gx, gy, gz);
-  else
-    grid_phi_grad_stencil_a<float2, float, Pair<float>>(g, d, a, px, py, pz, phi, gx, gy, gz);
+  grid_phi_grad_stencil_a<float2, TC, Pair<TC>>(g, d, a, px, py, pz, phi, gx, gy, gz);
 }
 template <typename TC, typename A>
 __device__ inline void grid_phi_grad_f32_a(const GridDev& g, const float* d, const A& a,
                                            double px, double py, double pz, TC& phi, TC& gx,
                                            TC& gy, TC& gz) {
-  if constexpr (std::is_same<TC, double>::value)
-    grid_phi_grad_a<float, double, double>(g, d, a, px, py, pz, phi, gx, gy, gz);
-  else
-    grid_phi_grad_stencil_a<float, float, float>(g, d, a, px, py, pz, phi, gx, gy, gz);
+  grid_phi_grad_stencil_a<float, TC, TC>(g, d, a, px, py, pz, phi, gx, gy, gz);
 }
 
 template <typename TC> __device__ inline TC pst_exp(TC x);

@@ -460,7 +460,7 @@ static hipError_t launch_split_g(const ForceSys& S, const ForceWork& W, int max_
   if (blocks < 8) blocks = 8;
   blocks = (blocks + 7) / 8 * 8;
   const size_t lds = S.M <= kLdsAtomsMax ? (size_t)S.M * sizeof(typename Vec4<TC>::type) : 0;
-  if (zp && std::is_same<TC, float>::value)
+  if (zp)
     hipLaunchKernelGGL((k_forces_split<TC, G, true>), dim3(blocks), dim3(kForceBlock), lds, st, S,
                        W);
   else
@@ -480,7 +480,7 @@ static hipError_t launch_g(const ForceSys& S, const ForceWork& W, int max_n, int
   const bool pose = W.quat != nullptr && W.lig_pos == nullptr;
   size_t lds = 0;
   if (pose && S.M <= kLdsAtomsMax) lds = (size_t)S.M * sizeof(typename Vec4<TC>::type);
-  if (pose && zp && std::is_same<TC, float>::value)
+  if (pose && zp)
     hipLaunchKernelGGL((k_forces<TC, G, true, true>), dim3(blocks), dim3(kForceBlock), lds, st, S,
                        W);
   else if (pose)

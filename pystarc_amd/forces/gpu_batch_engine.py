@@ -146,9 +146,9 @@ class GPUBatchForceEngine:
         staged once in LDS; identical results; buffers for max_traj
         trajectories.  Without the float2 receptor pair grid the direct
         kernel runs.
-        "zpair"  - the direct kernel fetching the two z-adjacent nodes of a
-        stencil column with one wide load (FP32; identical results; the
-        FP64 engine runs the direct kernel).
+        "zpair"  - the direct kernel fetching the z nodes of every stencil
+        column with wide loads (the same node values, so the same results
+        to the rounding of the arithmetic; both precisions).
         "split" / "split_zpair" - production steps only: the ligand-atom
         terms over the grid-path list, then the reverse Born over the list
         re-sorted by the ligand-frame direction of the receptor (identical

@@ -110,10 +110,10 @@ def test_binned_production_identical_to_direct(name, hydro):
 @pytest.mark.parametrize("name", ["protein_ligand", "protein_protein"])
 @pytest.mark.parametrize("torque", ["reference", "physical"])
 @pytest.mark.parametrize("G", [16, 64])
-def test_zpair_equals_direct_fp32(name, torque, G):
+def test_zpair_equals_direct(name, torque, G):
     """The z-pair launch fetches the same node values with wider loads: the
-    FP32 forces equal the direct kernel's to FP32 rounding, and an FP64
-    engine asked for z-pairs runs the direct kernel."""
+    FP32 forces equal the direct kernel's to FP32 rounding, the FP64 forces
+    to FP64 rounding."""
     sysm, (rmin, rmax) = _system(name)
     g = numpy_grids(sysm)
     eng = S.make_engine(sysm, dx_grids(g), precision="fp32", torque_mode=torque, force_group=G)
@@ -128,8 +128,13 @@ def test_zpair_equals_direct_fp32(name, torque, G):
         scale = float(a.abs().max()) + 1e-30
         assert float((a - b).abs().max()) <= 1e-6 * scale
     assert float(F0.abs().sum()) > 0
-    e64 = S.make_engine(sysm, dx_grids(g), precision="fp64", force_group=G)
+    e64 = S.make_engine(sysm, dx_grids(g), precision="fp64", torque_mode=torque, force_group=G)
     e64.set_ligand(sysm.mol2.positions_array() - sysm.mol2.centroid(), sysm.mol2.charges_array())
+    e64.set_gather("direct")
+    F0, T0, E0 = e64.forces_pose(c, q)
     e64.set_gather("zpair")
-    e64.forces_pose(c, q)
-    assert e64.last_eval()[0] == "direct"
+    F1, T1, E1 = e64.forces_pose(c, q)
+    assert e64.last_eval() == ("zpair", G)
+    for a, b in ((F0, F1), (T0, T1), (E0, E1)):
+        scale = float(a.abs().max()) + 1e-30
+        assert float((a - b).abs().max()) <= 1e-13 * scale
