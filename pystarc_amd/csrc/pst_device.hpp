@@ -72,6 +72,13 @@ __device__ inline void philox_uniform2(uint64_t seed, uint64_t traj, uint32_t st
   u1 = (double)b * s;
 }
 
+// sin and cos of 2 pi u.  Through sincospi the argument needs no reduction
+// (2u is reduced exactly), so the general sin / cos and their large-argument
+// reduction (dozens of registers and a scratch frame in every kernel that
+// draws a normal or a rotation) stay out of the step kernels.  The values
+// differ from sin(2 pi u) by about one ulp.
+__device__ inline void sincos_2pi(double u, double* s, double* c) { sincospi(2.0 * u, s, c); }
+
 // Two standard normals by Box-Muller from one Philox block.
 __device__ inline void philox_normal2(uint64_t seed, uint64_t traj, uint32_t step,
                                       uint32_t stream, uint32_t sub, double& z0, double& z1) {
@@ -80,7 +87,7 @@ __device__ inline void philox_normal2(uint64_t seed, uint64_t traj, uint32_t ste
   const double one_minus = 1.0 - u0;  // in (0, 1]
   double r = sqrt(-2.0 * log(one_minus));
   double s, c;
-  sincos(2.0 * M_PI * u1, &s, &c);
+  sincos_2pi(u1, &s, &c);
   z0 = r * c;
   z1 = r * s;
 }
@@ -99,10 +106,13 @@ __device__ inline void quat_to_rot(const double* q, double* R) {
 // Shoemake uniform rotation from three uniforms, gpu_batch_simulator.py:1597-1609.
 __device__ inline void shoemake(double u0, double u1, double u2, double* q) {
   const double a = sqrt(1.0 - u0), b = sqrt(u0);
-  q[0] = a * sin(2.0 * M_PI * u1);
-  q[1] = a * cos(2.0 * M_PI * u1);
-  q[2] = b * sin(2.0 * M_PI * u2);
-  q[3] = b * cos(2.0 * M_PI * u2);
+  double s1, c1, s2, c2;
+  sincos_2pi(u1, &s1, &c1);
+  sincos_2pi(u2, &s2, &c2);
+  q[0] = a * s1;
+  q[1] = a * c1;
+  q[2] = b * s2;
+  q[3] = b * c2;
 }
 
 // q <- normalize(q (x) dq(omega)), gpu_batch_simulator.py:1631-1660.
@@ -110,8 +120,9 @@ __device__ inline void apply_rotation(double* q, double ox, double oy, double oz
   const double ang = sqrt(ox * ox + oy * oy + oz * oz);
   double ax = 0.0, ay = 0.0, az = 0.0;
   if (ang > 1e-10) { ax = ox / ang; ay = oy / ang; az = oz / ang; }
-  const double half = ang * 0.5;
-  const double s = sin(half), c = cos(half);
+  // sin and cos of ang / 2 as sincospi of ang / (2 pi) (see sincos_2pi)
+  double s, c;
+  sincospi(ang * (0.5 * M_1_PI), &s, &c);
   const double w2 = c, x2 = ax * s, y2 = ay * s, z2 = az * s;
   const double w1 = q[0], x1 = q[1], y1 = q[2], z1 = q[3];
   double nw = w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2;

@@ -34,30 +34,34 @@ __device__ inline uint64_t rng_key(const StepParams& P, long long t) {
 // ----------------------------------------------------------------------------
 // shared step physics
 // ----------------------------------------------------------------------------
-// Pair distances |R b_p + c - r_p| (gpu_batch_simulator.py:1737-1751).
-__device__ inline void pair_dists(const StepParams& P, const double* R, double cx, double cy,
-                                  double cz, double* d) {
-  for (int p = 0; p < P.n_pairs; ++p) {
-    const double bx = P.lig_gho[3 * p], by = P.lig_gho[3 * p + 1], bz = P.lig_gho[3 * p + 2];
-    const double gx = R[0] * bx + R[1] * by + R[2] * bz + cx - P.rec_gho[3 * p];
-    const double gy = R[3] * bx + R[4] * by + R[5] * bz + cy - P.rec_gho[3 * p + 1];
-    const double gz = R[6] * bx + R[7] * by + R[8] * bz + cz - P.rec_gho[3 * p + 2];
-    d[p] = sqrt(gx * gx + gy * gy + gz * gz);
-  }
+// Pair distance |R b_p + c - r_p| (gpu_batch_simulator.py:1737-1751).
+__device__ inline double pair_dist(const StepParams& P, const double* R, double cx, double cy,
+                                   double cz, int p) {
+  const double bx = P.lig_gho[3 * p], by = P.lig_gho[3 * p + 1], bz = P.lig_gho[3 * p + 2];
+  const double gx = R[0] * bx + R[1] * by + R[2] * bz + cx - P.rec_gho[3 * p];
+  const double gy = R[3] * bx + R[4] * by + R[5] * bz + cy - P.rec_gho[3 * p + 1];
+  const double gz = R[6] * bx + R[7] * by + R[8] * bz + cz - P.rec_gho[3 * p + 2];
+  return sqrt(gx * gx + gy * gy + gz * gz);
+}
+
+// Pairs within their cutoff (the checks below use each distance as it is
+// formed: no per-lane array, which would live in scratch memory).
+__device__ inline int pairs_within(const StepParams& P, const double* pos, const double* q) {
+  double R[9];
+  quat_to_rot(q, R);
+  int n = 0;
+  for (int p = 0; p < P.n_pairs; ++p)
+    n += pair_dist(P, R, pos[0], pos[1], pos[2], p) < P.cutoff[p] ? 1 : 0;
+  return n;
 }
 
 __device__ inline bool reacted_at(const StepParams& P, const double* pos, const double* q) {
   if (P.n_pairs == 0) return false;
-  double R[9], d[PST_MAX_PAIRS];
-  quat_to_rot(q, R);
-  pair_dists(P, R, pos[0], pos[1], pos[2], d);
-  int n = 0;
-  for (int p = 0; p < P.n_pairs; ++p) n += d[p] < P.cutoff[p] ? 1 : 0;
-  return n >= P.n_needed;
+  return pairs_within(P, pos, q) >= P.n_needed;
 }
 
-// dt per trajectory, flattened mode (:975-1004, :1061-1067).
-__device__ inline double adaptive_dt(const StepParams& P, double r) {
+// dt per trajectory before the cap and the floor, flattened mode (:975-1004).
+__device__ inline double adaptive_dt_raw(const StepParams& P, double r) {
   const double D_t = P.D_t;
   const bool in_outer = r > 1.1 * P.r_b;
   const double dt_pair = (0.1 * 0.1 / 2.0) * r * r / D_t;
@@ -74,7 +78,12 @@ __device__ inline double adaptive_dt(const StepParams& P, double r) {
   const double dist_e = fmax(P.r_esc - r, 1e-3);
   const double de = fmin(dist_b, dist_e);
   const double dt_edge = de * de / (18.0 * D_t);
-  double dt = in_outer ? fmin(fmin(dt_force, dt_edge), dt_pair) : fmin(dt_force, dt_pair);
+  return in_outer ? fmin(fmin(dt_force, dt_edge), dt_pair) : fmin(dt_force, dt_pair);
+}
+
+// dt per trajectory, flattened mode (:975-1004, :1061-1067): cap, then floor.
+__device__ inline double adaptive_dt(const StepParams& P, double r) {
+  double dt = adaptive_dt_raw(P, r);
   if (P.max_dt > 0) dt = fmin(dt, P.max_dt);
   if (P.min_core_dt > 0) dt = fmax(dt, P.min_core_dt);
   return dt;
@@ -86,15 +95,11 @@ __device__ inline double adaptive_dt(const StepParams& P, double r) {
 // exceeds rc, dt <- min(dt, max(rc^2 / (288 D_t), min_core_rxn_dt)).
 __device__ inline double adaptive_dt_sm(const StepParams& P, double r, int state,
                                         const double* R, const double* pos) {
-  StepParams Q = P;
-  Q.max_dt = 0;
-  Q.min_core_dt = 0;
-  double dt = adaptive_dt(Q, r);
-  double d[PST_MAX_PAIRS];
-  pair_dists(P, R, pos[0], pos[1], pos[2], d);
+  double dt = adaptive_dt_raw(P, r);
   double rc = 1.0e30;
   for (int p = 0; p < P.n_pairs; ++p)
-    if (P.sm_before[p] == state) rc = fmin(rc, fmax(d[p] - P.cutoff[p], 0.0));
+    if (P.sm_before[p] == state)
+      rc = fmin(rc, fmax(pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p], 0.0));
   const double sig = sqrt(2.0 * P.D_t * dt);
   if (12.0 * sig > rc) {
     double crit = rc * rc / (288.0 * P.D_t);
@@ -110,11 +115,11 @@ __device__ inline double adaptive_dt_sm(const StepParams& P, double r, int state
 // source state is the trajectory's and whose pair is within its cutoff.
 __device__ inline int sm_check(const StepParams& P, const double* pos, const double* q,
                                int state) {
-  double R[9], d[PST_MAX_PAIRS];
+  double R[9];
   quat_to_rot(q, R);
-  pair_dists(P, R, pos[0], pos[1], pos[2], d);
   for (int p = 0; p < P.n_pairs; ++p)
-    if (P.sm_before[p] == state && d[p] < P.cutoff[p]) return p;
+    if (P.sm_before[p] == state && pair_dist(P, R, pos[0], pos[1], pos[2], p) < P.cutoff[p])
+      return p;
   return -1;
 }
 
@@ -164,10 +169,9 @@ struct MoveSave { double p[3], q[4]; };
 
 __device__ inline void move_begin(const StepParams& P, const double* pos, const double* q,
                                   double* x0, MoveSave& sv) {
-  double R[9], d[PST_MAX_PAIRS];
+  double R[9];
   quat_to_rot(q, R);
-  pair_dists(P, R, pos[0], pos[1], pos[2], d);
-  for (int p = 0; p < P.n_pairs; ++p) x0[p] = d[p] - P.cutoff[p];
+  for (int p = 0; p < P.n_pairs; ++p) x0[p] = pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p];
   sv.p[0] = pos[0]; sv.p[1] = pos[1]; sv.p[2] = pos[2];
   sv.q[0] = q[0]; sv.q[1] = q[1]; sv.q[2] = q[2]; sv.q[3] = q[3];
 }
@@ -185,11 +189,10 @@ __device__ inline bool move_end(const StepParams& P, const ForceSys& S, double* 
   }
   bool any = false;
   if (P.n_pairs) {
-    double R[9], d[PST_MAX_PAIRS];
+    double R[9];
     quat_to_rot(q, R);
-    pair_dists(P, R, pos[0], pos[1], pos[2], d);
     for (int p = 0; p < P.n_pairs; ++p) {
-      x1[p] = d[p] - P.cutoff[p];
+      x1[p] = pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p];
       any = any || (x0[p] > 0 && x1[p] > 0);
     }
   }
@@ -488,14 +491,7 @@ __device__ inline void obs_flush(const StepParams& P, const ProdBufs& B, int s, 
   rec[18] = bridge;
   for (int k = 0; k < 3; ++k) rec[19 + k] = pos[k];
   for (int k = 0; k < 4; ++k) rec[22 + k] = q[k];
-  int within = 0;
-  if (P.n_pairs) {
-    double R[9], d[PST_MAX_PAIRS];
-    quat_to_rot(q, R);
-    pair_dists(P, R, pos[0], pos[1], pos[2], d);
-    for (int p = 0; p < P.n_pairs; ++p) within += d[p] < P.cutoff[p] ? 1 : 0;
-  }
-  rec[26] = within;
+  rec[26] = P.n_pairs ? pairs_within(P, pos, q) : 0;
 }
 
 __device__ inline void finish_traj(const StepParams& P, const ProdBufs& B, int s, int outcome,
@@ -573,60 +569,60 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
   const bool stop = C->stop != 0;
   bool live = false, far = false;
   if (s < B.n_slots && !stop && B.status[s] == PST_RUNNING) {
+    // a refilled slot is checked again as its new trajectory (guarded); the
+    // outcome of a check ends the trajectory at one call site, so the refill
+    // code is inlined once
     for (int guard = 0; guard < 8; ++guard) {
       double* pos = B.pos + 3 * (size_t)s;
       double* q = B.quat + 4 * (size_t)s;
       const long long t = B.traj[s];
       const unsigned int ns = B.n_steps[s];
+      int outcome = 0;
       if ((long long)ns >= P.max_steps) {
-        finish_traj(P, B, s, PST_MAXSTEPS);
-        if (B.status[s] != PST_RUNNING) break;
-        continue;
-      }
-      bool fired = false;
-      if (P.sm_active) {
-        const int rx = sm_check(P, pos, q, B.sm_state[s]);
-        if (rx >= 0) {
-          fired = true;
-          if (sm_fire(P, B, s, rx, false)) {
-            finish_traj(P, B, s, PST_REACTED);
-            if (B.status[s] != PST_RUNNING) break;
-            continue;
+        outcome = PST_MAXSTEPS;
+      } else {
+        bool fired = false;
+        if (P.sm_active) {
+          const int rx = sm_check(P, pos, q, B.sm_state[s]);
+          if (rx >= 0) {
+            fired = true;
+            if (sm_fire(P, B, s, rx, false)) outcome = PST_REACTED;
+          }
+        } else if (reacted_at(P, pos, q)) {
+          outcome = PST_REACTED;
+        }
+        if (!outcome) {
+          const double r = norm3(pos[0], pos[1], pos[2]);
+          if (P.lmz_on) {
+            // the outer excursion ran in k_prod_outer this step (returned: the
+            // pose is already back on the b-sphere)
+            if (B.lmz_escaped[s]) {
+              B.lmz_escaped[s] = 0;
+              outcome = PST_ESCAPED;
+            }
+          } else if (r >= P.r_esc && !fired) {
+            double u, unused;
+            philox_uniform2(P.seed, rng_key(P, t), ns, RNG_ESC, 0, u, unused);
+            if (u < P.return_prob) {
+              pos[0] = pos[0] / r * P.r_b;
+              pos[1] = pos[1] / r * P.r_b;
+              pos[2] = pos[2] / r * P.r_b;
+              double u0, u1, u2, u3;
+              philox_uniform2(P.seed, rng_key(P, t), ns, RNG_RETQ, 0, u0, u1);
+              philox_uniform2(P.seed, rng_key(P, t), ns, RNG_RETQ, 1, u2, u3);
+              shoemake(u0, u1, u2, q);
+              atomicAdd(&B.ctr->n_returns, 1ull);
+              if (B.obs.traj) B.obs.s_ret[s] += 1;
+            } else {
+              outcome = PST_ESCAPED;
+            }
           }
         }
-      } else if (reacted_at(P, pos, q)) {
-        finish_traj(P, B, s, PST_REACTED);
+      }
+      if (outcome) {
+        finish_traj(P, B, s, outcome);
         if (B.status[s] != PST_RUNNING) break;
         continue;
-      }
-      const double r = norm3(pos[0], pos[1], pos[2]);
-      if (P.lmz_on) {
-        // the outer excursion ran in k_prod_outer this step (returned: the
-        // pose is already back on the b-sphere)
-        if (B.lmz_escaped[s]) {
-          B.lmz_escaped[s] = 0;
-          finish_traj(P, B, s, PST_ESCAPED);
-          if (B.status[s] != PST_RUNNING) break;
-          continue;
-        }
-      } else if (r >= P.r_esc && !fired) {
-        double u, unused;
-        philox_uniform2(P.seed, rng_key(P, t), ns, RNG_ESC, 0, u, unused);
-        if (u < P.return_prob) {
-          pos[0] = pos[0] / r * P.r_b;
-          pos[1] = pos[1] / r * P.r_b;
-          pos[2] = pos[2] / r * P.r_b;
-          double u0, u1, u2, u3;
-          philox_uniform2(P.seed, rng_key(P, t), ns, RNG_RETQ, 0, u0, u1);
-          philox_uniform2(P.seed, rng_key(P, t), ns, RNG_RETQ, 1, u2, u3);
-          shoemake(u0, u1, u2, q);
-          atomicAdd(&B.ctr->n_returns, 1ull);
-          if (B.obs.traj) B.obs.s_ret[s] += 1;
-        } else {
-          finish_traj(P, B, s, PST_ESCAPED);
-          if (B.status[s] != PST_RUNNING) break;
-          continue;
-        }
       }
       live = true;
       far = P.surrogate_r > 0 && norm3(pos[0], pos[1], pos[2]) > P.surrogate_r;
