@@ -857,6 +857,13 @@ int pst_forces_pose(pst_ctx* ctx, int n, const double* pos, const double* quat, 
   return forces_common(ctx, W, n, precision, n);
 }
 
+// Whether a force evaluation fetches z-pairs / z-rows (modes 2 / 4).
+static bool group_zp(const pst_ctx* ctx) { return ctx->gather_mode == 2 || ctx->gather_mode == 4; }
+static bool sim_zp(const pst_sim* s) {
+  const int mode = s->tune_mode >= 0 ? s->tune_mode : s->ctx->gather_mode;
+  return mode == 2 || mode == 4;
+}
+
 int pst_forces_pose_group(pst_ctx* const* ctxs, int L, const int* n, const double* const* pos,
                           const double* const* quat, double* const* F, double* const* T,
                           double* const* E, int precision) {
@@ -917,7 +924,8 @@ int pst_forces_pose_group(pst_ctx* const* ctxs, int L, const int* n, const doubl
   if (e == hipSuccess)
     e = hipMemcpy(dstart, start.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_forces_group(dS, dW, dstart, L, start[L], lds, precision, G, ctx->stream);
+    e = launch_forces_group(dS, dW, dstart, L, start[L], lds, precision, G, group_zp(ctx),
+                            ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   release();
   if (e != hipSuccess) {
@@ -1955,6 +1963,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       lds = std::max(lds, (size_t)S[l].M * (sims[0]->precision == 0 ? 32 : 16));
   }
   G = G >= 64 ? 64 : (G >= 32 ? 32 : 16);
+  const bool zp = sim_zp(sims[0]);
   // workgroups per system: the grid-stride loop covers any count, so the
   // launch holds about 4096 resident workgroups in all (a launch of mostly
   // idle workgroups costs dispatch time when few trajectories are left)
@@ -2033,7 +2042,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     const hipStream_t cs = sims[0]->ctx->stream;
     hipError_t e = prod_group_pre(dP, dB, dsstart, L, sstart[L], par, any_outer, cs);
     if (e == hipSuccess)
-      e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G, cs);
+      e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G, zp, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     if (n_surr > 0) {
       for (int l : surr_l) {

@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) k_rev_apply(const int* __restrict__ list,
 // Several systems (one ligand each, sharing the receptor grids) in one
 // launch: workgroup b belongs to system l with start[l] <= b < start[l + 1]
 // (start[] multiples of 8, so the XCD of a workgroup is that of its slice).
-template <typename TC, int G>
+template <typename TC, int G, bool ZP>
 __global__ void __launch_bounds__(kForceBlock) k_forces_group(const ForceSys* __restrict__ Sg,
                                                               const ForceWork* __restrict__ Wg,
                                                               const int* __restrict__ start, int L) {
@@ -395,31 +395,36 @@ __global__ void __launch_bounds__(kForceBlock) k_forces_group(const ForceSys* __
   l = __builtin_amdgcn_readfirstlane(l);
   const int b0 = __builtin_amdgcn_readfirstlane(start[l]);
   const int b1 = __builtin_amdgcn_readfirstlane(start[l + 1]);
-  forces_body<TC, G, true>(Sg[l], Wg[l], b - b0, b1 - b0);
+  forces_body<TC, G, true, ZP>(Sg[l], Wg[l], b - b0, b1 - b0);
 }
 
 template <typename TC, int G>
 static hipError_t launch_group_g(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
-                                 int blocks, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_forces_group<TC, G>), dim3(blocks), dim3(kForceBlock), lds, st, Sg, Wg,
-                     start, L);
+                                 int blocks, size_t lds, bool zp, hipStream_t st) {
+  if (zp)
+    hipLaunchKernelGGL((k_forces_group<TC, G, true>), dim3(blocks), dim3(kForceBlock), lds, st, Sg,
+                       Wg, start, L);
+  else
+    hipLaunchKernelGGL((k_forces_group<TC, G, false>), dim3(blocks), dim3(kForceBlock), lds, st,
+                       Sg, Wg, start, L);
   return hipGetLastError();
 }
 
 hipError_t launch_forces_group(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
-                               int blocks, size_t lds, int precision, int G, hipStream_t st) {
+                               int blocks, size_t lds, int precision, int G, bool zp,
+                               hipStream_t st) {
   if (blocks <= 0) return hipSuccess;
   if (precision == 0) {
     switch (G) {
-      case 16: return launch_group_g<double, 16>(Sg, Wg, start, L, blocks, lds, st);
-      case 32: return launch_group_g<double, 32>(Sg, Wg, start, L, blocks, lds, st);
-      default: return launch_group_g<double, 64>(Sg, Wg, start, L, blocks, lds, st);
+      case 16: return launch_group_g<double, 16>(Sg, Wg, start, L, blocks, lds, zp, st);
+      case 32: return launch_group_g<double, 32>(Sg, Wg, start, L, blocks, lds, zp, st);
+      default: return launch_group_g<double, 64>(Sg, Wg, start, L, blocks, lds, zp, st);
     }
   }
   switch (G) {
-    case 16: return launch_group_g<float, 16>(Sg, Wg, start, L, blocks, lds, st);
-    case 32: return launch_group_g<float, 32>(Sg, Wg, start, L, blocks, lds, st);
-    default: return launch_group_g<float, 64>(Sg, Wg, start, L, blocks, lds, st);
+    case 16: return launch_group_g<float, 16>(Sg, Wg, start, L, blocks, lds, zp, st);
+    case 32: return launch_group_g<float, 32>(Sg, Wg, start, L, blocks, lds, zp, st);
+    default: return launch_group_g<float, 64>(Sg, Wg, start, L, blocks, lds, zp, st);
   }
 }
 
