@@ -112,7 +112,9 @@ struct ProdBufs {
   int* key;
   int* bins;                   // [256] key histogram, [256..511] scatter cursors,
                                // [512..767] / [768..1023] the same for the reverse cells
-  double* F; double* T; double* dt; double* x0; double* x1;
+  double* F; double* T; double* dt;
+  double* x0; double* x1;      // pair distance minus cutoff before / after the move,
+                               // [PST_MAX_PAIRS][n_slots] (pair-major)
   RunCounters* ctr;
   signed char* out_status;     // per trajectory id (nullable)
   int* out_steps;              // per trajectory id (nullable)

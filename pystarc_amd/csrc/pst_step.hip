@@ -168,10 +168,11 @@ __device__ inline bool overlaps(const StepParams& P, const ForceSys& S, double c
 struct MoveSave { double p[3], q[4]; };
 
 __device__ inline void move_begin(const StepParams& P, const double* pos, const double* q,
-                                  double* x0, MoveSave& sv) {
+                                  double* x0, int xs, MoveSave& sv) {
   double R[9];
   quat_to_rot(q, R);
-  for (int p = 0; p < P.n_pairs; ++p) x0[p] = pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p];
+  for (int p = 0; p < P.n_pairs; ++p)
+    x0[p * xs] = pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p];
   sv.p[0] = pos[0]; sv.p[1] = pos[1]; sv.p[2] = pos[2];
   sv.q[0] = q[0]; sv.q[1] = q[1]; sv.q[2] = q[2]; sv.q[3] = q[3];
 }
@@ -179,7 +180,7 @@ __device__ inline void move_begin(const StepParams& P, const double* pos, const 
 // Overlap rejection (:1106-1128) and new pair distances.  Returns whether any
 // pair is above its cutoff both before and after the move.
 __device__ inline bool move_end(const StepParams& P, const ForceSys& S, double* pos, double* q,
-                                const MoveSave& sv, const double* x0, double* x1,
+                                const MoveSave& sv, const double* x0, double* x1, int xs,
                                 bool* rejected) {
   *rejected = false;
   if (P.use_overlap && S.n_rec > 0 && overlaps(P, S, pos[0], pos[1], pos[2])) {
@@ -192,8 +193,9 @@ __device__ inline bool move_end(const StepParams& P, const ForceSys& S, double* 
     double R[9];
     quat_to_rot(q, R);
     for (int p = 0; p < P.n_pairs; ++p) {
-      x1[p] = pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p];
-      any = any || (x0[p] > 0 && x1[p] > 0);
+      const double a = x0[p * xs], b = pair_dist(P, R, pos[0], pos[1], pos[2], p) - P.cutoff[p];
+      x1[p * xs] = b;
+      any = any || (a > 0 && b > 0);
     }
   }
   return any;
@@ -204,9 +206,9 @@ __device__ inline bool move_end(const StepParams& P, const ForceSys& S, double* 
 __device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* pos, double* q,
                                 const double* Fv, const double* Tv, const double* nt,
                                 const double* nr, double dt, double D, double* x0, double* x1,
-                                bool* rejected) {
+                                int xs, bool* rejected) {
   MoveSave sv;
-  move_begin(P, pos, q, x0, sv);
+  move_begin(P, pos, q, x0, xs, sv);
   const double sig = sqrt(2.0 * D * dt);
   pos[0] += D * dt * Fv[0] + sig * nt[0];
   pos[1] += D * dt * Fv[1] + sig * nt[1];
@@ -214,7 +216,7 @@ __device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* 
   const double sr = sqrt(2.0 * P.D_r * dt);
   apply_rotation(q, P.D_r * dt * Tv[0] + sr * nr[0], P.D_r * dt * Tv[1] + sr * nr[1],
                  P.D_r * dt * Tv[2] + sr * nr[2]);
-  return move_end(P, S, pos, q, sv, x0, x1, rejected);
+  return move_end(P, S, pos, q, sv, x0, x1, xs, rejected);
 }
 
 // Two-body RPY move: both bodies translate and turn with the 12x12 mobility
@@ -222,28 +224,29 @@ __device__ inline bool move_one(const StepParams& P, const ForceSys& S, double* 
 // re-expressed in the turned receptor frame.
 __device__ inline bool move_one_rpy(const StepParams& P, const ForceSys& S, double* pos,
                                     double* q, const double* FL, const double* TL,
-                                    const double* xi, double dt, double* x0, double* x1,
+                                    const double* xi, double dt, double* x0, double* x1, int xs,
                                     bool* rejected) {
   MoveSave sv;
-  move_begin(P, pos, q, x0, sv);
+  move_begin(P, pos, q, x0, xs, sv);
   double FR[3], TR[3], dX[12];
   rpy_receptor_load(S.torque_mode, pos, FL, TL, FR, TR);
   rpy_displacement(P.a_rec, P.a_lig, P.kT_over_eta, dt, pos, FR, FL, TR, TL, xi, dX);
   rpy_apply(dX, pos, q);
-  return move_end(P, S, pos, q, sv, x0, x1, rejected);
+  return move_end(P, S, pos, q, sv, x0, x1, xs, rejected);
 }
 
 // Brownian-bridge pair count (:1157-1184, :1236-1239).
 __device__ inline int bridge_count(const StepParams& P, const double* x0, const double* x1,
-                                   double dt, const double* u) {
+                                   int xs, double dt, const double* u) {
   int n = 0;
   for (int p = 0; p < P.n_pairs; ++p) {
+    const double a = x0[p * xs], b = x1[p * xs];
     double pc = 0.0;
-    if (x0[p] > 0 && x1[p] > 0) {
+    if (a > 0 && b > 0) {
       const double Ddt = fmax((P.has_bb_deff ? P.bb_deff[p] : P.D_t) * dt, 1e-30);
-      pc = exp(-x0[p] * x1[p] / Ddt);
+      pc = exp(-a * b / Ddt);
     }
-    const bool fired = (u[p] < pc) || (x1[p] < 0.0);
+    const bool fired = (u[p] < pc) || (b < 0.0);
     n += fired ? 1 : 0;
   }
   return n;
@@ -400,7 +403,7 @@ __global__ void k_replay_move(StepParams P, ForceSys S, ReplayBufs B, const doub
   bool rej;
   const bool any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt + 3 * (size_t)w,
                             nr + 3 * (size_t)w, dt, D, B.x0 + (size_t)w * PST_MAX_PAIRS,
-                            B.x1 + (size_t)w * PST_MAX_PAIRS, &rej);
+                            B.x1 + (size_t)w * PST_MAX_PAIRS, 1, &rej);
   if (rej) atomicAdd(&B.counts[6], 1);
   if (any) atomicOr(&B.counts[4], 1);
 }
@@ -421,7 +424,7 @@ __global__ void k_replay_bridge(StepParams P, ReplayBufs B, const double* u_bb, 
   const int s = B.sr_list[w];
   if (u_bb && P.n_pairs) {
     const int n = bridge_count(P, B.x0 + (size_t)w * PST_MAX_PAIRS,
-                               B.x1 + (size_t)w * PST_MAX_PAIRS, B.dt[w],
+                               B.x1 + (size_t)w * PST_MAX_PAIRS, 1, B.dt[w],
                                u_bb + (size_t)w * P.n_pairs);
     if (n >= P.n_needed && B.status[s] == PST_RUNNING) {
       B.status[s] = PST_REACTED;
@@ -548,16 +551,24 @@ hipError_t prod_init(const StepParams& P, const ProdBufs& B, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Wave-aggregated append to an unordered list.
-__device__ inline void wave_append(int* list, int* counter, int v, bool pred) {
+// Workgroup-aggregated append to an unordered list: the waves reserve their
+// ranges in LDS and the workgroup takes one global range (one atomic on the
+// counter per workgroup instead of one per wave; every wave of the launch
+// appends to the same counter).  All threads of the workgroup call it.
+__device__ inline void block_append(int* list, int* counter, int v, bool pred) {
+  __shared__ int wg_count, wg_base;
+  if (threadIdx.x == 0) wg_count = 0;
+  __syncthreads();
   const unsigned long long m = __ballot(pred);
-  if (m == 0ull) return;
   const int lane = __lane_id();
-  const int leader = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, __popcll(m));
-  base = __shfl(base, leader);
-  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+  const int leader = m ? __ffsll((long long)m) - 1 : 0;
+  int off = 0;
+  if (m && lane == leader) off = atomicAdd(&wg_count, __popcll(m));
+  off = __shfl(off, leader);
+  __syncthreads();
+  if (threadIdx.x == 0) wg_base = wg_count ? atomicAdd(counter, wg_count) : 0;
+  __syncthreads();
+  if (pred) list[wg_base + off + __popcll(m & ((1ull << lane) - 1ull))] = v;
 }
 
 // Start of step: stop test, reaction check, escape / return, refill,
@@ -630,7 +641,7 @@ __device__ inline void prod_pre_body(const StepParams& P, const ProdBufs& B, int
     }
   }
   RunCounters* Cw = B.ctr;
-  wave_append(B.mlp_list, &Cw->n_mlp, s, live && far);
+  block_append(B.mlp_list, &Cw->n_mlp, s, live && far);
   // Grid-path trajectories get a direction key for the locality sort.  For
   // the fused force kernel (no reverse list): lab octant of the centroid
   // (bits 5-7, the receptor-grid region it gathers from) and the octant +
@@ -772,29 +783,67 @@ __global__ void __launch_bounds__(256) k_bins_scan_group(const ProdBufs* Bg) {
   bins_scan_body(Bg[blockIdx.x]);
 }
 
+// The scatter of the counting sort.  Workgroup `blk` takes kSPT slots per
+// thread (blockDim.x * kSPT consecutive slots): it ranks its slots inside
+// each bin with LDS atomics, reserves one range per non-empty bin with one
+// global atomic on the bin's cursor, then writes.  Every slot of the launch
+// bumps one of ~216 shared cursors otherwise, and those atomics serialise
+// per address; with 4,096 slots per workgroup there is one per bin and
+// workgroup.  The order inside a bin is arbitrary either way.
+template <int kSPT>
 __device__ inline void bins_scatter_body(const ProdBufs& B, int blk) {
-  const int s = blk * blockDim.x + threadIdx.x;
-  if (s >= B.n_slots) return;
-  const int k = B.key[s];
-  if (k < 0) return;
-  const int p = atomicAdd(&B.bins[256 + (k & 255)], 1);
-  B.plain_list[p] = s;
-  if (B.rev_list) B.rev_list[atomicAdd(&B.bins[768 + (k >> 8)], 1)] = s;
+  __shared__ int cnt[256], cnt2[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) { cnt[i] = 0; cnt2[i] = 0; }
+  __syncthreads();
+  const int s0 = blk * blockDim.x * kSPT;
+  const bool rev = B.rev_list != nullptr;
+  int key[kSPT], loc[kSPT], loc2[kSPT];
+#pragma unroll
+  for (int j = 0; j < kSPT; ++j) {
+    const int s = s0 + j * blockDim.x + threadIdx.x;
+    const int k = s < B.n_slots ? B.key[s] : -1;
+    key[j] = k;
+    loc[j] = loc2[j] = 0;
+    if (k >= 0) {
+      loc[j] = atomicAdd(&cnt[k & 255], 1);
+      if (rev) loc2[j] = atomicAdd(&cnt2[k >> 8], 1);
+    }
+  }
+  __syncthreads();
+  // the bins' base offsets replace their local counts
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    cnt[i] = cnt[i] ? atomicAdd(&B.bins[256 + i], cnt[i]) : 0;
+    if (rev) cnt2[i] = cnt2[i] ? atomicAdd(&B.bins[768 + i], cnt2[i]) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSPT; ++j) {
+    const int k = key[j];
+    if (k < 0) continue;
+    const int s = s0 + j * blockDim.x + threadIdx.x;
+    B.plain_list[cnt[k & 255] + loc[j]] = s;
+    if (rev) B.rev_list[cnt2[k >> 8] + loc2[j]] = s;
+  }
 }
 
-__global__ void k_bins_scatter(ProdBufs B) { bins_scatter_body(B, blockIdx.x); }
+constexpr int kScatterSPT = 16;   // slots per thread of k_bins_scatter (4,096 per workgroup)
 
+__global__ void __launch_bounds__(256) k_bins_scatter(ProdBufs B) {
+  bins_scatter_body<kScatterSPT>(B, blockIdx.x);
+}
+
+// grouped: the step launches' partition (kStepBlock slots per workgroup)
 __global__ void k_bins_scatter_group(const ProdBufs* Bg, const int* start, int L) {
   const int l = slice_of(start, L, blockIdx.x);
-  bins_scatter_body(Bg[l], blockIdx.x - start[l]);
+  bins_scatter_body<1>(Bg[l], blockIdx.x - start[l]);
 }
 
 hipError_t prod_step_sort(const ProdBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_bins_scan, dim3(1), dim3(256), 0, st, B);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bins_scatter, dim3((B.n_slots + kStepBlock - 1) / kStepBlock),
-                     dim3(kStepBlock), 0, st, B);
+  const int per = 256 * kScatterSPT;
+  hipLaunchKernelGGL(k_bins_scatter, dim3((B.n_slots + per - 1) / per), dim3(256), 0, st, B);
   return hipGetLastError();
 }
 
@@ -840,22 +889,24 @@ __device__ inline void prod_post_body(const StepParams& P, const ForceSys& S, co
     const double D = diffusion(P, r);
     B.dt[s] = dt;
     bool rej;
-    double* x0 = B.x0 + (size_t)s * PST_MAX_PAIRS;
-    double* x1 = B.x1 + (size_t)s * PST_MAX_PAIRS;
+    // pair distances [pair][slot]: one coalesced row per pair
+    const int xs = B.n_slots;
+    double* x0 = B.x0 + s;
+    double* x1 = B.x1 + s;
     if (P.hydro_mode == 1) {
       double xi[12];
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, (uint32_t)k, xi[2 * k], xi[2 * k + 1]);
       any = move_one_rpy(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, xi, dt, x0, x1,
-                         &rej);
+                         xs, &rej);
     } else {
       double nt[3], nr[3];
       philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, 0, nt[0], nt[1]);
       philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, 1, nt[2], nr[0]);
       philox_normal2(P.seed, rng_key(P, t), ns, RNG_NOISE, 2, nr[1], nr[2]);
       any = move_one(P, S, pos, q, B.F + 3 * (size_t)s, B.T + 3 * (size_t)s, nt, nr, dt, D, x0, x1,
-                     &rej);
+                     xs, &rej);
     }
     if (rej) atomicAdd(&B.ctr->n_overlap, 1ull);
     if (OBS) {
@@ -889,13 +940,17 @@ __device__ inline void prod_post_body(const StepParams& P, const ForceSys& S, co
       const int c0 = tr0 + kObsTrans * kObsTrans;
       // contacts at the pose the step started from (gpu_batch_simulator.py:1300-1311)
       for (int p = 0; p < P.n_pairs; ++p)
-        if (x0[p] < 0.0) atomicAdd(&h[c0 + p], 1);
+        if (x0[p * xs] < 0.0) atomicAdd(&h[c0 + p], 1);
       atomicAdd(&h[c0 + PST_MAX_PAIRS], 1);
       O.s_prev_r[s] = rn;
     }
   }
   const unsigned long long m = __ballot(any);
-  if (m && __lane_id() == __ffsll((long long)m) - 1) atomicOr(&B.ctr->any_above[par], 1);
+  // one flag for the whole launch: set it only while it is still clear (once
+  // set, every later wave would serialise on the same address for nothing)
+  if (m && __lane_id() == __ffsll((long long)m) - 1 &&
+      __hip_atomic_load(&B.ctr->any_above[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    atomicOr(&B.ctr->any_above[par], 1);
   if (OBS) {
     __syncthreads();
     for (int i = threadIdx.x; i < kObsHist; i += blockDim.x)
@@ -951,25 +1006,28 @@ __device__ inline void prod_bridge_body(const StepParams& P, const ProdBufs& B, 
       u[p] = a;
       if (p + 1 < PST_MAX_PAIRS) u[p + 1] = b;
     }
-    const double* x0 = B.x0 + (size_t)s * PST_MAX_PAIRS;
-    const double* x1 = B.x1 + (size_t)s * PST_MAX_PAIRS;
+    // pair distances [pair][slot] (written by the post phase)
+    const int xs = B.n_slots;
+    const double* x0 = B.x0 + s;
+    const double* x1 = B.x1 + s;
     if (P.sm_active) {
       // fired pairs gated by the current state, lowest index (:1185-1234)
       const int state = B.sm_state[s];
       for (int p = 0; p < P.n_pairs; ++p) {
         if (P.sm_before[p] != state) continue;
+        const double a = x0[p * xs], b = x1[p * xs];
         double pc = 0.0;
-        if (x0[p] > 0 && x1[p] > 0) {
+        if (a > 0 && b > 0) {
           const double Ddt = fmax((P.has_bb_deff ? P.bb_deff[p] : P.D_t) * B.dt[s], 1e-30);
-          pc = exp(-x0[p] * x1[p] / Ddt);
+          pc = exp(-a * b / Ddt);
         }
-        if (u[p] < pc || x1[p] < 0.0) {
+        if (u[p] < pc || b < 0.0) {
           react = sm_fire(P, B, s, p, true);
           break;
         }
       }
     } else {
-      const int n = bridge_count(P, x0, x1, B.dt[s], u);
+      const int n = bridge_count(P, x0, x1, xs, B.dt[s], u);
       react = n >= P.n_needed;
     }
   }
