@@ -1511,10 +1511,18 @@ static hipError_t launch_audit_forces(pst_sim* s, const ForceSys& S, const Force
   return launch_forces(S, A, s->precision, group_for(s->ctx, S), s->sp.audit_n, 2048, st);
 }
 
-// Fork: the audit launch on the second stream, ordered after everything
-// enqueued so far on the context stream (captured graphs take it as a
-// parallel branch).
-static int fork_audit(pst_sim* s, const ForceSys& S, const ForceWork& W) {
+// Whether the step's surrogate work (the network's forward for the far
+// trajectories and the audit's grid-path launch) runs on the second stream.
+static bool forks(const pst_sim* s) {
+  return s->mlp && s->P.surrogate_r > 0;
+}
+
+// Fork: the surrogate trajectories' work on the second stream, ordered after
+// everything enqueued so far on the context stream (captured graphs take it
+// as a parallel branch beside the grid-path force launch).  It touches only
+// the surrogate list's slots of F / T, the network's buffers and the audit
+// buffers, none of which the grid-path launch reads or writes.
+static int fork_surr(pst_sim* s, const ForceSys& S, const ForceWork& W) {
   pst_ctx* ctx = s->ctx;
   if (!s->aux) {
     CK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
@@ -1523,26 +1531,28 @@ static int fork_audit(pst_sim* s, const ForceSys& S, const ForceWork& W) {
   }
   CK(hipEventRecord(s->fork_ev, ctx->stream));
   CK(hipStreamWaitEvent(s->aux, s->fork_ev, 0));
-  CK(launch_audit_forces(s, S, W, s->aux));
+  if (audits(s)) CK(launch_audit_forces(s, S, W, s->aux));
+  CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
+               s->aux));
   CK(hipEventRecord(s->join_ev, s->aux));
   return 0;
 }
 
 // Surrogate forces of the step and their audit (after the grid-path forces).
-// forked: the audit's grid-path launch was issued by fork_audit.
+// forked: fork_surr issued the forward and the audit's grid-path launch.
 static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool forked = false) {
   pst_ctx* ctx = s->ctx;
   if (s->mlp && s->P.surrogate_r > 0) {
-    CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
-                 ctx->stream));
-    if (s->sp.audit_n > 0) {
-      if (forked)
-        CK(hipStreamWaitEvent(ctx->stream, s->join_ev, 0));
-      else
-        CK(launch_audit_forces(s, S, W, ctx->stream));
+    if (forked) {
+      CK(hipStreamWaitEvent(ctx->stream, s->join_ev, 0));
+    } else {
+      CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F,
+                   s->T, ctx->stream));
+      if (s->sp.audit_n > 0) CK(launch_audit_forces(s, S, W, ctx->stream));
+    }
+    if (s->sp.audit_n > 0)
       CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
                     s->sp.use_torque ? s->audit_T : nullptr, s->ctr, ctx->stream));
-    }
   }
   return 0;
 }
@@ -1582,9 +1592,9 @@ static int enqueue_step(pst_sim* s, int par) {
   ForceSys S;
   fill_sys(ctx, S);
   const int G = s->tune_G > 0 ? s->tune_G : group_for(ctx, S);
-  const bool forked = audits(s);
+  const bool forked = forks(s);
   if (forked) {
-    const int r = fork_audit(s, S, W);
+    const int r = fork_surr(s, S, W);
     if (r) return r;
   }
   if (s->census_on) CK(launch_census(S, W, s->census, ctx->stream));
