@@ -701,7 +701,7 @@ def main():
         tc = "double" if head == "fp64" else "float"
         zp = "true" if lc and lc.get("mode") in ("zpair", "split_zpair") else "false"
         if lc and lc.get("mode", "").startswith("split"):
-            kname = f"k_rev_sort + k_forces_split<{tc}, {lc['G']}, {zp}> + k_rev_apply"
+            kname = f"k_forces_split<{tc}, {lc['G']}, {zp}> + k_rev_apply"
         else:
             kname = f"k_forces<{tc}, {lc['G']}, true, {zp}>" if lc else "k_forces"
         roof = {"bound": "hbm", "achieved": rf["achieved_gbs"], "peak": HBM_PEAK_GBS,
