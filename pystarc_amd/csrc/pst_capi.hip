@@ -1557,20 +1557,6 @@ static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool 
   return 0;
 }
 
-// The part of enqueue_surr after the network's forward (a grouped forward
-// of several simulators ran it): forces applied, then the audit.
-static int enqueue_surr_apply(pst_sim* s, const ForceSys& S, const ForceWork& W) {
-  pst_ctx* ctx = s->ctx;
-  CK(surr_apply(s->sp, s->mlp->m, s->pos, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
-                ctx->stream));
-  if (s->sp.audit_n > 0) {
-    CK(launch_audit_forces(s, S, W, ctx->stream));
-    CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
-                  s->sp.use_torque ? s->audit_T : nullptr, s->ctr, ctx->stream));
-  }
-  return 0;
-}
-
 // Step phases after the grid-path forces: surrogate forces and their audit,
 // the move, the Brownian bridge.
 static int enqueue_post(pst_sim* s, const ForceSys& S, const ForceWork& W, int par,
@@ -1999,6 +1985,43 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     mstart.push_back(mstart.back() + (Sp + kInferRows - 1) / kInferRows);
   }
   const int n_surr = (int)surr_l.size();
+  // their surrogate phase: features, the grouped forward and the applied
+  // forces (workgroup partition fstart), the audit sample and the audit's
+  // grid-path launch (one system per surrogate, partition astart)
+  std::vector<SurrTask> tasks(n_surr);
+  std::vector<ForceSys> Sa(n_surr);
+  std::vector<ForceWork> Wa(n_surr);
+  std::vector<int> fstart(n_surr + 1, 0), astart(n_surr + 1, 0);
+  for (int i = 0; i < n_surr; ++i) {
+    pst_sim* sl = sims[surr_l[i]];
+    SurrTask& t = tasks[i];
+    t.sp = sl->sp;
+    t.pos = sl->pos;
+    t.quat = sl->quat;
+    t.list = sl->mlp_list;
+    t.n_dev = &sl->ctr->n_mlp;
+    t.x0f = sl->mlp->m.x0f;
+    t.y = sl->mlp->m.y;
+    t.F = sl->F;
+    t.T = sl->T;
+    t.ctr = sl->ctr;
+    t.audit_list = sl->audit_list;
+    t.audit_count = sl->audit_count;
+    t.audit_F = sl->audit_F;
+    t.audit_T = sl->sp.use_torque ? sl->audit_T : nullptr;
+    fstart[i + 1] = fstart[i] + (sl->S + 255) / 256;
+    Sa[i] = S[surr_l[i]];
+    ForceWork A = W[surr_l[i]];
+    A.list = sl->audit_list;
+    A.n_dev = sl->audit_count;
+    A.n_cap = sl->sp.audit_n;
+    A.out_compact = 1;
+    A.xcd_chunks = 0;
+    A.F = sl->audit_F;
+    A.T = sl->audit_T;
+    Wa[i] = A;
+    astart[i + 1] = astart[i] + (audits(sl) ? (sl->sp.audit_n + gpb - 1) / gpb : 0);
+  }
   ForceSys* dS = nullptr;
   ForceWork* dW = nullptr;
   StepParams* dP = nullptr;
@@ -2007,6 +2030,13 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
   int* dsstart = nullptr;
   MlpInfer* dMI = nullptr;
   int* dmstart = nullptr;
+  SurrTask* dT = nullptr;
+  ForceSys* dSa = nullptr;
+  ForceWork* dWa = nullptr;
+  int* dfstart = nullptr;
+  int* dastart = nullptr;
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   auto cleanup = [&]() {
     for (int l = 0; l < L; ++l) sims[l]->ctx->stream = saved[l];
     if (dS) hipFree(dS);
@@ -2017,6 +2047,14 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     if (dsstart) hipFree(dsstart);
     if (dMI) hipFree(dMI);
     if (dmstart) hipFree(dmstart);
+    if (dT) hipFree(dT);
+    if (dSa) hipFree(dSa);
+    if (dWa) hipFree(dWa);
+    if (dfstart) hipFree(dfstart);
+    if (dastart) hipFree(dastart);
+    if (fork_ev) hipEventDestroy(fork_ev);
+    if (join_ev) hipEventDestroy(join_ev);
+    if (aux) hipStreamDestroy(aux);
   };
   if (hipMalloc(&dS, sizeof(ForceSys) * L) != hipSuccess ||
       hipMalloc(&dW, sizeof(ForceWork) * L) != hipSuccess ||
@@ -2025,16 +2063,36 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       hipMalloc(&dstart, sizeof(int) * (L + 1)) != hipSuccess ||
       hipMalloc(&dsstart, sizeof(int) * (L + 1)) != hipSuccess ||
       (n_surr > 0 && (hipMalloc(&dMI, sizeof(MlpInfer) * n_surr) != hipSuccess ||
-                      hipMalloc(&dmstart, sizeof(int) * (n_surr + 1)) != hipSuccess))) {
+                      hipMalloc(&dmstart, sizeof(int) * (n_surr + 1)) != hipSuccess ||
+                      hipMalloc(&dT, sizeof(SurrTask) * n_surr) != hipSuccess ||
+                      hipMalloc(&dSa, sizeof(ForceSys) * n_surr) != hipSuccess ||
+                      hipMalloc(&dWa, sizeof(ForceWork) * n_surr) != hipSuccess ||
+                      hipMalloc(&dfstart, sizeof(int) * (n_surr + 1)) != hipSuccess ||
+                      hipMalloc(&dastart, sizeof(int) * (n_surr + 1)) != hipSuccess))) {
     cleanup();
     FAIL("pst_group_run: out of device memory");
   }
   if (n_surr > 0 &&
       (hipMemcpy(dMI, MI.data(), sizeof(MlpInfer) * n_surr, hipMemcpyHostToDevice) != hipSuccess ||
        hipMemcpy(dmstart, mstart.data(), sizeof(int) * (n_surr + 1), hipMemcpyHostToDevice) !=
+           hipSuccess ||
+       hipMemcpy(dT, tasks.data(), sizeof(SurrTask) * n_surr, hipMemcpyHostToDevice) !=
+           hipSuccess ||
+       hipMemcpy(dSa, Sa.data(), sizeof(ForceSys) * n_surr, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dWa, Wa.data(), sizeof(ForceWork) * n_surr, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dfstart, fstart.data(), sizeof(int) * (n_surr + 1), hipMemcpyHostToDevice) !=
+           hipSuccess ||
+       hipMemcpy(dastart, astart.data(), sizeof(int) * (n_surr + 1), hipMemcpyHostToDevice) !=
            hipSuccess)) {
     cleanup();
     FAIL("pst_group_run: upload failed");
+  }
+  if (n_surr > 0 &&
+      (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&join_ev, hipEventDisableTiming) != hipSuccess)) {
+    cleanup();
+    FAIL("pst_group_run: stream / event creation failed");
   }
   if (hipMemcpy(dS, S.data(), sizeof(ForceSys) * L, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dW, W.data(), sizeof(ForceWork) * L, hipMemcpyHostToDevice) != hipSuccess ||
@@ -2046,28 +2104,33 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     cleanup();
     FAIL("pst_group_run: upload failed");
   }
-  // one step of every system: each phase is one launch over all systems, the
-  // surrogate phase (own network per system) one set of launches per system
+  // one step of every system: each phase is one launch over all systems; the
+  // surrogate phase (features, every network's forward, applied forces, the
+  // audit sample and its grid-path launch) runs on the second stream beside
+  // the grouped force launch, touching only the surrogate lists' slots and
+  // the networks' and audits' buffers; the audit comparison after the join
   auto step = [&](int par) -> int {
     const hipStream_t cs = sims[0]->ctx->stream;
+    const int prec = sims[0]->precision;
     hipError_t e = prod_group_pre(dP, dB, dsstart, L, sstart[L], par, any_outer, cs);
-    if (e == hipSuccess)
-      e = launch_forces_group(dS, dW, dstart, L, start[L], lds, sims[0]->precision, G, zp, cs);
-    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     if (n_surr > 0) {
-      for (int l : surr_l) {
-        pst_sim* sl = sims[l];
-        e = surr_features(sl->sp, sl->mlp->m, sl->pos, sl->quat, sl->mlp_list, &sl->ctr->n_mlp,
-                          sl->S, cs);
-        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
-      }
-      e = launch_mlp_infer_group(dMI, dmstart, n_surr, mstart[n_surr], cs);
-      if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
-      for (int l : surr_l) {
-        const int r = enqueue_surr_apply(sims[l], S[l], W[l]);
-        if (r) return r;
-      }
+      if (e == hipSuccess) e = hipEventRecord(fork_ev, cs);
+      if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork_ev, 0);
+      if (e == hipSuccess) e = surr_group_audit_pick(dT, n_surr, aux);
+      if (e == hipSuccess)
+        e = launch_forces_group(dSa, dWa, dastart, n_surr, astart[n_surr], lds, prec, G, false,
+                                aux);
+      if (e == hipSuccess) e = surr_group_features(dT, dfstart, n_surr, fstart[n_surr], aux);
+      if (e == hipSuccess) e = launch_mlp_infer_group(dMI, dmstart, n_surr, mstart[n_surr], aux);
+      if (e == hipSuccess) e = surr_group_apply(dT, dfstart, n_surr, fstart[n_surr], aux);
+      if (e == hipSuccess) e = hipEventRecord(join_ev, aux);
     }
+    if (e == hipSuccess) e = launch_forces_group(dS, dW, dstart, L, start[L], lds, prec, G, zp, cs);
+    if (n_surr > 0) {
+      if (e == hipSuccess) e = hipStreamWaitEvent(cs, join_ev, 0);
+      if (e == hipSuccess) e = surr_group_audit(dT, n_surr, cs);
+    }
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     e = prod_group_post(dP, dS, dB, dsstart, L, sstart[L], obs, par, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     return 0;

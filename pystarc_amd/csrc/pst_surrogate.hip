@@ -26,10 +26,8 @@ __device__ inline double envelope(const SurrParams& S, double r) {
 }
 
 // FP32 features of list entry w into row w of x0f (the first layer runs in FP32).
-__global__ void k_surr_features(SurrParams S, const double* pos, const double* quat,
-                                 const int* list, const int* n_dev, float* x0f) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= *n_dev) return;
+__device__ inline void features_row(const SurrParams& S, const double* pos, const double* quat,
+                                    const int* list, float* x0f, int w) {
   const int s = list[w];
   double R[9];
   quat_to_rot(quat + 4 * (size_t)s, R);
@@ -41,10 +39,9 @@ __global__ void k_surr_features(SurrParams S, const double* pos, const double* q
   row[6] = (float)R[1]; row[7] = (float)R[4]; row[8] = (float)R[7];
 }
 
-__global__ void k_surr_apply(SurrParams S, const double* pos, const int* list, const int* n_dev,
-                             const float* y, double* F, double* T) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= *n_dev) return;
+// Force (and torque) of list entry w from row w of the network's output.
+__device__ inline void apply_row(const SurrParams& S, const double* pos, const int* list,
+                                 const float* y, double* F, double* T, int w) {
   const int s = list[w];
   const double r = norm3(pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]);
   const double env = envelope(S, r);
@@ -58,15 +55,28 @@ __global__ void k_surr_apply(SurrParams S, const double* pos, const int* list, c
   }
 }
 
+__global__ void k_surr_features(SurrParams S, const double* pos, const double* quat,
+                                 const int* list, const int* n_dev, float* x0f) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= *n_dev) return;
+  features_row(S, pos, quat, list, x0f, w);
+}
+
+__global__ void k_surr_apply(SurrParams S, const double* pos, const int* list, const int* n_dev,
+                             const float* y, double* F, double* T) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= *n_dev) return;
+  apply_row(S, pos, list, y, F, T, w);
+}
+
 // Audit sample of one step: k = min(n, cap) rows of the surrogate list at a
 // stride of n / k from an offset that a hash of the run's trajectory-step
 // counter moves every step (stable within the step: only the pre and post
 // phases change it), so the audit sees every part of the list over a run.
-__global__ void k_surr_audit_pick(const int* list, const int* n_dev, int cap,
-                                  const RunCounters* C, int* out, int* n_out) {
-  const int n = *n_dev;
+// Entry i of the sample (i == 0 also stores k).
+__device__ inline void audit_pick_entry(const int* list, int n, int cap, const RunCounters* C,
+                                        int* out, int* n_out, int i) {
   const int k = min(n, cap);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *n_out = k;
   if (i >= k) return;
   unsigned long long h = (C->total_steps + 1) * 0x9E3779B97F4A7C15ull;
@@ -78,15 +88,19 @@ __global__ void k_surr_audit_pick(const int* list, const int* n_dev, int cap,
   out[i] = list[(off + (long long)i * stride) % n];
 }
 
+__global__ void k_surr_audit_pick(const int* list, const int* n_dev, int cap,
+                                  const RunCounters* C, int* out, int* n_out) {
+  audit_pick_entry(list, *n_dev, cap, C, out, n_out, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Squared errors of the surrogate's forces (and, with Tref, torques) against
-// the grid path on the audited rows, summed into the run counters.
-__global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const double* F,
-                             const double* T, const double* Fref, const double* Tref,
-                             RunCounters* C) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = min(*n_dev, cap);
+// the grid path on audited rows w = w0, w0 + dw, ... < n, summed into the run
+// counters (one atomic per wave and sum).
+__device__ inline void audit_rows(const int* list, int n, int w0, int dw, const double* F,
+                                  const double* T, const double* Fref, const double* Tref,
+                                  RunCounters* C) {
   double e2 = 0, r2 = 0, te2 = 0, tr2 = 0;
-  if (w < n) {
+  for (int w = w0; w < n; w += dw) {
     const int s = list[w];
     for (int d = 0; d < 3; ++d) {
       const double a = F[3 * s + d], b = Fref[3 * w + d];
@@ -101,7 +115,6 @@ __global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const d
       }
     }
   }
-  // wave reduction, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     e2 += __shfl_xor(e2, off);
     r2 += __shfl_xor(r2, off);
@@ -116,7 +129,52 @@ __global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const d
     atomicAdd(&C->audit_terr2, te2);
     atomicAdd(&C->audit_tref2, tr2);
   }
-  if (w == 0) atomicAdd(&C->audit_n, (unsigned long long)n);
+  if (w0 == 0) atomicAdd(&C->audit_n, (unsigned long long)n);
+}
+
+__global__ void k_surr_audit(const int* list, const int* n_dev, int cap, const double* F,
+                             const double* T, const double* Fref, const double* Tref,
+                             RunCounters* C) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  audit_rows(list, min(*n_dev, cap), w, 1 << 30, F, T, Fref, Tref, C);
+}
+
+// Grouped forms: task l of L for workgroup b (start[l] <= b < start[l + 1]).
+__device__ inline int task_of(const int* start, int L, int b) {
+  int l = 0;
+  while (l + 1 < L && b >= start[l + 1]) ++l;
+  return l;
+}
+
+__global__ void k_surr_group_features(const SurrTask* __restrict__ Tk, const int* start, int L) {
+  const int l = task_of(start, L, blockIdx.x);
+  const SurrTask& t = Tk[l];
+  const int w = (blockIdx.x - start[l]) * blockDim.x + threadIdx.x;
+  if (w >= *t.n_dev) return;
+  features_row(t.sp, t.pos, t.quat, t.list, t.x0f, w);
+}
+
+__global__ void k_surr_group_apply(const SurrTask* __restrict__ Tk, const int* start, int L) {
+  const int l = task_of(start, L, blockIdx.x);
+  const SurrTask& t = Tk[l];
+  const int w = (blockIdx.x - start[l]) * blockDim.x + threadIdx.x;
+  if (w >= *t.n_dev) return;
+  apply_row(t.sp, t.pos, t.list, t.y, t.F, t.T, w);
+}
+
+// one workgroup per task
+__global__ void k_surr_group_audit_pick(const SurrTask* __restrict__ Tk) {
+  const SurrTask& t = Tk[blockIdx.x];
+  const int n = *t.n_dev;
+  for (int i = threadIdx.x; i < max(t.sp.audit_n, 1); i += blockDim.x)
+    audit_pick_entry(t.list, n, t.sp.audit_n, t.ctr, t.audit_list, t.audit_count, i);
+}
+
+__global__ void k_surr_group_audit(const SurrTask* __restrict__ Tk) {
+  const SurrTask& t = Tk[blockIdx.x];
+  if (t.sp.audit_n <= 0) return;
+  audit_rows(t.audit_list, min(*t.n_dev, t.sp.audit_n), threadIdx.x, blockDim.x, t.F, t.T,
+             t.audit_F, t.audit_T, t.ctr);
 }
 
 hipError_t surr_features(const SurrParams& S, const MlpBufs& m, const double* pos,
@@ -157,6 +215,31 @@ hipError_t surr_audit(const int* list, const int* n_dev, int cap, const double* 
                       const double* Fref, const double* Tref, RunCounters* C, hipStream_t st) {
   hipLaunchKernelGGL(k_surr_audit, dim3((cap + 255) / 256), dim3(256), 0, st, list, n_dev, cap, F,
                      T, Fref, Tref, C);
+  return hipGetLastError();
+}
+
+hipError_t surr_group_features(const SurrTask* T, const int* start, int L, int blocks,
+                               hipStream_t st) {
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_surr_group_features, dim3(blocks), dim3(256), 0, st, T, start, L);
+  return hipGetLastError();
+}
+
+hipError_t surr_group_apply(const SurrTask* T, const int* start, int L, int blocks, hipStream_t st) {
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_surr_group_apply, dim3(blocks), dim3(256), 0, st, T, start, L);
+  return hipGetLastError();
+}
+
+hipError_t surr_group_audit_pick(const SurrTask* T, int L, hipStream_t st) {
+  if (L <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_surr_group_audit_pick, dim3(L), dim3(256), 0, st, T);
+  return hipGetLastError();
+}
+
+hipError_t surr_group_audit(const SurrTask* T, int L, hipStream_t st) {
+  if (L <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_surr_group_audit, dim3(L), dim3(256), 0, st, T);
   return hipGetLastError();
 }
 
