@@ -1908,6 +1908,47 @@ int pst_prod_set_launch(pst_sim* s, int mode, int G, int blocks_cap, int xcd) {
   return s->prod_ready ? sim_ensure_bins(s) : 0;
 }
 
+// Workgroup partition of a grouped force launch, XCD by XCD (per workgroups
+// each, per >= L): systems are placed longest-first (weights w) on the XCD
+// with the least weight so far, and each XCD's workgroups are split among its
+// systems in proportion to their weights, one at least each.  Slice i =
+// [start[i], start[i + 1]) of the XCD-major workgroup index belongs to
+// system order[i]; the slices of XCD x lie in [x per, (x + 1) per).
+static void xcd_partition(const std::vector<double>& w, int per, std::vector<int>& start,
+                          std::vector<int>& order) {
+  const int L = (int)w.size();
+  std::vector<int> idx(L), xcd(L);
+  for (int l = 0; l < L; ++l) idx[l] = l;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return w[a] > w[b]; });
+  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int l : idx) {
+    int x = 0;
+    for (int y = 1; y < 8; ++y)
+      if (load[y] < load[x]) x = y;
+    xcd[l] = x;
+    load[x] += w[l];
+  }
+  start.assign(L + 1, 0);
+  order.assign(L, 0);
+  int i = 0;
+  for (int x = 0; x < 8; ++x) {
+    int k = 0;
+    for (int l = 0; l < L; ++l) k += xcd[l] == x;
+    const int extra = per - k;   // every system keeps one workgroup
+    double cum = 0;
+    int j = 0;
+    for (int l = 0; l < L; ++l) {
+      if (xcd[l] != x) continue;
+      start[i] = x * per + j + (int)(extra * (load[x] > 0 ? cum / load[x] : 0.0));
+      order[i] = l;
+      cum += w[l];
+      ++i;
+      ++j;
+    }
+  }
+  start[L] = 8 * per;
+}
+
 // Several simulators (one ligand each, receptor grids shared) stepped in
 // lock-step: per step every simulator's pre phases, ONE force launch over
 // all of their grid-path lists, then every simulator's post phases; captured
@@ -1971,6 +2012,20 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     b = (b + 7) / 8 * 8;
     start[l + 1] = start[l] + b;
   }
+  // With 8 or more systems each system's workgroups run on one XCD, so an
+  // XCD's L2 holds the ligand grids of about L / 8 systems instead of all L
+  // (xcd_partition; re-balanced between graph launches from the systems'
+  // grid-path counts of the last step).
+  const bool xcd_major = L >= 8;
+  const int n_wg = start[L];
+  std::vector<int> order(L);
+  for (int l = 0; l < L; ++l) order[l] = l;
+  if (xcd_major) {
+    std::vector<double> w(L);
+    for (int l = 0; l < L; ++l) w[l] = start[l + 1] - start[l];
+    xcd_partition(w, n_wg / 8, start, order);
+    for (int l = 0; l < L; ++l) W[l].xcd_chunks = 0;
+  }
   // the simulators whose surrogate is attached: one grouped network forward
   // per step over all of them
   std::vector<int> surr_l;
@@ -2030,6 +2085,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
   int* dsstart = nullptr;
   MlpInfer* dMI = nullptr;
   int* dmstart = nullptr;
+  int* dorder = nullptr;
   SurrTask* dT = nullptr;
   ForceSys* dSa = nullptr;
   ForceWork* dWa = nullptr;
@@ -2047,6 +2103,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     if (dsstart) hipFree(dsstart);
     if (dMI) hipFree(dMI);
     if (dmstart) hipFree(dmstart);
+    if (dorder) hipFree(dorder);
     if (dT) hipFree(dT);
     if (dSa) hipFree(dSa);
     if (dWa) hipFree(dWa);
@@ -2061,6 +2118,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       hipMalloc(&dP, sizeof(StepParams) * L) != hipSuccess ||
       hipMalloc(&dB, sizeof(ProdBufs) * L) != hipSuccess ||
       hipMalloc(&dstart, sizeof(int) * (L + 1)) != hipSuccess ||
+      hipMalloc(&dorder, sizeof(int) * L) != hipSuccess ||
       hipMalloc(&dsstart, sizeof(int) * (L + 1)) != hipSuccess ||
       (n_surr > 0 && (hipMalloc(&dMI, sizeof(MlpInfer) * n_surr) != hipSuccess ||
                       hipMalloc(&dmstart, sizeof(int) * (n_surr + 1)) != hipSuccess ||
@@ -2099,6 +2157,7 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       hipMemcpy(dP, P.data(), sizeof(StepParams) * L, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dB, B.data(), sizeof(ProdBufs) * L, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dstart, start.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dorder, order.data(), sizeof(int) * L, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dsstart, sstart.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) !=
           hipSuccess) {
     cleanup();
@@ -2125,7 +2184,8 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       if (e == hipSuccess) e = surr_group_apply(dT, dfstart, n_surr, fstart[n_surr], aux);
       if (e == hipSuccess) e = hipEventRecord(join_ev, aux);
     }
-    if (e == hipSuccess) e = launch_forces_group(dS, dW, dstart, L, start[L], lds, prec, G, zp, cs);
+    if (e == hipSuccess)
+      e = launch_forces_group(dS, dW, dstart, L, n_wg, lds, prec, G, zp, cs, xcd_major, dorder);
     if (n_surr > 0) {
       if (e == hipSuccess) e = hipStreamWaitEvent(cs, join_ev, 0);
       if (e == hipSuccess) e = surr_group_audit(dT, n_surr, cs);
@@ -2192,6 +2252,26 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
     for (int l = 0; l < L; ++l)
       if (!(h[l].stop || (h[l].n_sr + h[l].n_mlp) == 0)) all_done = false;
     stop = all_done;
+    if (xcd_major && !stop) {
+      // weights: each system's grid-path trajectories of the last step, plus a
+      // floor so that a system whose count grows within the next graph
+      // launch keeps a share
+      std::vector<double> w(L);
+      double mean = 0;
+      for (int l = 0; l < L; ++l) mean += (double)h[l].n_sr / L;
+      for (int l = 0; l < L; ++l) w[l] = (double)h[l].n_sr + 0.05 * mean + 1.0;
+      std::vector<int> st2(L + 1), or2(L);
+      xcd_partition(w, n_wg / 8, st2, or2);
+      if (st2 != start || or2 != order) {
+        start = st2;
+        order = or2;
+        hipError_t e2 = hipMemcpyAsync(dstart, start.data(), sizeof(int) * (L + 1),
+                                       hipMemcpyHostToDevice, st);
+        if (e2 == hipSuccess)
+          e2 = hipMemcpyAsync(dorder, order.data(), sizeof(int) * L, hipMemcpyHostToDevice, st);
+        if (e2 != hipSuccess) { ctx->err = hipGetErrorString(e2); rc = 2; break; }
+      }
+    }
   }
   hipEventRecord(e1, st);
   hipEventSynchronize(e1);

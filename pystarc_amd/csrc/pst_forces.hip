@@ -381,50 +381,67 @@ __global__ void __launch_bounds__(256) k_rev_apply(const int* __restrict__ list,
 
 
 // Several systems (one ligand each, sharing the receptor grids) in one
-// launch: workgroup b belongs to system l with start[l] <= b < start[l + 1]
-// (start[] multiples of 8, so the XCD of a workgroup is that of its slice).
+// launch: slice i = [start[i], start[i + 1]) of the workgroup index v belongs
+// to system order[i] (system i when order is null), with v = b (start[]
+// multiples of 8, so the XCD of a workgroup is that of its slice) or, with
+// xcd_major, v = (b % 8) * (gridDim.x / 8) + b / 8: the workgroups of XCD x
+// are v = x * gridDim.x / 8 ...; a system whose slice lies in one XCD's
+// range runs on that XCD alone, and its ligand grid is read through that
+// XCD's L2 only.
 template <typename TC, int G, bool ZP>
 __global__ void __launch_bounds__(kForceBlock) k_forces_group(const ForceSys* __restrict__ Sg,
                                                               const ForceWork* __restrict__ Wg,
-                                                              const int* __restrict__ start, int L) {
-  const int b = blockIdx.x;
-  int l = 0;
-  while (l + 1 < L && b >= start[l + 1]) ++l;
-  // l is the same for the whole workgroup: made wave-uniform, the system's
-  // records are read with scalar loads instead of being held per lane
-  l = __builtin_amdgcn_readfirstlane(l);
-  const int b0 = __builtin_amdgcn_readfirstlane(start[l]);
-  const int b1 = __builtin_amdgcn_readfirstlane(start[l + 1]);
+                                                              const int* __restrict__ start, int L,
+                                                              int xcd_major,
+                                                              const int* __restrict__ order) {
+  const int b = xcd_major ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+  int i = 0;
+  while (i + 1 < L && b >= start[i + 1]) ++i;
+  // the slice is the same for the whole workgroup: made wave-uniform, the
+  // system's records are read with scalar loads instead of being held per lane
+  i = __builtin_amdgcn_readfirstlane(i);
+  const int b0 = __builtin_amdgcn_readfirstlane(start[i]);
+  const int b1 = __builtin_amdgcn_readfirstlane(start[i + 1]);
+  const int l = order ? __builtin_amdgcn_readfirstlane(order[i]) : i;
   forces_body<TC, G, true, ZP>(Sg[l], Wg[l], b - b0, b1 - b0);
 }
 
 template <typename TC, int G>
 static hipError_t launch_group_g(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
-                                 int blocks, size_t lds, bool zp, hipStream_t st) {
+                                 int blocks, size_t lds, bool zp, int xm, const int* order,
+                                 hipStream_t st) {
   if (zp)
     hipLaunchKernelGGL((k_forces_group<TC, G, true>), dim3(blocks), dim3(kForceBlock), lds, st, Sg,
-                       Wg, start, L);
+                       Wg, start, L, xm, order);
   else
     hipLaunchKernelGGL((k_forces_group<TC, G, false>), dim3(blocks), dim3(kForceBlock), lds, st,
-                       Sg, Wg, start, L);
+                       Sg, Wg, start, L, xm, order);
   return hipGetLastError();
 }
 
 hipError_t launch_forces_group(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
                                int blocks, size_t lds, int precision, int G, bool zp,
-                               hipStream_t st) {
+                               hipStream_t st, bool xcd_major, const int* order) {
   if (blocks <= 0) return hipSuccess;
+  if (xcd_major && (blocks & 7)) return hipErrorInvalidValue;
+  const int xm = xcd_major ? 1 : 0;
   if (precision == 0) {
     switch (G) {
-      case 16: return launch_group_g<double, 16>(Sg, Wg, start, L, blocks, lds, zp, st);
-      case 32: return launch_group_g<double, 32>(Sg, Wg, start, L, blocks, lds, zp, st);
-      default: return launch_group_g<double, 64>(Sg, Wg, start, L, blocks, lds, zp, st);
+      case 16:
+        return launch_group_g<double, 16>(Sg, Wg, start, L, blocks, lds, zp, xm, order, st);
+      case 32:
+        return launch_group_g<double, 32>(Sg, Wg, start, L, blocks, lds, zp, xm, order, st);
+      default:
+        return launch_group_g<double, 64>(Sg, Wg, start, L, blocks, lds, zp, xm, order, st);
     }
   }
   switch (G) {
-    case 16: return launch_group_g<float, 16>(Sg, Wg, start, L, blocks, lds, zp, st);
-    case 32: return launch_group_g<float, 32>(Sg, Wg, start, L, blocks, lds, zp, st);
-    default: return launch_group_g<float, 64>(Sg, Wg, start, L, blocks, lds, zp, st);
+    case 16:
+      return launch_group_g<float, 16>(Sg, Wg, start, L, blocks, lds, zp, xm, order, st);
+    case 32:
+      return launch_group_g<float, 32>(Sg, Wg, start, L, blocks, lds, zp, xm, order, st);
+    default:
+      return launch_group_g<float, 64>(Sg, Wg, start, L, blocks, lds, zp, xm, order, st);
   }
 }
 

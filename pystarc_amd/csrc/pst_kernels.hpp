@@ -18,11 +18,13 @@ hipError_t launch_chunk_min_r(const double* pos, const int* list, int n, int fb,
                               double* out, int n_chunks, hipStream_t st);
 int pick_group(int M, int n_rec_active);
 // One launch over L systems (device arrays of ForceSys / ForceWork, pose
-// mode); workgroups [start[l], start[l+1]) evaluate system l; zp: z-pair /
-// z-row fetches (the same node values).
+// mode); workgroups [start[i], start[i+1]) evaluate system order[i] (system
+// i when order is null); xcd_major: start[] indexes workgroups XCD by XCD
+// (blocks a multiple of 8); zp: z-pair / z-row fetches (the same node values).
 hipError_t launch_forces_group(const ForceSys* Sg, const ForceWork* Wg, const int* start, int L,
                                int blocks, size_t lds, int precision, int G, bool zp,
-                               hipStream_t st);
+                               hipStream_t st, bool xcd_major = false,
+                               const int* order = nullptr);
 // Gathers of one force launch over W's list (pose source): out4 += {trajectories,
 // elec-grid atoms, born-grid atoms, reverse-grid atoms}.
 hipError_t launch_census(const ForceSys& S, const ForceWork& W, unsigned long long* out4,

@@ -14,12 +14,12 @@ from pystarc_amd.simulation.gpu_batch_simulator import GPUBatchGroup, GPUBatchSi
 from tests._systems import dx_grids, numpy_grids  # noqa: E402
 
 
-def _small(n_traj=600, precision="fp64"):
-    """Four ligands of 12-30 atoms against a 300-atom receptor; every engine
+def _small(n_traj=600, precision="fp64", n_lig=4):
+    """n_lig ligands of 12-30 atoms against a 300-atom receptor; every engine
     uses 64 lanes per trajectory, the group launch's width, so the FP64
     per-trajectory sums (and with them the trajectories) are identical."""
-    ls = S.multi_ligand(n_traj=n_traj, n_rec=300, n_lig=4, lig_atoms=(12, 30), rec_grid=(41, 0.5),
-                        lig_grid=(25, 0.4), b=16.0, max_steps=20000)
+    ls = S.multi_ligand(n_traj=n_traj, n_rec=300, n_lig=n_lig, lig_atoms=(12, 30),
+                        rec_grid=(41, 0.5), lig_grid=(25, 0.4), b=16.0, max_steps=20000)
     engines = []
     for k, s in enumerate(ls):
         g = dx_grids(numpy_grids(s))
@@ -64,12 +64,28 @@ def test_group_run_equals_separate_runs(graph_steps):
     assert sum(r.n_reacted for r in grp) > 0
 
 
-def test_group_with_surrogates_equals_separate_runs():
+@pytest.mark.parametrize("graph_steps", [0, 8])
+def test_group_of_nine_equals_separate_runs(graph_steps):
+    """Nine ligands: each ligand's force work runs on one XCD (the partition
+    re-balanced between graph launches from the last step's grid-path
+    counts), and every ligand's trajectories are still those of its own run."""
+    ls, engines = _small(n_traj=300, n_lig=9)
+    sep = [_sim(s, e).run() for s, e in zip(ls, engines)]
+    grp = GPUBatchGroup([_sim(s, e) for s, e in zip(ls, engines)], graph_steps=graph_steps).run()
+    for a, b in zip(sep, grp):
+        assert (a.n_reacted, a.n_escaped, a.total_steps) == (b.n_reacted, b.n_escaped, b.total_steps)
+        assert np.array_equal(a.sim_data["outcome"], b.sim_data["outcome"])
+        assert np.array_equal(a.sim_data["n_steps"], b.sim_data["n_steps"])
+
+
+@pytest.mark.parametrize("n_lig", [4, 8])
+def test_group_with_surrogates_equals_separate_runs(n_lig):
     """Each ligand with its own surrogate: the group step runs every network
-    in one grouped forward launch, and every ligand's trajectories are those
-    of its separate run with the same network."""
+    in one grouped forward launch (the surrogate phase of all ligands on the
+    second stream beside the grouped force launch), and every ligand's
+    trajectories are those of its separate run with the same network."""
     from pystarc_amd.surrogate import build_surrogate
-    ls, engines = _small(n_traj=400)
+    ls, engines = _small(n_traj=400, n_lig=n_lig)
     surr = [build_surrogate(e, s, n_train=100_000, epochs=2, batch=2048, shell=2.0,
                             max_rows=2048, seed=k)
             for k, (s, e) in enumerate(zip(ls, engines))]
