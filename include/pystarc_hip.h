@@ -229,8 +229,10 @@ int pst_prod_obs_sizes(int* traj_doubles, int* hist_len);
 /* Several simulators (one ligand each; receptor grids shared with
  * pst_share_receptor_grids; same precision) stepped in lock-step: per step
  * each simulator's pre phases, ONE force launch over all grid-path lists
- * (workgroups partitioned by simulator), each simulator's post phases;
- * graph_steps per captured hipGraph.  c16 = (L,16) counters as
+ * (workgroups partitioned by simulator; with L >= 8 each simulator's
+ * workgroups on one XCD, re-balanced between graph launches), the surrogate
+ * phase of every simulator with a surrogate beside it on a second stream,
+ * each simulator's post phases; graph_steps per captured hipGraph.  c16 = (L,16) counters as
  * pst_prod_run; ms_per_step = device time per group step.  No reference
  * counterpart (the reference runs one ligand per process, multi_GPU_runs.py). */
 int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, double* c16,
