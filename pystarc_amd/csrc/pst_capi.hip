@@ -1534,12 +1534,18 @@ static int fork_surr(pst_sim* s, const ForceSys& S, const ForceWork& W) {
   if (audits(s)) CK(launch_audit_forces(s, S, W, s->aux));
   CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F, s->T,
                s->aux));
+  // the audit compares the network's forces with the audit's grid-path
+  // forces, both produced on this stream
+  if (audits(s))
+    CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
+                  s->sp.use_torque ? s->audit_T : nullptr, s->ctr, s->aux));
   CK(hipEventRecord(s->join_ev, s->aux));
   return 0;
 }
 
 // Surrogate forces of the step and their audit (after the grid-path forces).
-// forked: fork_surr issued the forward and the audit's grid-path launch.
+// forked: fork_surr issued the forward, the audit's grid-path launch and the
+// audit comparison.
 static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool forked = false) {
   pst_ctx* ctx = s->ctx;
   if (s->mlp && s->P.surrogate_r > 0) {
@@ -1548,11 +1554,12 @@ static int enqueue_surr(pst_sim* s, const ForceSys& S, const ForceWork& W, bool 
     } else {
       CK(surr_step(s->sp, s->mlp->m, s->pos, s->quat, s->mlp_list, &s->ctr->n_mlp, s->S, s->F,
                    s->T, ctx->stream));
-      if (s->sp.audit_n > 0) CK(launch_audit_forces(s, S, W, ctx->stream));
+      if (s->sp.audit_n > 0) {
+        CK(launch_audit_forces(s, S, W, ctx->stream));
+        CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
+                      s->sp.use_torque ? s->audit_T : nullptr, s->ctr, ctx->stream));
+      }
     }
-    if (s->sp.audit_n > 0)
-      CK(surr_audit(s->audit_list, s->audit_count, s->sp.audit_n, s->F, s->T, s->audit_F,
-                    s->sp.use_torque ? s->audit_T : nullptr, s->ctr, ctx->stream));
   }
   return 0;
 }
@@ -2165,9 +2172,9 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
   }
   // one step of every system: each phase is one launch over all systems; the
   // surrogate phase (features, every network's forward, applied forces, the
-  // audit sample and its grid-path launch) runs on the second stream beside
-  // the grouped force launch, touching only the surrogate lists' slots and
-  // the networks' and audits' buffers; the audit comparison after the join
+  // audit sample, its grid-path launch and the audit comparison) runs on the
+  // second stream beside the grouped force launch, touching only the
+  // surrogate lists' slots and the networks' and audits' buffers
   auto step = [&](int par) -> int {
     const hipStream_t cs = sims[0]->ctx->stream;
     const int prec = sims[0]->precision;
@@ -2182,14 +2189,12 @@ int pst_group_run(pst_sim** sims, int L, long long max_steps, int graph_steps, d
       if (e == hipSuccess) e = surr_group_features(dT, dfstart, n_surr, fstart[n_surr], aux);
       if (e == hipSuccess) e = launch_mlp_infer_group(dMI, dmstart, n_surr, mstart[n_surr], aux);
       if (e == hipSuccess) e = surr_group_apply(dT, dfstart, n_surr, fstart[n_surr], aux);
+      if (e == hipSuccess) e = surr_group_audit(dT, n_surr, aux);
       if (e == hipSuccess) e = hipEventRecord(join_ev, aux);
     }
     if (e == hipSuccess)
       e = launch_forces_group(dS, dW, dstart, L, n_wg, lds, prec, G, zp, cs, xcd_major, dorder);
-    if (n_surr > 0) {
-      if (e == hipSuccess) e = hipStreamWaitEvent(cs, join_ev, 0);
-      if (e == hipSuccess) e = surr_group_audit(dT, n_surr, cs);
-    }
+    if (n_surr > 0 && e == hipSuccess) e = hipStreamWaitEvent(cs, join_ev, 0);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
     e = prod_group_post(dP, dS, dB, dsstart, L, sstart[L], obs, par, cs);
     if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return 2; }
