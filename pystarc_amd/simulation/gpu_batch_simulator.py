@@ -866,8 +866,8 @@ class GPUBatchSimulator:
                     audit["terr2"] += a6[3]
                     audit["tref2"] += a6[4]
                     ctx.check(lib.pst_sim_reset_audit(h), "pst_sim_reset_audit")
-                    verdict = window.add(a6[0], a6[1], a6[2], a6[3], a6[4])
-                    err, terr = verdict if verdict is not None else (0.0, 0.0)
+                    res = window.add(a6[0], a6[1], a6[2], a6[3], a6[4])
+                    err, terr = res if res is not None else (0.0, 0.0)
                     audit["worst"] = max(audit["worst"], err)
                     audit["worst_torque"] = max(audit["worst_torque"], terr)
                     if err > self.audit_tol or terr > self.audit_tol:
@@ -1323,8 +1323,8 @@ class GPUBatchGroup:
                     a["ref2"] += c16[l, 13]
                     a["n"] += int(c16[l, 14])
                     ctx.check(lib.pst_sim_reset_audit(handles[l]), "pst_sim_reset_audit")
-                    verdict = windows[l].add(c16[l, 12], c16[l, 13], c16[l, 14])
-                    err = verdict[0] if verdict is not None else 0.0
+                    res = windows[l].add(c16[l, 12], c16[l, 13], c16[l, 14])
+                    err = res[0] if res is not None else 0.0
                     a["worst"] = max(a["worst"], err)
                     if err > s.audit_tol:
                         ctx.check(lib.pst_sim_set_surrogate(handles[l], None, None),
